@@ -1,0 +1,158 @@
+"""fastc_amd: MI355X block-texture encoders (BC7/BPTC, DXT1/DXT5, ETC1).
+
+Python host bindings over the C ABI in include/fasthip.h (libfasthip.so,
+built in-tree by fastc_amd/build.py).  The encoders run only as HIP kernels;
+importing this module fails loudly if the library is missing.
+
+The names mirror FasTC's API: `CompressImageData` (Core/src/TexComp.cpp:427),
+`SCompressionSettings` (Core/include/FasTC/TexComp.h:30-76) and the
+`ECompressionFormat` values (Base/include/FasTC/CompressionFormat.h:26-30).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfasthip.so")
+
+# FasTC::ECompressionFormat
+eCompressionFormat_DXT1 = 0
+eCompressionFormat_DXT5 = 1
+eCompressionFormat_ETC1 = 2
+eCompressionFormat_BPTC = 3
+
+BLOCK_BYTES = {0: 8, 1: 16, 2: 8, 3: 16}
+
+FH_OK = 0
+FH_FLAG_CALLER_WATERMARK = 1
+STATUS_NAMES = {0: "FH_OK", 1: "FH_ERR_INVALID_ARGUMENT", 2: "FH_ERR_NO_DATA",
+                3: "FH_ERR_BLOCK_DIMS", 4: "FH_ERR_UNKNOWN_FORMAT",
+                5: "FH_ERR_BUFFER_TOO_SMALL", 6: "FH_ERR_UNSUPPORTED",
+                7: "FH_ERR_NO_DEVICE", 8: "FH_ERR_HIP"}
+
+
+class FhJob(ctypes.Structure):
+    _fields_ = [("format", ctypes.c_uint32),
+                ("in_", ctypes.c_void_p),
+                ("out", ctypes.c_void_p),
+                ("out_size", ctypes.c_size_t),
+                ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("x_start", ctypes.c_uint32), ("y_start", ctypes.c_uint32),
+                ("x_end", ctypes.c_uint32), ("y_end", ctypes.c_uint32),
+                ("quality", ctypes.c_uint32),
+                ("block_modes", ctypes.c_uint32),
+                ("error_metric", ctypes.c_uint32),
+                ("seed", ctypes.c_uint64),
+                ("wm_counter", ctypes.POINTER(ctypes.c_uint32)),
+                ("flags", ctypes.c_uint32)]
+
+
+class FastHipError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libfasthip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing; run fastc_amd/build.py "
+                              "(or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.fh_compress.argtypes = [ctypes.POINTER(FhJob)]
+        L.fh_compress.restype = ctypes.c_int
+        L.fh_compress_device.argtypes = [ctypes.POINTER(FhJob), ctypes.c_void_p]
+        L.fh_compress_device.restype = ctypes.c_int
+        L.fh_decompress.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t,
+                                    ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
+        L.fh_decompress.restype = ctypes.c_int
+        L.fh_compressed_size.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                         ctypes.POINTER(ctypes.c_size_t)]
+        L.fh_compressed_size.restype = ctypes.c_int
+        L.fh_last_error.restype = ctypes.c_char_p
+        L.fh_device_available.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+EXPORTED_SYMBOLS = ["fh_compress", "fh_compress_device", "fh_decompress",
+                    "fh_compressed_size", "fh_last_error", "fh_device_available"]
+
+
+def _check(status):
+    if status != FH_OK:
+        raise FastHipError(status, lib().fh_last_error().decode(errors="replace"))
+
+
+def device_available():
+    return bool(lib().fh_device_available())
+
+
+def compressed_size(fmt, width, height):
+    n = ctypes.c_size_t(0)
+    _check(lib().fh_compressed_size(fmt, width, height, ctypes.byref(n)))
+    return n.value
+
+
+def _job(fmt, inp, out, out_size, width, height, rect, quality, seed, block_modes,
+         wm_counter):
+    x0, y0, x1, y1 = rect if rect is not None else (0, 0, width, height)
+    j = FhJob()
+    j.format = fmt
+    j.in_ = inp
+    j.out = out
+    j.out_size = out_size
+    j.width, j.height = width, height
+    j.x_start, j.y_start, j.x_end, j.y_end = x0, y0, x1, y1
+    j.quality = quality
+    j.block_modes = block_modes
+    j.error_metric = 0
+    j.seed = seed
+    if wm_counter is not None:
+        j.wm_counter = ctypes.pointer(wm_counter)
+        j.flags = FH_FLAG_CALLER_WATERMARK
+    return j
+
+
+def compress(img, fmt=eCompressionFormat_BPTC, quality=0, seed=0, block_modes=0xFF,
+             rect=None, out=None, wm_counter=None):
+    """Encode a (h, w) uint32 RGBA8 image (R in the low byte) on the GPU.
+
+    Mirrors CompressImageData: returns the compressed bytes (uint8 array of
+    CompressedImage::GetCompressedSize bytes).  `wm_counter` is a
+    ctypes.c_uint32 holding the BC7 solid-block watermark counter (the
+    reference's gWMVal); it is updated in place.  Without it the library's
+    process-wide counter is used, like the reference."""
+    img = np.ascontiguousarray(img, dtype=np.uint32)
+    h, w = img.shape
+    size = compressed_size(fmt, w, h)
+    if out is None:
+        out = np.zeros(size, dtype=np.uint8)
+    j = _job(fmt, img.ctypes.data, out.ctypes.data, out.nbytes, w, h, rect, quality, seed,
+             block_modes, wm_counter)
+    _check(lib().fh_compress(ctypes.byref(j)))
+    return out
+
+
+def compress_device(in_ptr, out_ptr, out_size, width, height, fmt=eCompressionFormat_BPTC,
+                    quality=0, seed=0, block_modes=0xFF, stream=0, wm_counter=None):
+    """Encode device-resident data (raw device pointers, e.g. torch
+    tensor.data_ptr()) asynchronously on `stream` (a hipStream_t as int)."""
+    j = _job(fmt, in_ptr, out_ptr, out_size, width, height, None, quality, seed, block_modes,
+             wm_counter)
+    _check(lib().fh_compress_device(ctypes.byref(j), ctypes.c_void_p(stream)))
+
+
+def decompress(data, width, height, fmt=eCompressionFormat_BPTC):
+    """Decode on the GPU to a (h, w) uint32 image (BPTCC::Decompress)."""
+    data = np.ascontiguousarray(data, dtype=np.uint8)
+    out = np.zeros((height, width), dtype=np.uint32)
+    _check(lib().fh_decompress(fmt, data.ctypes.data, data.nbytes, out.ctypes.data, width,
+                               height))
+    return out

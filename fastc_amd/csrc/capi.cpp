@@ -1,0 +1,342 @@
+// C-ABI entry points (include/fasthip.h).
+//
+// Argument validation follows CompressImageData (Core/src/TexComp.cpp:427-525)
+// and reports the same conditions as status codes; the message the reference
+// prints to stderr is available from fh_last_error().
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fasthip.h"
+#include "internal.h"
+
+namespace fh {
+namespace {
+
+thread_local std::string t_last_error;
+
+fh_status fail(fh_status s, const char *msg) {
+  t_last_error = msg;
+  return s;
+}
+
+fh_status fail_hip(hipError_t e, const char *what) {
+  char buf[256];
+  snprintf(buf, sizeof(buf), "%s: %s", what, hipGetErrorString(e));
+  t_last_error = buf;
+  return FH_ERR_HIP;
+}
+
+// Per-device state: a growable scratch pool, the process-wide watermark
+// counter (the reference's gWMVal, initially 0xFFFFFFFF) and the BC7
+// single-colour table.
+struct DeviceState {
+  std::mutex mu;
+  void *scratch = nullptr;
+  size_t scratch_size = 0;
+  uint32_t *counter = nullptr;  // device word
+  uint32_t *sctab = nullptr;
+  uint8_t *in_buf = nullptr;
+  size_t in_size = 0;
+  uint8_t *out_buf = nullptr;
+  size_t out_size = 0;
+  bool init = false;
+};
+
+std::mutex g_devices_mu;
+std::vector<DeviceState *> g_devices;
+
+DeviceState *device_state(int dev) {
+  std::lock_guard<std::mutex> g(g_devices_mu);
+  if ((int)g_devices.size() <= dev) g_devices.resize(dev + 1, nullptr);
+  if (!g_devices[dev]) g_devices[dev] = new DeviceState();
+  return g_devices[dev];
+}
+
+hipError_t ensure_init(DeviceState *st) {
+  if (st->init) return hipSuccess;
+  hipError_t e = hipMalloc(&st->counter, 2 * sizeof(uint32_t));
+  if (e != hipSuccess) return e;
+  const uint32_t c0[2] = {0xFFFFFFFFu, 0};
+  e = hipMemcpy(st->counter, c0, sizeof(c0), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return e;
+  std::vector<uint32_t> tab(kSingleColorEntries);
+  build_bc7_single_color_table(tab.data());
+  e = hipMalloc(&st->sctab, tab.size() * sizeof(uint32_t));
+  if (e != hipSuccess) return e;
+  e = hipMemcpy(st->sctab, tab.data(), tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return e;
+  st->init = true;
+  return hipSuccess;
+}
+
+hipError_t grow(void **p, size_t *cur, size_t need) {
+  if (*cur >= need) return hipSuccess;
+  if (*p) {
+    hipError_t e = hipFree(*p);
+    if (e != hipSuccess) return e;
+    *p = nullptr;
+    *cur = 0;
+  }
+  size_t sz = need + need / 4 + 4096;
+  hipError_t e = hipMalloc(p, sz);
+  if (e != hipSuccess) return e;
+  *cur = sz;
+  return hipSuccess;
+}
+
+uint32_t block_bytes(uint32_t format) {
+  switch (format) {
+    case FH_FORMAT_DXT1:
+    case FH_FORMAT_ETC1:
+      return 8;
+    case FH_FORMAT_DXT5:
+    case FH_FORMAT_BPTC:
+      return 16;
+    default:
+      return 0;
+  }
+}
+
+// The block range a (sub-)job covers in raster order, following the loop of
+// BPTCC::Compress / DXTC::CompressImageDXT1 (Compressor.cpp:1476-1482):
+// rows j = YStart .. min(YEnd, H-4) step 4, the first row from XStart, the
+// row j == YEnd only up to XEnd.  Coordinates must be block-aligned.
+fh_status job_range(const fh_job *j, uint32_t *first, uint32_t *count) {
+  const uint32_t bw = j->width / 4;
+  if ((j->x_start | j->y_start | j->x_end) & 3u)
+    return fail(FH_ERR_INVALID_ARGUMENT, "job rectangle must be block aligned");
+  if (j->y_end < j->height && (j->y_end & 3u))
+    return fail(FH_ERR_INVALID_ARGUMENT, "job rectangle must be block aligned");
+  if (j->x_start > j->width || j->x_end > j->width || j->y_start > j->height ||
+      j->y_end > j->height)
+    return fail(FH_ERR_INVALID_ARGUMENT, "job rectangle outside the image");
+  const uint32_t start = (j->y_start / 4) * bw + j->x_start / 4;
+  const uint32_t end_y = j->y_end < j->height - 4 ? j->y_end : j->height - 4;
+  if (j->y_start > end_y) {
+    *first = start;
+    *count = 0;
+    return FH_OK;
+  }
+  // last processed row
+  const uint32_t last_row = j->y_start + ((end_y - j->y_start) / 4) * 4;
+  uint32_t end;
+  if (last_row == j->y_end) {
+    end = (last_row / 4) * bw + (j->x_end + 3) / 4;
+    if (last_row == j->y_start && j->x_end <= j->x_start) end = start;
+  } else {
+    end = (last_row / 4 + 1) * bw;
+  }
+  *first = start;
+  *count = end > start ? end - start : 0;
+  return FH_OK;
+}
+
+fh_status validate(const fh_job *j, bool device) {
+  (void)device;
+  if (!j) return fail(FH_ERR_INVALID_ARGUMENT, "null job");
+  if (!j->in || (uint64_t)j->width * j->height == 0)
+    return fail(FH_ERR_NO_DATA, "No data sent to compress!");
+  if (block_bytes(j->format) == 0) return fail(FH_ERR_UNKNOWN_FORMAT, "Unknown compression format");
+  if (j->width % 4 || j->height % 4)
+    return fail(FH_ERR_BLOCK_DIMS,
+                "ERROR - CompressImageData: width or height is not multiple of block dimension");
+  const size_t need = (size_t)(j->width / 4) * (j->height / 4) * block_bytes(j->format);
+  if (!j->out || j->out_size < need)
+    return fail(FH_ERR_BUFFER_TOO_SMALL, "Not enough space for compressed data!");
+  if (j->format == FH_FORMAT_BPTC && j->error_metric != 0)
+    return fail(FH_ERR_UNSUPPORTED, "only the uniform error metric is implemented");
+  if ((j->flags & FH_FLAG_CALLER_WATERMARK) && !j->wm_counter)
+    return fail(FH_ERR_INVALID_ARGUMENT, "FH_FLAG_CALLER_WATERMARK needs wm_counter");
+  return FH_OK;
+}
+
+// Runs the encoder on device buffers.  Caller holds st->mu.
+fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8_t *dout,
+                     hipStream_t stream, uint32_t first, uint32_t count) {
+  BlockRange r;
+  r.img = reinterpret_cast<const uint32_t *>(din);
+  r.out = dout;
+  r.width = j->width;
+  r.blocks_wide = j->width / 4;
+  r.first_block = first;
+  r.num_blocks = count;
+  if (count == 0) return FH_OK;
+  hipError_t e = hipSuccess;
+  switch (j->format) {
+    case FH_FORMAT_BPTC: {
+      const uint32_t nchunks = (count + 255) / 256;
+      const size_t need = (size_t)count + 256 + (size_t)nchunks * 4 + 256;
+      e = grow(&st->scratch, &st->scratch_size, need);
+      if (e != hipSuccess) return fail_hip(e, "scratch allocation");
+      Bc7Scratch s;
+      s.local_prefix = reinterpret_cast<uint8_t *>(st->scratch);
+      s.chunk_prefix =
+          reinterpret_cast<uint32_t *>(reinterpret_cast<uint8_t *>(st->scratch) +
+                                       ((count + 255) & ~255u));
+      s.num_chunks = nchunks;
+      s.counter = st->counter;
+      uint32_t caller_c = 0;
+      if (j->flags & FH_FLAG_CALLER_WATERMARK) {
+        // caller's counter lives in host memory: stage it in counter[1]
+        s.counter = st->counter + 1;
+        caller_c = *j->wm_counter;
+        e = hipMemcpyAsync(s.counter, &caller_c, 4, hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) return fail_hip(e, "watermark upload");
+      }
+      Bc7Params p;
+      p.quality = j->quality;
+      p.block_modes = j->block_modes ? (j->block_modes & 0xFFu) : 0xFFu;
+      p.seed = j->seed;
+      p.block_id_base = 0;
+      e = bc7_encode(r, p, s, stream);
+      if (e != hipSuccess) return fail_hip(e, "BC7 encode launch");
+      if (j->flags & FH_FLAG_CALLER_WATERMARK) {
+        e = hipMemcpyAsync(&caller_c, s.counter, 4, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return fail_hip(e, "watermark download");
+        *j->wm_counter = caller_c;
+      }
+      return FH_OK;
+    }
+    case FH_FORMAT_DXT1:
+    case FH_FORMAT_DXT5:
+      e = dxt_encode(r, j->format == FH_FORMAT_DXT5, stream);
+      if (e != hipSuccess) return fail_hip(e, "DXT encode launch");
+      return FH_OK;
+    case FH_FORMAT_ETC1:
+      e = etc1_encode(r, stream);
+      if (e != hipSuccess) return fail_hip(e, "ETC1 encode launch");
+      return FH_OK;
+  }
+  return fail(FH_ERR_UNKNOWN_FORMAT, "Unknown compression format");
+}
+
+int current_device(fh_status *s) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    *s = fail(FH_ERR_NO_DEVICE, "no HIP device available");
+    return -1;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    *s = fail(FH_ERR_NO_DEVICE, "hipGetDevice failed");
+    return -1;
+  }
+  *s = FH_OK;
+  return dev;
+}
+
+}  // namespace
+
+const uint32_t *bc7_single_color_table(int device) {
+  DeviceState *st = device_state(device);
+  return st->sctab;
+}
+
+}  // namespace fh
+
+using namespace fh;
+
+extern "C" {
+
+fh_status fh_compress(const fh_job *j) {
+  fh_status s = validate(j, false);
+  if (s != FH_OK) return s;
+  uint32_t first = 0, count = 0;
+  s = job_range(j, &first, &count);
+  if (s != FH_OK) return s;
+  const int dev = current_device(&s);
+  if (dev < 0) return s;
+  DeviceState *st = device_state(dev);
+  std::lock_guard<std::mutex> g(st->mu);
+  hipError_t e = ensure_init(st);
+  if (e != hipSuccess) return fail_hip(e, "device initialisation");
+  const size_t in_bytes = (size_t)j->width * j->height * 4;
+  const uint32_t bb = block_bytes(j->format);
+  const size_t out_bytes = (size_t)(j->width / 4) * (j->height / 4) * bb;
+  e = grow(reinterpret_cast<void **>(&st->in_buf), &st->in_size, in_bytes);
+  if (e == hipSuccess)
+    e = grow(reinterpret_cast<void **>(&st->out_buf), &st->out_size, out_bytes);
+  if (e != hipSuccess) return fail_hip(e, "device buffer allocation");
+  hipStream_t stream = nullptr;
+  e = hipMemcpyAsync(st->in_buf, j->in, in_bytes, hipMemcpyHostToDevice, stream);
+  if (e != hipSuccess) return fail_hip(e, "input upload");
+  s = run_device(st, j, st->in_buf, st->out_buf, stream, first, count);
+  if (s != FH_OK) return s;
+  if (count) {
+    e = hipMemcpyAsync(j->out + (size_t)first * bb, st->out_buf + (size_t)first * bb,
+                       (size_t)count * bb, hipMemcpyDeviceToHost, stream);
+    if (e != hipSuccess) return fail_hip(e, "output download");
+  }
+  e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) return fail_hip(e, "encode");
+  return FH_OK;
+}
+
+fh_status fh_compress_device(const fh_job *j, void *hip_stream) {
+  fh_status s = validate(j, true);
+  if (s != FH_OK) return s;
+  uint32_t first = 0, count = 0;
+  s = job_range(j, &first, &count);
+  if (s != FH_OK) return s;
+  const int dev = current_device(&s);
+  if (dev < 0) return s;
+  DeviceState *st = device_state(dev);
+  std::lock_guard<std::mutex> g(st->mu);
+  hipError_t e = ensure_init(st);
+  if (e != hipSuccess) return fail_hip(e, "device initialisation");
+  return run_device(st, j, j->in, j->out, reinterpret_cast<hipStream_t>(hip_stream), first,
+                    count);
+}
+
+fh_status fh_decompress(uint32_t format, const uint8_t *in, size_t in_size, uint8_t *out,
+                        uint32_t width, uint32_t height) {
+  if (!in || !out || width == 0 || height == 0)
+    return fail(FH_ERR_INVALID_ARGUMENT, "null buffer or empty image");
+  if (format != FH_FORMAT_BPTC) return fail(FH_ERR_UNSUPPORTED, "decode implemented for BPTC only");
+  const size_t need = (size_t)((width + 3) / 4) * ((height + 3) / 4) * 16;
+  if (in_size < need) return fail(FH_ERR_BUFFER_TOO_SMALL, "compressed buffer too small");
+  fh_status s;
+  const int dev = current_device(&s);
+  if (dev < 0) return s;
+  DeviceState *st = device_state(dev);
+  std::lock_guard<std::mutex> g(st->mu);
+  hipError_t e = ensure_init(st);
+  if (e != hipSuccess) return fail_hip(e, "device initialisation");
+  const size_t out_bytes = (size_t)width * height * 4;
+  e = grow(reinterpret_cast<void **>(&st->in_buf), &st->in_size, need);
+  if (e == hipSuccess)
+    e = grow(reinterpret_cast<void **>(&st->out_buf), &st->out_size, out_bytes);
+  if (e != hipSuccess) return fail_hip(e, "device buffer allocation");
+  e = hipMemcpy(st->in_buf, in, need, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail_hip(e, "input upload");
+  e = bc7_decode(st->in_buf, reinterpret_cast<uint32_t *>(st->out_buf), width, height, nullptr);
+  if (e != hipSuccess) return fail_hip(e, "decode launch");
+  e = hipMemcpy(out, st->out_buf, out_bytes, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return fail_hip(e, "output download");
+  return FH_OK;
+}
+
+fh_status fh_compressed_size(uint32_t format, uint32_t width, uint32_t height, size_t *size) {
+  if (!size) return fail(FH_ERR_INVALID_ARGUMENT, "null size");
+  const uint32_t bb = block_bytes(format);
+  if (!bb) return fail(FH_ERR_UNKNOWN_FORMAT, "Unknown compression format");
+  *size = (size_t)((width + 3) / 4) * ((height + 3) / 4) * bb;
+  return FH_OK;
+}
+
+const char *fh_last_error(void) { return t_last_error.c_str(); }
+
+int fh_device_available(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n > 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+/*
+ * fasthip: MI355X (gfx950) block-texture encoders behind a C ABI.
+ *
+ * Drop-in boundary for FasTC's per-block encode path.  Each entry point
+ * replaces one reference interface:
+ *
+ *   fh_compress          CompressImageData (Core/include/FasTC/TexComp.h:81-88,
+ *                        Core/src/TexComp.cpp:427-525) and the per-format
+ *                        CompressionFunc it selects (Core/src/CompressionFuncs.h:28):
+ *                        BPTCC::Compress (BPTCEncoder/include/FasTC/BPTCCompressor.h:166),
+ *                        DXTC::CompressImageDXT1/DXT5 (DXTEncoder/include/FasTC/DXTCompressor.h:24-25),
+ *                        ETCC::Compress_RG (ETCEncoder/include/FasTC/ETCCompressor.h:35).
+ *                        The fh_job fields are FasTC::CompressionJob's
+ *                        (Base/include/FasTC/CompressionJob.h:40-140) plus the
+ *                        BPTCC::CompressionSettings fields (BPTCCompressor.h:123-158).
+ *   fh_compress_device   the same on device-resident buffers, asynchronous on
+ *                        a caller stream (no host copies; used by benchmarks).
+ *   fh_decompress        BPTCC::Decompress (BPTCCompressor.h:216),
+ *                        DXTC::DecompressDXT1/5, ETCC::Decompress
+ *                        (via CompressedImage::DecompressImage,
+ *                        Core/src/CompressedImage.cpp:86-118).
+ *   fh_compressed_size   CompressedImage::GetCompressedSize
+ *                        (Core/src/CompressedImage.cpp:136-149).
+ *   fh_last_error        the stderr message CompressImageData prints
+ *                        (Core/src/TexComp.cpp:157-159) before returning false.
+ *   fh_device_available  no reference counterpart (the reference is CPU only);
+ *                        lets callers choose this backend.
+ *
+ * All functions are thread-safe.  Buffers are owned by the caller; no pointer
+ * is retained after a call returns.
+ */
+#ifndef FASTHIP_H_
+#define FASTHIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  FH_ERR_NO_DATA, FH_ERR_BLOCK_DIMS, FH_ERR_UNKNOWN_FORMAT and
+ * FH_ERR_BUFFER_TOO_SMALL are the failures CompressImageData reports
+ * (Core/src/TexComp.cpp:454-496). */
+typedef enum fh_status {
+  FH_OK = 0,
+  FH_ERR_INVALID_ARGUMENT = 1,
+  FH_ERR_NO_DATA = 2,
+  FH_ERR_BLOCK_DIMS = 3,
+  FH_ERR_UNKNOWN_FORMAT = 4,
+  FH_ERR_BUFFER_TOO_SMALL = 5,
+  FH_ERR_UNSUPPORTED = 6,
+  FH_ERR_NO_DEVICE = 7,
+  FH_ERR_HIP = 8
+} fh_status;
+
+/* Values of FasTC::ECompressionFormat (Base/include/FasTC/CompressionFormat.h:26-30). */
+enum {
+  FH_FORMAT_DXT1 = 0,
+  FH_FORMAT_DXT5 = 1,
+  FH_FORMAT_ETC1 = 2,
+  FH_FORMAT_BPTC = 3
+};
+
+/* fh_job.flags */
+enum {
+  /* Keep the solid-block watermark counter in *wm_counter (caller-owned)
+   * instead of the library's process-wide counter. */
+  FH_FLAG_CALLER_WATERMARK = 1u << 0
+};
+
+/* One compression job: FasTC::CompressionJob plus BPTC settings. */
+typedef struct fh_job {
+  uint32_t format;        /* FH_FORMAT_* */
+  const uint8_t *in;      /* width*height*4 bytes, RGBA8, R first */
+  uint8_t *out;           /* the whole image's compressed buffer */
+  size_t out_size;        /* bytes available at out */
+  uint32_t width, height; /* pixels, multiples of 4 */
+  /* Sub-rectangle in block-raster order (CompressionJob XStart..YEnd).
+   * A full image is x_start=y_start=0, x_end=width, y_end=height. */
+  uint32_t x_start, y_start, x_end, y_end;
+  /* BPTC only (BPTCC::CompressionSettings) */
+  uint32_t quality;       /* simulated-annealing steps; the reference sets it from iQuality */
+  uint32_t block_modes;   /* mask of BC7 modes (bit m = mode m); 0 means all */
+  uint32_t error_metric;  /* 0 = uniform (the only metric the reference's tc uses) */
+  uint64_t seed;          /* annealing random stream key */
+  uint32_t *wm_counter;   /* with FH_FLAG_CALLER_WATERMARK: in/out watermark counter */
+  uint32_t flags;
+} fh_job;
+
+fh_status fh_compress(const fh_job *job);
+fh_status fh_compress_device(const fh_job *job, void *hip_stream);
+
+fh_status fh_decompress(uint32_t format, const uint8_t *in, size_t in_size, uint8_t *out,
+                        uint32_t width, uint32_t height);
+
+fh_status fh_compressed_size(uint32_t format, uint32_t width, uint32_t height, size_t *size);
+
+const char *fh_last_error(void);
+int fh_device_available(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTHIP_H_ */

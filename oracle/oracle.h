@@ -1,0 +1,106 @@
+/*
+ * CPU oracle for the per-4x4-block encode search of FasTC.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library restates, in plain C, the
+ * reference's CPU algorithms (BPTCEncoder/src/Compressor.cpp,
+ * RGBAEndpoints.{h,cpp}, Decompressor.cpp, Base/src/Image.cpp) so that the
+ * tests can check the HIP kernels against it.  Only tests/, the smoke() entry
+ * and the benchmark's CPU-baseline leg may load it; the product library never
+ * links or calls it.
+ *
+ * Every exported function cites the reference lines it follows.  Functions
+ * whose results are not pinned by a golden vector, a closed form, an
+ * invariant or a brute-force check are marked "parity unpinned" in their
+ * comment.
+ */
+#ifndef FASTC_ORACLE_H_
+#define FASTC_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Random-number source for simulated annealing.
+ *  ORA_RNG_LCG     : the reference's process-global LCG
+ *                    (Compressor.cpp:358-362), state in *lcg_state, which the
+ *                    caller seeds (the reference seeds it with time(NULL)).
+ *  ORA_RNG_COUNTER : a counter-based stream keyed by (seed, block, fit); the
+ *                    same definition is used by the HIP kernels, so annealed
+ *                    output is bit-comparable between CPU and GPU. */
+enum { ORA_RNG_LCG = 0, ORA_RNG_COUNTER = 1 };
+
+typedef struct {
+  uint32_t quality;       /* m_NumSimulatedAnnealingSteps (BPTCCompressor.h:149) */
+  uint32_t block_modes;   /* m_BlockModes mask (BPTCCompressor.h:139) */
+  int      rng_kind;      /* ORA_RNG_* */
+  uint32_t *lcg_state;    /* ORA_RNG_LCG: the shared g_seed */
+  uint64_t seed;          /* ORA_RNG_COUNTER: job seed */
+} ora_bc7_settings;
+
+/* Per-block statistics (mode chosen, path) for tests. */
+typedef struct {
+  int path;          /* 0 solid, 1 transparent, 2 searched */
+  int mode;          /* mode written */
+  double best_error; /* internal error of the chosen mode (searched path) */
+  double mode_error[8]; /* min internal error per mode, -1 if not tried */
+} ora_bc7_block_stats;
+
+/* Compress one 4x4 block (16 RGBA8 pixels, R in the low byte).
+ * Follows CompressBC7Block (Compressor.cpp:1819-1860).
+ * wm_index: the watermark table entry for a solid block (the reference's
+ * gWMVal after its pre-increment, Compressor.cpp:1457).
+ * block_id: global block index (keys the counter RNG). */
+void ora_bc7_compress_block(const uint32_t px[16], uint8_t out[16],
+                            const ora_bc7_settings *s, uint32_t wm_index,
+                            uint64_t block_id, ora_bc7_block_stats *stats);
+
+/* BPTCC::Compress over a (sub-)job (Compressor.cpp:1473-1520).
+ * in: width*height RGBA8; out: base of the whole-image BC7 buffer.
+ * *wm_counter is the reference's gWMVal (starts at 0xFFFFFFFF in a fresh
+ * process); it is advanced once per solid block in processing order. */
+void ora_bc7_compress_job(const uint8_t *in, uint8_t *out, uint32_t width,
+                          uint32_t height, uint32_t x_start, uint32_t y_start,
+                          uint32_t x_end, uint32_t y_end,
+                          const ora_bc7_settings *s, uint32_t *wm_counter);
+
+/* BPTCC::Decompress (Decompressor.cpp:345-367), including the reference's
+ * treatment of mode 4 with index mode 1 (Decompressor.cpp:263-295 interpolate
+ * with the per-mode bit counts, not the swapped ones). */
+void ora_bc7_decompress(const uint8_t *in, uint8_t *out, uint32_t width,
+                        uint32_t height);
+
+/* Spec-correct BC7 decode of one block (BC7 format specification); differs
+ * from ora_bc7_decompress only for mode 4 with index mode 1. */
+void ora_bc7_decode_block_spec(const uint8_t blk[16], uint32_t px[16]);
+
+/* Image::ComputePSNR (Base/src/Image.cpp:205-255): premultiplied-alpha RGB. */
+double ora_psnr(const uint8_t *a, const uint8_t *b, uint32_t width,
+                uint32_t height);
+
+/* Sum over pixels and RGB channels of the squared premultiplied difference,
+ * the numerator of ComputePSNR's mse (Image.cpp:244-248). */
+double ora_premul_sse(const uint8_t *a, const uint8_t *b, uint32_t npix);
+
+/* Building blocks exported for the function-level tests. */
+uint8_t ora_quantize_channel(uint8_t val, uint8_t mask, int pbit);
+uint32_t ora_to_pixel(const float v[4], uint32_t mask, int pbit);
+float ora_length(const float v[4]);
+int ora_power_method(const float m[16], float eig[4], float *eigval);
+double ora_quantized_error(const uint32_t px[16], const uint8_t *sel, int n,
+                           const float p1[4], const float p2[4], int nbuckets,
+                           uint32_t mask, const int *pbits, uint8_t *indices);
+void ora_bc7_box_selection(const uint32_t px[16], uint32_t *nshapes,
+                           uint32_t *shape2, uint32_t *shape3,
+                           uint32_t *modes);
+const uint8_t *ora_table_mode5(void);   /* [256][2] 7-bit endpoint pairs */
+const uint8_t *ora_table_dxt6(void);    /* [256][2][3] */
+uint32_t ora_counter_rng(uint32_t key, uint32_t n);
+uint32_t ora_fit_key(uint64_t seed, uint64_t block_id, uint32_t fit_tag);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
