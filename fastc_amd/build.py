@@ -49,6 +49,10 @@ def build(verbose=False, jobs=8):
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs
     _run(cmd, verbose)
     os.replace(LIB + ".tmp", LIB)
+    synth = os.path.join(HERE, "libsynth.so")
+    _run(["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-o", synth + ".tmp",
+          os.path.join(CSRC, "synth.c")], verbose)
+    os.replace(synth + ".tmp", synth)
     return LIB
 
 

@@ -1,0 +1,88 @@
+"""Regenerate the golden fixtures in tests/golden/ from the reference.
+
+Run in a container that has the FasTC tree at /root/reference and the built
+oracle/_ref library (make -C oracle/ref):
+
+    python tests/golden/make_golden.py
+
+Fixtures written (data only, no reference source text):
+  bc7_tables.json  SHA-256 of the reference's two BC7 lookup tables
+                   (BPTCEncoder/src/BCLookupTables.h: Optimal7CompressBC7Mode5,
+                   Optimal6CompressDXT1) as flat uint8 arrays.
+  dxt_etc.npz      DXT1 / DXT5 / ETC1 output of the reference encoders
+                   (stb_dxt.h, rg_etc1.cpp built from the reference tree) for a
+                   64x64 generator image (seed 2, hard edges).
+  dxt_etc_hashes.json  SHA-256 of the same encoders' output for the 256x256
+                   and 2048x2048 seed-2 images (the DXT workload's full size).
+"""
+import ctypes
+import hashlib
+import json
+import os
+import re
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+REF = os.environ.get("FASTC_REF", "/root/reference")
+
+
+def parse_table(text, name):
+    i = text.index(name)
+    body = text[text.index("=", i):text.index("};", i)]
+    return np.array([int(x, 0) for x in re.findall(r"0x[0-9a-fA-F]+|\d+", body)], dtype=np.uint8)
+
+
+def ref_lib():
+    path = os.path.join(ROOT, "oracle", "_ref", "libfastc_ref.so")
+    L = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    L.ref_dxt_compress.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, vp]
+    L.ref_etc1_compress.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, vp]
+    L.ref_etc1_decompress.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, vp]
+    return L
+
+
+def encode_all(L, img):
+    h, w = img.shape
+    n = (h // 4) * (w // 4)
+    d1 = np.zeros(n * 8, np.uint8)
+    d5 = np.zeros(n * 16, np.uint8)
+    e1 = np.zeros(n * 8, np.uint8)
+    L.ref_dxt_compress(img.ctypes.data, w, h, 0, d1.ctypes.data)
+    L.ref_dxt_compress(img.ctypes.data, w, h, 1, d5.ctypes.data)
+    L.ref_etc1_compress(img.ctypes.data, w, h, e1.ctypes.data)
+    return d1, d5, e1
+
+
+def main():
+    from fastc_amd import synth
+    text = open(os.path.join(REF, "BPTCEncoder", "src", "BCLookupTables.h")).read()
+    m5 = parse_table(text, "Optimal7CompressBC7Mode5[256][2]")
+    d6 = parse_table(text, "Optimal6CompressDXT1[256][2][3]")
+    assert m5.size == 512 and d6.size == 1536
+    with open(os.path.join(HERE, "bc7_tables.json"), "w") as f:
+        json.dump({"mode5_sha256": hashlib.sha256(m5.tobytes()).hexdigest(),
+                   "dxt6_sha256": hashlib.sha256(d6.tobytes()).hexdigest()}, f, indent=1)
+
+    L = ref_lib()
+    img = synth.generate(64, 64, 2, edges=True)
+    d1, d5, e1 = encode_all(L, img)
+    np.savez_compressed(os.path.join(HERE, "dxt_etc.npz"), image=img, dxt1=d1, dxt5=d5, etc1=e1)
+    hashes = {}
+    for size in (256, 2048):
+        img = synth.generate(size, size, 2, edges=True)
+        d1, d5, e1 = encode_all(L, img)
+        hashes[str(size)] = {"dxt1": hashlib.sha256(d1.tobytes()).hexdigest(),
+                             "dxt5": hashlib.sha256(d5.tobytes()).hexdigest(),
+                             "etc1": hashlib.sha256(e1.tobytes()).hexdigest()}
+    with open(os.path.join(HERE, "dxt_etc_hashes.json"), "w") as f:
+        json.dump(hashes, f, indent=1)
+    print("fixtures written")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,80 @@
+"""C-ABI boundary checks that need no GPU: the library loads, exports every
+function include/fasthip.h declares, and reports CompressImageData's
+validation failures (Core/src/TexComp.cpp:440-496) as status codes."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import fastc_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "fasthip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_ ]+\*?\s*(fh_[a-z_]+)\(", text, re.M)))
+
+
+def test_library_exports_header_symbols():
+    L = fastc_amd.lib()
+    names = declared_functions()
+    assert set(names) >= {"fh_compress", "fh_compress_device", "fh_decompress",
+                          "fh_compressed_size", "fh_last_error", "fh_device_available"}
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(fastc_amd.EXPORTED_SYMBOLS) <= set(names)
+
+
+@pytest.mark.parametrize("fmt,w,h,want", [(0, 256, 256, 256 * 256 // 2),
+                                          (1, 256, 256, 256 * 256),
+                                          (2, 64, 32, 64 * 32 // 2),
+                                          (3, 4096, 4096, 4096 * 4096),
+                                          (3, 6, 6, 4 * 16)])
+def test_compressed_size(fmt, w, h, want):
+    """CompressedImage::GetCompressedSize (Core/src/CompressedImage.cpp:136-149)."""
+    assert fastc_amd.compressed_size(fmt, w, h) == want
+
+
+def _status(img, fmt=3, out_size=None, w=None, h=None):
+    L = fastc_amd.lib()
+    h0, w0 = img.shape
+    w = w0 if w is None else w
+    h = h0 if h is None else h
+    size = fastc_amd.compressed_size(fmt, w, h) if fmt in (0, 1, 2, 3) else 16
+    out = np.zeros(max(size, 16), np.uint8)
+    j = fastc_amd._job(fmt, img.ctypes.data, out.ctypes.data,
+                       size if out_size is None else out_size, w, h, None, 0, 0, 0xFF, None)
+    return L.fh_compress(ctypes.byref(j))
+
+
+def test_validation_errors_match_reference_checks():
+    img = np.zeros((8, 8), np.uint32)
+    assert _status(img, w=6) == 3          # not a multiple of the block size
+    assert _status(img, fmt=9) == 4        # unknown format
+    assert _status(img, out_size=10) == 5  # "Not enough space for compressed data!"
+    assert b"space" in fastc_amd.lib().fh_last_error()
+    L = fastc_amd.lib()
+    j = fastc_amd._job(3, None, img.ctypes.data, 1024, 8, 8, None, 0, 0, 0xFF, None)
+    assert L.fh_compress(ctypes.byref(j)) == 2  # "No data sent to compress!"
+
+
+def test_misaligned_rectangle_rejected():
+    img = np.zeros((8, 8), np.uint32)
+    out = np.zeros(64, np.uint8)
+    j = fastc_amd._job(3, img.ctypes.data, out.ctypes.data, 64, 8, 8, (2, 0, 8, 8), 0, 0,
+                       0xFF, None)
+    assert fastc_amd.lib().fh_compress(ctypes.byref(j)) == 1
+
+
+def test_no_device_is_reported_not_emulated():
+    """Without a GPU a valid job fails with FH_ERR_NO_DEVICE: there is no CPU
+    fallback inside the library."""
+    if fastc_amd.device_available():
+        pytest.skip("a HIP device is present")
+    img = np.zeros((8, 8), np.uint32)
+    assert _status(img) == 7
+    with pytest.raises(fastc_amd.FastHipError):
+        fastc_amd.compress(img)
