@@ -40,7 +40,21 @@ def parse():
     ap.add_argument("--cpu-tile", type=int, default=512,
                     help="side of the image tile the CPU baseline encodes")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a device-to-device copy of 512 MiB (3 times), the "
+                         "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE")
     return ap.parse_args()
+
+
+def copy_calibration():
+    import torch
+    n = 512 << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": n, "launches": 3}))
 
 
 def cpu_baseline(img, quality, tile, seed):
@@ -88,6 +102,9 @@ def cpu_baseline(img, quality, tile, seed):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build targets one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
