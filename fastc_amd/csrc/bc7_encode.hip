@@ -5,19 +5,26 @@
 // annealing (Compressor.cpp:538-630) with a counter-based random stream at
 // quality > 0.
 //
-// Work mapping: one wavefront (64 lanes) owns four 4x4 blocks.
-//   phase 1  the four blocks' 64 pixels are loaded (one pixel per lane, rows of
-//            16 consecutive pixels, so each quarter-wave reads 64 contiguous
-//            bytes) into LDS.
-//   phase 2  BoxSelection (Compressor.cpp:1670-1750) block by block: lane i
-//            scores two-subset shape i and three-subset shape i; a wave
-//            argmin with lowest-index ties picks the shapes.
-//   phase 3  lane (block = lane/16, slot = lane%16) runs one subset fit of
-//            CompressClusters (Compressor.cpp:1752-1817): CompressCluster,
-//            k-means, least squares, grid clamp and the annealing loop.
-//   phase 4  one lane per block sums the subset errors per mode in the
-//            reference's mode order, keeps the first strict minimum and
-//            writes the 128-bit block (CompressionMode::Pack, :1096-1298).
+// The search runs as a pipeline of kernels, each shaped so that the lanes of
+// a wavefront execute the same code:
+//   select  one wave per four blocks: classify (solid / transparent / search)
+//           and run BoxSelection (Compressor.cpp:1670-1750) with lane i scoring
+//           two- and three-subset shape i; a wave argmin with lowest-index ties
+//           picks the shapes.  Each block's candidates (one per mode and shape
+//           entry of CompressClusters, :1752-1817, with the rotation modes split
+//           per rotation and index mode) are appended to per-mode queues.
+//   setup   persistent, one lane per candidate, queue by queue, so a wave runs
+//           one mode: principal axis, k-means, least squares and grid clamp for
+//           every subset (CompressCluster, :921-1094), and the alpha fit of the
+//           rotation modes (:632-919).
+//   anneal  persistent, one lane per candidate: each iteration advances every
+//           still-annealing subset of the candidate by one step and scores all
+//           16 pixels in one pass, each against its subset's endpoints.  Lanes
+//           that finish refill from a wave-local window of the work list.
+//   pack    one lane per block: sum the subset errors per mode in the
+//           reference's mode order, keep the first strict minimum and write the
+//           128-bit block (CompressionMode::Pack, :1096-1298); solid blocks get
+//           the watermark entry from an exclusive scan of solid-block flags.
 //
 // Uniform error metric only: all errors are then exact integers (< 2^24), so
 // they are kept in uint32 and equal the reference's float/double sums.
@@ -530,77 +537,38 @@ __device__ __forceinline__ void pick_neighbor(const Fit &f, Rng &rng, uint32_t p
   }
 }
 
-// OptimizeEndpointsForCluster (Compressor.cpp:538-630).  p1/p2 come in as
-// floats (grid-clamped) and leave as the best integer endpoints.
-__device__ uint32_t optimize(const Fit &f, uint32_t q, Rng &rng, const uint32_t px[16],
-                             uint32_t smask, int ptmode, const float fp1[4], const float fp2[4],
-                             uint32_t &bp1o, uint32_t &bp2o, uint64_t &best_idx, int &best_pbc) {
-  const int nb = 1 << f.nbits_idx;
-  // QuantizedError always receives a p-bit combo here (GetPBitCombo never
-  // returns NULL), so modes without p-bits quantise with p-bit 0.
-  uint32_t q1 = to_pixel_f(fp1, f.qmask, pbc_bit(f.pbit_type, best_pbc, 0));
-  uint32_t q2 = to_pixel_f(fp2, f.qmask, pbc_bit(f.pbit_type, best_pbc, 1));
-  uint32_t cur = qerr(px, smask, ptmode, q1, q2, nb, best_idx);
-  int cur_pbc = best_pbc;
-  uint32_t best = cur;
-  uint32_t p1, p2;
-  if (f.pbit_type != PB_NONE) {
-    p1 = q1;
-    p2 = q2;
-  } else {
-    p1 = to_pixel_f(fp1, f.qmask, -1);
-    p2 = to_pixel_f(fp2, f.qmask, -1);
-  }
-  uint32_t bp1 = p1, bp2 = p2, vp1 = p1, vp2 = p2;
-  int vpbc = cur_pbc;
-  const int max_energy = (int)q;
-  for (int energy = 0; best > 0 && energy < max_energy; energy++) {
-    const float temp = (float)energy / (float)(max_energy - 1);
-    uint32_t np1 = 0, np2 = 0;
-    int npbc = 0;
-    pick_neighbor(f, rng, p1, p2, cur_pbc, np1, np2, npbc, vp1, vp2, vpbc);
-    const uint32_t e1 = to_pixel_int(np1, f.qmask, pbc_bit(f.pbit_type, npbc, 0));
-    const uint32_t e2 = to_pixel_int(np2, f.qmask, pbc_bit(f.pbit_type, npbc, 1));
-    uint64_t idx;
-    const uint32_t err = qerr(px, smask, ptmode, e1, e2, nb, idx);
-    // AcceptNewEndpointError (Compressor.cpp:524-536)
-    bool accept = err < cur;
-    if (!accept) {
-      const double pr = exp(((double)0.1f * ((double)cur - (double)err)) / (double)temp);
-      const double r = (double)rng.frand();
-      accept = r < pr;
-    }
-    if (accept) {
-      cur = err;
-      p1 = np1;
-      p2 = np2;
-      cur_pbc = npbc;
-    }
-    if (err < best) {
-      best_idx = idx;
-      bp1 = np1;
-      bp2 = np2;
-      best_pbc = npbc;
-      best = err;
-      vp1 = np1;
-      vp2 = np2;
-      vpbc = npbc;
-      energy = 0;
-    }
-  }
-  bp1o = bp1;
-  bp2o = bp2;
-  return best;
+// ---------------------------------------------------------------------------
+// Per-subset fit set-up: CompressCluster up to the annealing loop.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void set_fit(Fit &f, int mode, int rot, int im) {
+  const ModeAttr a = kModes[mode];
+  f.mode = mode;
+  f.rot = rot;
+  f.idx_mode = im;
+  f.nbits_idx = im == 0 ? a.idx_bits : a.alpha_idx_bits;
+  f.color_prec = a.color_prec;
+  f.alpha_prec = a.alpha_prec;
+  f.qmask = qmask_of(a.color_prec, a.alpha_prec);
+  f.pbit_type = a.pbit_type;
+  f.npbc = a.pbit_type == PB_SHARED ? 2 : (a.pbit_type == PB_NOTSHARED ? 4 : 1);
 }
 
-// CompressCluster, RGB variant (Compressor.cpp:921-1094).  avg/mn/mx are the
-// cluster statistics the reference uses (stale ones for the rotation copy).
-// Returns the error; endpoints are integer-valued on return.
+// Where a subset fit stands once CompressCluster reaches
+// OptimizeEndpointsForCluster (Compressor.cpp:550-580), or its final result
+// when no annealing follows (single colour, quality 0, zero error).
+struct SubStart {
+  uint32_t err;  // QuantizedError of the start endpoints (or the final error)
+  uint32_t p1, p2;  // endpoint state (integers): annealing start, or final endpoints
+  uint64_t idx;     // indices at pixel positions of this subset
+  int pbc;
+  bool anneal;
+};
+
 template <int NB>
-__device__ uint32_t kmeans_lsq_fit(const Fit &f, uint32_t q, Rng &rng, const uint32_t *sctab,
-                                   const uint32_t px[16], uint32_t smask, int ptmode, int n,
-                                   const float avg[4], uint32_t &p1o, uint32_t &p2o,
-                                   uint64_t &idx, int &pbc) {
+__device__ void kmeans_setup(const Fit &f, uint32_t q, const uint32_t *sctab,
+                             const uint32_t px[16], uint32_t smask, int ptmode, int n,
+                             const float avg[4], SubStart &out) {
   float axis[4];
   principal_axis(px, smask, ptmode, n, avg, axis);
   float mindp = FLT_MAX, maxdp = -FLT_MAX;
@@ -624,6 +592,7 @@ __device__ uint32_t kmeans_lsq_fit(const Fit &f, uint32_t q, Rng &rng, const uin
     p1[k] = p1[k] < 0.0f ? 0.0f : (p1[k] > 255.0f ? 255.0f : p1[k]);
     p2[k] = p2[k] < 0.0f ? 0.0f : (p2[k] > 255.0f ? 255.0f : p2[k]);
   }
+  // Lloyd k-means from NB evenly spaced seeds (Compressor.cpp:961-1026)
   float c[NB][4];
 #pragma unroll
   for (int i = 0; i < NB; i++) {
@@ -690,6 +659,10 @@ __device__ uint32_t kmeans_lsq_fit(const Fit &f, uint32_t q, Rng &rng, const uin
       nfilled++;
       last = j;
     }
+  uint64_t ones = 0;
+#pragma unroll
+  for (int i = 0; i < 16; i++)
+    if ((smask >> i) & 1) ones |= (uint64_t)1 << (4 * i);
   if (nfilled == 1) {
     float cl[4];
 #pragma unroll
@@ -698,13 +671,14 @@ __device__ uint32_t kmeans_lsq_fit(const Fit &f, uint32_t q, Rng &rng, const uin
 #pragma unroll
         for (int k = 0; k < 4; k++) cl[k] = c[j][k];
       }
-    const uint32_t e = single_color(f, sctab, to_pixel_f(cl, 0xFFFFFFFFu, -1), p1o, p2o, pbc);
-    idx = 0;
-#pragma unroll
-    for (int i = 0; i < 16; i++)
-      if ((smask >> i) & 1) idx |= (uint64_t)1 << (4 * i);
-    return (uint32_t)n * e;
+    const uint32_t e =
+        single_color(f, sctab, to_pixel_f(cl, 0xFFFFFFFFu, -1), out.p1, out.p2, out.pbc);
+    out.err = (uint32_t)n * e;
+    out.idx = ones;
+    out.anneal = false;
+    return;
   }
+  // least squares (Compressor.cpp:1053-1077)
   float asq = 0.0f, bsq = 0.0f, ab = 0.0f;
   float ax[4] = {0.0f, 0.0f, 0.0f, 0.0f}, bx[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -736,10 +710,11 @@ __device__ uint32_t kmeans_lsq_fit(const Fit &f, uint32_t q, Rng &rng, const uin
     p1[k] = p1[k] < 0.0f ? 0.0f : (p1[k] > 255.0f ? 255.0f : p1[k]);
     p2[k] = p2[k] < 0.0f ? 0.0f : (p2[k] > 255.0f ? 255.0f : p2[k]);
   }
+  int pbc = 0;
+  uint32_t g1 = 0xFFFFFFFFu, g2 = 0xFFFFFFFFu;
   {
     const bool has_pb = f.npbc > 1;
     float mind = FLT_MAX;
-    float g1[4] = {-1.0f, -1.0f, -1.0f, -1.0f}, g2[4] = {-1.0f, -1.0f, -1.0f, -1.0f};
     for (int i = 0; i < f.npbc; i++) {
       const uint32_t q1 = to_pixel_f(p1, f.qmask, has_pb ? pbc_bit(f.pbit_type, i, 0) : -1);
       const uint32_t q2 = to_pixel_f(p2, f.qmask, has_pb ? pbc_bit(f.pbit_type, i, 1) : -1);
@@ -754,285 +729,60 @@ __device__ uint32_t kmeans_lsq_fit(const Fit &f, uint32_t q, Rng &rng, const uin
       const float dist = dot4(d1, d1) + dot4(d2, d2);
       if (dist < mind) {
         mind = dist;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          g1[k] = n1[k];
-          g2[k] = n2[k];
-        }
+        g1 = q1;
+        g2 = q2;
         pbc = i;
       }
     }
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      p1[k] = g1[k];
-      p2[k] = g2[k];
-    }
   }
-  return optimize(f, q, rng, px, smask, ptmode, p1, p2, p1o, p2o, idx, pbc);
+  // OptimizeEndpointsForCluster's start (Compressor.cpp:550-580): the error
+  // is measured with the combo's p-bits even for modes without p-bits (it
+  // always receives a combo), the state keeps the grid endpoints.
+  const int nb = 1 << f.nbits_idx;
+  const uint32_t q1 = to_pixel_int(g1, f.qmask, pbc_bit(f.pbit_type, pbc, 0));
+  const uint32_t q2 = to_pixel_int(g2, f.qmask, pbc_bit(f.pbit_type, pbc, 1));
+  out.err = qerr(px, smask, ptmode, q1, q2, nb, out.idx);
+  out.pbc = pbc;
+  if (f.pbit_type != PB_NONE) {
+    out.p1 = q1;
+    out.p2 = q2;
+  } else {
+    out.p1 = to_pixel_int(g1, f.qmask, -1);
+    out.p2 = to_pixel_int(g2, f.qmask, -1);
+  }
+  out.anneal = q > 0 && out.err > 0;
 }
 
-__device__ uint32_t fit_rgb(const Fit &f, uint32_t q, Rng &rng, const uint32_t *sctab,
-                            const uint32_t px[16], uint32_t smask, int ptmode, int n,
-                            const float avg[4], bool all_same, uint32_t &p1, uint32_t &p2,
-                            uint64_t &idx, int &pbc) {
+// CompressCluster, RGB variant (Compressor.cpp:921-1094) up to the annealing.
+__device__ void subset_setup(const Fit &f, uint32_t q, const uint32_t *sctab,
+                             const uint32_t px[16], uint32_t smask, int ptmode, int n,
+                             const float avg[4], bool all_same, SubStart &out) {
   if (all_same) {
-    // GetPoint(0): the first point of the subset
     const int first = __ffs(smask) - 1;
     uint32_t pt = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++)
       if (i == first) pt = point_of(px[i], ptmode);
-    const uint32_t e = single_color(f, sctab, pt, p1, p2, pbc);
-    idx = 0;
+    const uint32_t e = single_color(f, sctab, pt, out.p1, out.p2, out.pbc);
+    out.err = (uint32_t)n * e;
+    out.idx = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++)
-      if ((smask >> i) & 1) idx |= (uint64_t)1 << (4 * i);
-    return (uint32_t)n * e;
+      if ((smask >> i) & 1) out.idx |= (uint64_t)1 << (4 * i);
+    out.anneal = false;
+    return;
   }
-  if (f.nbits_idx == 2)
-    return kmeans_lsq_fit<4>(f, q, rng, sctab, px, smask, ptmode, n, avg, p1, p2, idx, pbc);
-  if (f.nbits_idx == 3)
-    return kmeans_lsq_fit<8>(f, q, rng, sctab, px, smask, ptmode, n, avg, p1, p2, idx, pbc);
-  return kmeans_lsq_fit<16>(f, q, rng, sctab, px, smask, ptmode, n, avg, p1, p2, idx, pbc);
+  if (f.nbits_idx == 2) kmeans_setup<4>(f, q, sctab, px, smask, ptmode, n, avg, out);
+  else if (f.nbits_idx == 3) kmeans_setup<8>(f, q, sctab, px, smask, ptmode, n, avg, out);
+  else kmeans_setup<16>(f, q, sctab, px, smask, ptmode, n, avg, out);
 }
 
-// Result of one slot, kept in LDS for the pack phase.
-struct FitRec {
-  uint32_t err;   // 0xFFFFFFFF: slot unused
-  uint32_t px1, px2;
-  uint32_t cidx_lo, cidx_hi;  // color indices, 4 bits per pixel
-  uint32_t aidx_lo, aidx_hi;  // alpha indices (rotation modes), 4 bits per pixel
-  uint32_t meta;              // pbc | rot << 8 | idx_mode << 16
-};
-
-struct BlockSel {
-  int cls;     // 0 solid, 1 transparent, 2 searched
-  int layout;  // 0: modes 0/1/2/3/6/7 layout, 1: alpha layout (modes 4/5/6/7)
-  uint32_t modes;
-  int nshapes;
-  int shape2, shape3;
-};
-
-// Slot -> fit.  Layout 0: 0-2 mode 0, 3-5 mode 2, 6-7 mode 1, 8-9 mode 3,
-// 10-11 mode 7, 12 mode 6 (shape entry 0), 13 mode 6 (shape entry 1).
-// Layout 1: 0-7 mode 4 (rot, idx mode), 8-11 mode 5 (rot), 12 mode 6,
-// 13-14 mode 7.
-__device__ __forceinline__ bool slot_fit(const BlockSel &s, int slot, uint32_t q, int &mode,
-                                         int &entry, int &subset, int &rot, int &im) {
-  rot = 0;
-  im = 0;
-  subset = 0;
-  entry = 0;
-  if (s.layout == 0) {
-    if (slot < 6) {
-      mode = slot < 3 ? 0 : 2;
-      subset = slot % 3;
-      entry = 1;
-      if (s.nshapes < 2) return false;
-      if (mode == 0 && s.shape3 >= 16) return false;
-    } else if (slot < 12) {
-      mode = slot < 8 ? 1 : (slot < 10 ? 3 : 7);
-      subset = slot & 1;
-      entry = 0;
-    } else if (slot < 14) {
-      mode = 6;
-      entry = slot - 12;
-      if (entry >= s.nshapes) return false;
-      // Both shape entries fit the same 16-pixel cluster; without annealing
-      // the second fit repeats the first exactly and cannot win (strict <).
-      if (entry == 1 && q == 0) return false;
-    } else {
-      return false;
-    }
-  } else {
-    if (slot < 8) {
-      mode = 4;
-      rot = slot >> 1;
-      im = slot & 1;
-    } else if (slot < 12) {
-      mode = 5;
-      rot = slot - 8;
-    } else if (slot == 12) {
-      mode = 6;
-    } else if (slot < 15) {
-      mode = 7;
-      subset = slot - 13;
-    } else {
-      return false;
-    }
-  }
-  return (s.modes >> mode) & 1;
-}
-
-// 128-bit LSB-first writer (BitStream::WriteBits, BitStream.h:65-91).
-struct BitWriter {
-  uint64_t lo = 0, hi = 0;
-  int pos = 0;
-  __device__ __forceinline__ void put(uint32_t v, int n) {
-    if (n <= 0) return;
-    const uint64_t vv = (uint64_t)(n >= 32 ? v : (v & ((1u << n) - 1u)));
-    if (pos >= 64) {
-      hi |= vv << (pos - 64);
-    } else {
-      lo |= vv << pos;
-      if (pos + n > 64) hi |= vv >> (64 - pos);
-    }
-    pos += n;
-  }
-};
-
-__device__ __forceinline__ uint32_t idx_at(uint32_t lo, uint32_t hi, int i) {
-  return i < 8 ? (lo >> (4 * i)) & 0xFu : (hi >> (4 * (i - 8))) & 0xFu;
-}
-
-__device__ void write_block(uint8_t *dst, const BitWriter &w) {
-  uint4 v;
-  v.x = (uint32_t)w.lo;
-  v.y = (uint32_t)(w.lo >> 32);
-  v.z = (uint32_t)w.hi;
-  v.w = (uint32_t)(w.hi >> 32);
-  *reinterpret_cast<uint4 *>(dst) = v;
-}
-
-// CompressOptimalColorBC7 (Compressor.cpp:1424-1458).
-__device__ void pack_solid(uint32_t pixel, uint32_t wm, BitWriter &w) {
-  w.put(1u << 5, 6);
-  w.put(0, 2);
-#pragma unroll
-  for (int ch = 0; ch < 3; ch++) {
-    const uint32_t v = chan(pixel, ch);
-    w.put(kMode5Tab[v][0], 7);
-    w.put(kMode5Tab[v][1], 7);
-  }
-  w.put(pixel >> 24, 8);
-  w.put(pixel >> 24, 8);
-  w.put(0xaaaaaaabu, 31);
-  w.put(kWM[wm], 31);
-}
-
-// CompressionMode::Pack (Compressor.cpp:1096-1298) from per-subset records.
-// The p-bits are written unswapped after an anchor fix-up swap, as the
-// reference does (:1149-1170 vs :1234-1241).
-__device__ void pack_mode(int mode, int shape, const FitRec *rec, BitWriter &w) {
-  const ModeAttr a = kModes[mode];
-  const int ns = a.subsets;
-  const int rot = (rec[0].meta >> 8) & 0xFF;
-  const int im = (rec[0].meta >> 16) & 0xFF;
-  w.put(1u << mode, mode + 1);
-  w.put((uint32_t)shape, a.part_bits);
-  if (a.has_rot) w.put((uint32_t)rot, 2);
-  if (a.has_idx_mode) w.put((uint32_t)im, 1);
-  uint32_t px1[3], px2[3];
-  uint32_t cl = 0, ch = 0;  // color indices at pixel positions
-  uint32_t al = rec[0].aidx_lo, ah = rec[0].aidx_hi;
-  const int nib = (im == 0) ? a.idx_bits : a.alpha_idx_bits;
-  const int nab = (im == 0) ? a.alpha_idx_bits : a.idx_bits;
-  for (int s = 0; s < ns; s++) {
-    px1[s] = rec[s].px1;
-    px2[s] = rec[s].px2;
-    cl |= rec[s].cidx_lo;
-    ch |= rec[s].cidx_hi;
-  }
-  for (int s = 0; s < ns; s++) {
-    const uint32_t smask = subset_mask(shape, ns, s);
-    const int anchor = anchor_of(s, shape, ns);
-    if (idx_at(cl, ch, anchor) >> (nib - 1)) {
-      const uint32_t t = px1[s];
-      px1[s] = px2[s];
-      px2[s] = t;
-      const uint32_t nv = (1u << nib) - 1;
-      // invert this subset's color indices (all 16 entries in the reference,
-      // but the others belong to other subsets and are never written)
-      uint32_t nl = 0, nh = 0;
-      for (int i = 0; i < 16; i++) {
-        uint32_t v = idx_at(cl, ch, i);
-        if ((smask >> i) & 1) v = nv - v;
-        if (i < 8) nl |= v << (4 * i);
-        else nh |= v << (4 * (i - 8));
-      }
-      cl = nl;
-      ch = nh;
-      if (a.has_rot) {
-        const uint32_t nav = (1u << nab) - 1;
-        uint32_t ml = 0, mh = 0;
-        for (int i = 0; i < 16; i++) {
-          const uint32_t v = nav - idx_at(al, ah, i);
-          if (i < 8) ml |= v << (4 * i);
-          else mh |= v << (4 * (i - 8));
-        }
-        al = ml;
-        ah = mh;
-      }
-    }
-    if (a.has_rot && (idx_at(al, ah, anchor) >> (nab - 1)) > 0) {
-      const uint32_t a1 = px1[s] >> 24, a2 = px2[s] >> 24;
-      px1[s] = (px1[s] & 0x00FFFFFFu) | (a2 << 24);
-      px2[s] = (px2[s] & 0x00FFFFFFu) | (a1 << 24);
-      const uint32_t nav = (1u << nab) - 1;
-      uint32_t ml = 0, mh = 0;
-      for (int i = 0; i < 16; i++) {
-        const uint32_t v = nav - idx_at(al, ah, i);
-        if (i < 8) ml |= v << (4 * i);
-        else mh |= v << (4 * (i - 8));
-      }
-      al = ml;
-      ah = mh;
-    }
-  }
-  const int cp = a.color_prec, ap = a.alpha_prec;
-  for (int c = 0; c < 3; c++)
-    for (int s = 0; s < ns; s++) {
-      w.put(chan(px1[s], c) >> (8 - cp), cp);
-      w.put(chan(px2[s], c) >> (8 - cp), cp);
-    }
-  if (ap) {
-    for (int s = 0; s < ns; s++) {
-      w.put(chan(px1[s], 3) >> (8 - ap), ap);
-      w.put(chan(px2[s], 3) >> (8 - ap), ap);
-    }
-  }
-  if (a.pbit_type != PB_NONE) {
-    for (int s = 0; s < ns; s++) {
-      const int pbc = rec[s].meta & 0xFF;
-      w.put((uint32_t)pbc_bit(a.pbit_type, pbc, 0), 1);
-      if (a.pbit_type != PB_SHARED) w.put((uint32_t)pbc_bit(a.pbit_type, pbc, 1), 1);
-    }
-  }
-  if (a.has_idx_mode && im == 1) {
-    for (int i = 0; i < 16; i++) w.put(idx_at(al, ah, i), i == 0 ? 1 : 2);
-    for (int i = 0; i < 16; i++) w.put(idx_at(cl, ch, i), i == 0 ? 2 : 3);
-  } else {
-    for (int i = 0; i < 16; i++) {
-      int s = 0;
-      for (int t = 1; t < ns; t++)
-        if ((subset_mask(shape, ns, t) >> i) & 1) s = t;
-      const int anchor = anchor_of(s, shape, ns);
-      w.put(idx_at(cl, ch, i), i == anchor ? nib - 1 : nib);
-    }
-    if (a.has_rot)
-      for (int i = 0; i < 16; i++) w.put(idx_at(al, ah, i), i == 0 ? nab - 1 : nab);
-  }
-}
-
-__device__ __forceinline__ void set_fit(Fit &f, int mode, int rot, int im) {
-  const ModeAttr a = kModes[mode];
-  f.mode = mode;
-  f.rot = rot;
-  f.idx_mode = im;
-  f.nbits_idx = im == 0 ? a.idx_bits : a.alpha_idx_bits;
-  f.color_prec = a.color_prec;
-  f.alpha_prec = a.alpha_prec;
-  f.qmask = qmask_of(a.color_prec, a.alpha_prec);
-  f.pbit_type = a.pbit_type;
-  f.npbc = a.pbit_type == PB_SHARED ? 2 : (a.pbit_type == PB_NOTSHARED ? 4 : 1);
-}
-
-// The alpha/rotation variant of CompressCluster (Compressor.cpp:632-919) for a
-// full 16-pixel cluster.  stats of the unrotated cluster feed the RGB fit.
-__device__ uint32_t fit_alpha(const Fit &f, uint32_t q, Rng &rng, const uint32_t *sctab,
-                              const uint32_t px[16], const float avg[4], uint32_t &out_px1,
-                              uint32_t &out_px2, uint64_t &cidx, uint64_t &aidx) {
-  const int ptmode = f.rot + 1;
+// The alpha channel of the rotation modes (Compressor.cpp:697-911): a 1-D
+// k-means and least-squares fit (or the table lookup for a constant alpha).
+// Returns the alpha error; a1q/a2q are the alpha bytes Pack will write
+// (rounded, then quantised to the mode's alpha precision).
+__device__ uint32_t alpha_fit(const Fit &f, const uint32_t px[16], uint64_t &aidx, uint32_t &a1q,
+                              uint32_t &a2q) {
   float avals[16];
   float amin = FLT_MAX, amax = -FLT_MAX;
 #pragma unroll
@@ -1042,13 +792,6 @@ __device__ uint32_t fit_alpha(const Fit &f, uint32_t q, Rng &rng, const uint32_t
     amin = avals[i] < amin ? avals[i] : amin;
     amax = amax < avals[i] ? avals[i] : amax;
   }
-  uint32_t rp1, rp2;
-  int dummy_pbc = 0;
-  // the source cluster is not a single point here (solid blocks never reach
-  // the search), so the stale AllSamePoint test is false
-  const uint32_t rgb_err =
-      fit_rgb(f, q, rng, sctab, px, 0xFFFFu, ptmode, 16, avg, false, rp1, rp2, cidx, dummy_pbc);
-
   const int abits = f.idx_mode == 0 ? kModes[f.mode].alpha_idx_bits : kModes[f.mode].idx_bits;
   const int nb = 1 << abits;
   float a1 = amin, a2 = amax;
@@ -1056,9 +799,7 @@ __device__ uint32_t fit_alpha(const Fit &f, uint32_t q, Rng &rng, const uint32_t
   aidx = 0;
   if (a1 == a2) {
     const uint32_t a1be = (uint32_t)a1, a2be = (uint32_t)a2;
-    if (f.mode == 5) {
-      alpha_err = 0;
-    } else {
+    if (f.mode != 5) {
       if (kDxt6Tab[a1be][0][0]) {
         a1 = (float)((kDxt6Tab[a1be][1][1] << 2) | (kDxt6Tab[a1be][0][1] >> 4));
         a2 = (float)((kDxt6Tab[a2be][1][2] << 2) | (kDxt6Tab[a2be][0][1] >> 4));
@@ -1112,9 +853,7 @@ __device__ uint32_t fit_alpha(const Fit &f, uint32_t q, Rng &rng, const uint32_t
           }
         if (c > 0.0f) s /= c;
         npts[j] = c;
-        if (j < nb) {
-          fixed = fixed && (s == vals[j]);
-        }
+        if (j < nb) fixed = fixed && (s == vals[j]);
         vals[j] = s;
       }
 #pragma unroll
@@ -1157,6 +896,7 @@ __device__ uint32_t fit_alpha(const Fit &f, uint32_t q, Rng &rng, const uint32_t
       t = 0.0f < a2 ? a2 : 0.0f;
       a2 = t < 255.0f ? t : 255.0f;
     }
+    // QuantizeChannel(uint8(a)) truncates here, while Pack rounds (:876-879)
     const uint32_t amask = (uint32_t)((int8_t)-0x7F >> (f.alpha_prec - 1)) & 0xFFu;
     const uint32_t a1b = quant_channel((uint32_t)a1 & 0xFFu, amask, -1);
     const uint32_t a2b = quant_channel((uint32_t)a2 & 0xFFu, amask, -1);
@@ -1181,41 +921,176 @@ __device__ uint32_t fit_alpha(const Fit &f, uint32_t q, Rng &rng, const uint32_t
       aidx |= (uint64_t)bb << (4 * i);
     }
   }
-  // Pack quantises (rgb of the RGB fit, fitted alpha) with rounding and no p-bit.
-  float e1[4], e2[4];
-  unpack4f(rp1, e1);
-  unpack4f(rp2, e2);
-  e1[3] = a1;
-  e2[3] = a2;
-  out_px1 = to_pixel_f(e1, f.qmask, -1);
-  out_px2 = to_pixel_f(e2, f.qmask, -1);
-  return rgb_err + alpha_err;
+  const uint32_t am = chan(f.qmask, 3);
+  a1q = quant_channel(round_half_up(a1) & 0xFFu, am, -1);
+  a2q = quant_channel(round_half_up(a2) & 0xFFu, am, -1);
+  return alpha_err;
 }
 
-__global__ void __launch_bounds__(64) encode_kernel(BlockRange r, Bc7Params prm,
-                                                    Bc7Scratch scr, const uint32_t *sctab) {
-  __shared__ uint32_t s_px[4][16];
-  __shared__ BlockSel s_sel[4];
-  __shared__ FitRec s_rec[4][16];
-  const int lane = threadIdx.x;
-  const uint32_t base = r.first_block + blockIdx.x * 4;
+// ---------------------------------------------------------------------------
+// Work items.
+// ---------------------------------------------------------------------------
 
-  // phase 1: rows of 16 pixels over 4 consecutive blocks
+// Candidates of a block: one per (mode, shape entry) of CompressClusters
+// (Compressor.cpp:1771-1809), with the rotation modes split per
+// (rotation, index mode).  Layout 0 (all but non-opaque blocks):
+//   0 mode 0, 1 mode 2, 2 mode 1, 3 mode 3, 4 mode 7, 5 mode 6 (shape entry 0),
+//   6 mode 6 (shape entry 1).
+// Layout 1 (non-opaque): 0-7 mode 4 (rotation = c/2, index mode = c%2),
+//   8-11 mode 5 (rotation), 12 mode 6, 13 mode 7.
+// Candidate types (work queues, sorted so that a wave runs one code path):
+//   0 mode 0, 1 mode 2, 2 mode 1, 3 mode 3, 4 mode 7, 5 mode 6, 6 mode 4, 7 mode 5.
+constexpr int kNumTypes = 8;
+__constant__ int kTypeCap[kNumTypes] = {1, 1, 1, 1, 1, 2, 8, 4};
+__constant__ int kTypeOfMode[8] = {0, 2, 1, 3, 6, 7, 5, 4};
+
+// Block selection, packed: cls 2 | layout 1 | nshapes 2 | shape2 6 | shape3 6 | modes 8.
+__device__ __forceinline__ uint32_t sel_pack(int cls, int layout, int nshapes, int s2, int s3,
+                                             uint32_t modes) {
+  return (uint32_t)cls | ((uint32_t)layout << 2) | ((uint32_t)nshapes << 3) |
+         ((uint32_t)s2 << 5) | ((uint32_t)s3 << 11) | (modes << 17);
+}
+__device__ __forceinline__ int sel_cls(uint32_t s) { return s & 3; }
+__device__ __forceinline__ int sel_layout(uint32_t s) { return (s >> 2) & 1; }
+__device__ __forceinline__ int sel_nshapes(uint32_t s) { return (s >> 3) & 3; }
+__device__ __forceinline__ int sel_shape2(uint32_t s) { return (s >> 5) & 63; }
+__device__ __forceinline__ int sel_shape3(uint32_t s) { return (s >> 11) & 63; }
+__device__ __forceinline__ uint32_t sel_modes(uint32_t s) { return (s >> 17) & 0xFF; }
+
+struct Cand {
+  int mode, entry, rot, im, shape, nsub;
+};
+
+__device__ __forceinline__ bool cand_of(uint32_t sel, int c, uint32_t q, Cand &k) {
+  k.rot = 0;
+  k.im = 0;
+  k.entry = 0;
+  k.nsub = 1;
+  k.shape = 0;
+  const uint32_t modes = sel_modes(sel);
+  if (sel_layout(sel) == 0) {
+    switch (c) {
+      case 0: k.mode = 0; break;
+      case 1: k.mode = 2; break;
+      case 2: k.mode = 1; break;
+      case 3: k.mode = 3; break;
+      case 4: k.mode = 7; break;
+      case 5: k.mode = 6; break;
+      case 6: k.mode = 6; k.entry = 1; break;
+      default: return false;
+    }
+    if (k.mode == 0 || k.mode == 2) {
+      if (sel_nshapes(sel) < 2) return false;
+      k.entry = 1;
+      k.nsub = 3;
+      k.shape = sel_shape3(sel);
+      if (k.mode == 0 && k.shape >= 16) return false;
+    } else if (k.mode != 6) {
+      k.nsub = 2;
+      k.shape = sel_shape2(sel);
+    } else {
+      k.shape = k.entry ? sel_shape3(sel) : sel_shape2(sel);
+      if (k.entry >= sel_nshapes(sel)) return false;
+      // both mode-6 entries fit the same 16 pixels; without annealing the
+      // second repeats the first exactly and cannot win (strict <)
+      if (k.entry == 1 && q == 0) return false;
+    }
+  } else {
+    if (c < 8) {
+      k.mode = 4;
+      k.rot = c >> 1;
+      k.im = c & 1;
+    } else if (c < 12) {
+      k.mode = 5;
+      k.rot = c - 8;
+    } else if (c == 12) {
+      k.mode = 6;
+      k.shape = sel_shape2(sel);
+    } else if (c == 13) {
+      k.mode = 7;
+      k.nsub = 2;
+      k.shape = sel_shape2(sel);
+    } else {
+      return false;
+    }
+  }
+  return (modes >> k.mode) & 1;
+}
+
+// Result of one candidate, read by the pack kernel.
+struct CandRec {
+  uint32_t err;       // total error; 0xFFFFFFFF = not a candidate
+  uint32_t px1[3], px2[3];
+  uint32_t cidx_lo, cidx_hi;
+  uint32_t aidx_lo, aidx_hi;
+  uint32_t meta;      // pbc per subset (2 bits each) | rot << 8 | im << 10
+};
+
+// Annealing work item (start state from the set-up kernel).
+struct AState {
+  uint32_t id;        // relative block << 4 | candidate
+  uint32_t info;      // mode | rot << 3 | im << 5 | nsub << 6 | shape << 8 | active << 14 | entry << 17
+  uint32_t p1[3], p2[3];
+  uint32_t err[3];
+  uint32_t pbc;       // 2 bits per subset
+  uint32_t idx_lo, idx_hi;
+  uint32_t extra_err; // alpha error of rotation candidates
+  uint32_t pad;
+};
+
+struct Work {
+  uint32_t *sel;          // per block
+  CandRec *rec;           // per block x 16
+  uint32_t *queue;        // per-type candidate queues
+  uint32_t *counters;     // [0..7] queue sizes, [8..15] set-up fetch, [16] anneal size, [17] anneal fetch
+  AState *astate;
+  uint32_t qoff[kNumTypes];
+  uint32_t astate_cap;
+};
+
+__device__ __forceinline__ void load_block(const BlockRange &r, uint32_t rel, uint32_t px[16]) {
+  const uint32_t b = r.first_block + rel;
+  const uint32_t bx = b % r.blocks_wide, by = b / r.blocks_wide;
+  const uint32_t *p = r.img + (size_t)(by * 4) * r.width + bx * 4;
+#pragma unroll
+  for (int y = 0; y < 4; y++) {
+    const uint4 row = *reinterpret_cast<const uint4 *>(p + (size_t)y * r.width);
+    px[y * 4 + 0] = row.x;
+    px[y * 4 + 1] = row.y;
+    px[y * 4 + 2] = row.z;
+    px[y * 4 + 3] = row.w;
+  }
+}
+
+__device__ __forceinline__ uint32_t cand_tag(int mode, int entry, int subset, int rot, int im) {
+  return (uint32_t)mode | ((uint32_t)entry << 3) | ((uint32_t)subset << 5) | ((uint32_t)rot << 7) |
+         ((uint32_t)im << 9);
+}
+
+// ---------------------------------------------------------------------------
+// Kernel 1: classification and BoxSelection (Compressor.cpp:1670-1750,
+// 1819-1860); one wave per four blocks, lane i scores shape i.  Emits the
+// candidates into per-type queues (wave-aggregated appends).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) select_kernel(BlockRange r, Bc7Params prm, Work w) {
+  __shared__ uint32_t s_px[4][16];
+  __shared__ uint32_t s_sel[4];
+  const int lane = threadIdx.x;
+  const uint32_t base = blockIdx.x * 4;  // relative block index
   {
     const int row = lane >> 4, x = lane & 15;
     const int blk = x >> 2;
-    const uint32_t b = base + blk;
-    if (b < r.first_block + r.num_blocks) {
+    const uint32_t rel = base + blk;
+    if (rel < r.num_blocks) {
+      const uint32_t b = r.first_block + rel;
       const uint32_t bx = b % r.blocks_wide, by = b / r.blocks_wide;
       s_px[blk][row * 4 + (x & 3)] = r.img[(size_t)(by * 4 + row) * r.width + bx * 4 + (x & 3)];
     }
   }
   __syncthreads();
-
-  // phase 2: classify and select shapes, one block at a time
   for (int blk = 0; blk < 4; blk++) {
-    const uint32_t b = base + blk;
-    if (b >= r.first_block + r.num_blocks) break;  // uniform
+    const uint32_t rel = base + blk;
+    if (rel >= r.num_blocks) break;  // uniform
     uint32_t px[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) px[i] = s_px[blk][i];
@@ -1226,21 +1101,18 @@ __global__ void __launch_bounds__(64) encode_kernel(BlockRange r, Bc7Params prm,
       transparent = transparent && (px[i] >> 24) == 0;
       opaque = opaque && (px[i] >> 24) >= 250;
     }
-    BlockSel sel;
-    sel.cls = solid ? 0 : (transparent ? 1 : 2);
-    sel.layout = 0;
-    sel.modes = 0;
-    sel.nshapes = 1;
-    sel.shape2 = sel.shape3 = 0;
-    if (sel.cls == 2) {
-      // EstimateTwoClusterError / EstimateThreeClusterError per shape
+    uint32_t sel;
+    if (solid) {
+      sel = sel_pack(0, 0, 1, 0, 0, 0);
+    } else if (transparent) {
+      sel = sel_pack(1, 0, 1, 0, 0, 0);
+    } else {
       double err2 = 0.0, err3 = 0.0;
       for (int ns = 2; ns <= 3; ns++) {
         if (ns == 3 && !opaque) break;
         double err = 0.0;
         for (int s = 0; s < ns; s++) {
           const uint32_t m = subset_mask(lane, ns, s);
-          uint32_t mn = 0xFFFFFFFFu, mx = 0;
           uint32_t mnc[4] = {255, 255, 255, 255}, mxc[4] = {0, 0, 0, 0};
 #pragma unroll
           for (int i = 0; i < 16; i++) {
@@ -1252,12 +1124,12 @@ __global__ void __launch_bounds__(64) encode_kernel(BlockRange r, Bc7Params prm,
               mxc[k] = c > mxc[k] ? c : mxc[k];
             }
           }
-          mn = mnc[0] | (mnc[1] << 8) | (mnc[2] << 16) | (mnc[3] << 24);
-          mx = mxc[0] | (mxc[1] << 8) | (mxc[2] << 16) | (mxc[3] << 24);
+          const uint32_t mn = mnc[0] | (mnc[1] << 8) | (mnc[2] << 16) | (mnc[3] << 24);
+          const uint32_t mx = mxc[0] | (mxc[1] << 8) | (mxc[2] << 16) | (mxc[3] << 24);
+          // EstimateTwo/ThreeClusterError (Compressor.cpp:1626-1655)
           if (mn != mx) {
             uint64_t dummy;
-            const uint32_t qe = qerr(px, m, 0, mn, mx, ns == 2 ? 8 : 4, dummy);
-            err += 0.0001 + (double)qe;
+            err += 0.0001 + (double)qerr(px, m, 0, mn, mx, ns == 2 ? 8 : 4, dummy);
           } else {
             err += 0.0;
           }
@@ -1265,11 +1137,8 @@ __global__ void __launch_bounds__(64) encode_kernel(BlockRange r, Bc7Params prm,
         if (ns == 2) err2 = err;
         else err3 = err;
       }
-      // wave argmin, lowest index on ties
-      double v2 = err2;
-      int i2 = lane;
-      double v3 = err3;
-      int i3 = lane;
+      double v2 = err2, v3 = err3;
+      int i2 = lane, i3 = lane;
 #pragma unroll
       for (int off = 32; off >= 1; off >>= 1) {
         const double o2 = __shfl_xor(v2, off);
@@ -1285,71 +1154,121 @@ __global__ void __launch_bounds__(64) encode_kernel(BlockRange r, Bc7Params prm,
           i3 = oi3;
         }
       }
-      sel.shape2 = i2;
+      uint32_t modes;
+      int layout = 0, nshapes = 1, s3 = 0;
       if (v2 < 1e-9) {
-        sel.modes = (1u << 1) | (1u << 3) | (1u << 7);
-        sel.nshapes = 1;
-        sel.layout = 0;
+        modes = (1u << 1) | (1u << 3) | (1u << 7);
       } else if (!opaque) {
-        sel.modes = (1u << 4) | (1u << 5) | (1u << 6) | (1u << 7);
-        sel.nshapes = 1;
-        sel.layout = 1;
+        modes = (1u << 4) | (1u << 5) | (1u << 6) | (1u << 7);
+        layout = 1;
       } else {
-        sel.modes = 0xFFu & ~((1u << 4) | (1u << 5));
-        sel.nshapes = 2;
-        sel.shape3 = i3;
-        sel.layout = 0;
-        if (v3 < 1e-9) sel.modes = (1u << 0) | (1u << 2);
+        modes = 0xFFu & ~((1u << 4) | (1u << 5));
+        nshapes = 2;
+        s3 = i3;
+        if (v3 < 1e-9) modes = (1u << 0) | (1u << 2);
       }
-      sel.modes &= prm.block_modes;
+      sel = sel_pack(2, layout, nshapes, i2, s3, modes & prm.block_modes);
     }
-    if (lane == 0) s_sel[blk] = sel;
+    if (lane == 0) {
+      s_sel[blk] = sel;
+      w.sel[rel] = sel;
+    }
   }
   __syncthreads();
+  // enqueue: lane = (block, candidate)
+  const int blk = lane >> 4, c = lane & 15;
+  const uint32_t rel = base + blk;
+  bool valid = false;
+  int type = 0;
+  if (rel < r.num_blocks) {
+    const uint32_t sel = s_sel[blk];
+    Cand k;
+    if (sel_cls(sel) == 2 && cand_of(sel, c, prm.quality, k)) {
+      valid = true;
+      type = kTypeOfMode[k.mode];
+    }
+    CandRec *rc = &w.rec[(size_t)rel * 16 + c];
+    rc->err = 0xFFFFFFFFu;
+  }
+  const uint64_t lt = (1ull << lane) - 1ull;
+  for (int t = 0; t < kNumTypes; t++) {
+    const uint64_t m = __ballot(valid && type == t);
+    if (!m) continue;
+    uint32_t b0 = 0;
+    if (lane == (int)(__ffsll((unsigned long long)m) - 1)) b0 = atomicAdd(&w.counters[t], (uint32_t)__popcll(m));
+    b0 = __shfl(b0, (int)(__ffsll((unsigned long long)m) - 1));
+    if (valid && type == t) w.queue[w.qoff[t] + b0 + __popcll(m & lt)] = (rel << 4) | (uint32_t)c;
+  }
+}
 
-  // phase 3: one subset fit per lane
-  {
-    const int blk = lane >> 4, slot = lane & 15;
-    const uint32_t b = base + blk;
-    FitRec rec;
-    rec.err = 0xFFFFFFFFu;
-    rec.px1 = rec.px2 = 0;
-    rec.cidx_lo = rec.cidx_hi = rec.aidx_lo = rec.aidx_hi = 0;
-    rec.meta = 0;
-    if (b < r.first_block + r.num_blocks) {
-      const BlockSel sel = s_sel[blk];
-      int mode, entry, subset, rot, im;
-      if (sel.cls == 2 && slot_fit(sel, slot, prm.quality, mode, entry, subset, rot, im)) {
-        uint32_t px[16];
+// ---------------------------------------------------------------------------
+// Kernel 2: candidate set-up, persistent over the type queues in order.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) setup_kernel(BlockRange r, Bc7Params prm, Work w,
+                                                   const uint32_t *sctab) {
+  const int lane = threadIdx.x;
+  for (int t = 0; t < kNumTypes; t++) {
+    const uint32_t count = *(volatile uint32_t *)&w.counters[t];
+    for (;;) {
+      uint32_t b0 = 0;
+      if (lane == 0) b0 = atomicAdd(&w.counters[8 + t], 64u);
+      b0 = __shfl(b0, 0);
+      if (b0 >= count) break;
+      const uint32_t item = b0 + lane;
+      if (item >= count) continue;
+      const uint32_t id = w.queue[w.qoff[t] + item];
+      const uint32_t rel = id >> 4;
+      const int c = id & 15;
+      const uint32_t sel = w.sel[rel];
+      Cand k;
+      cand_of(sel, c, prm.quality, k);
+      uint32_t px[16];
+      load_block(r, rel, px);
+      Fit f;
+      set_fit(f, k.mode, k.rot, k.im);
+      const uint64_t bid = prm.block_id_base + r.first_block + rel;
+      CandRec rec;
+      rec.err = 0;
+      rec.px1[0] = rec.px1[1] = rec.px1[2] = 0;
+      rec.px2[0] = rec.px2[1] = rec.px2[2] = 0;
+      rec.aidx_lo = rec.aidx_hi = 0;
+      AState st;
+      st.id = id;
+      st.extra_err = 0;
+      st.pad = 0;
+      uint32_t active = 0, pbcs = 0, total = 0;
+      uint64_t cidx = 0;
+      if (kModes[k.mode].has_rot) {
+        uint64_t aidx;
+        uint32_t a1q, a2q;
+        const uint32_t aerr = alpha_fit(f, px, aidx, a1q, a2q);
+        // the RGB fit sees the stale statistics of the unrotated cluster
+        float avg[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int i = 0; i < 16; i++) px[i] = s_px[blk][i];
-        Fit f;
-        set_fit(f, mode, rot, im);
-        const int ns = kModes[mode].subsets;
-        const int shape = ns == 3 ? sel.shape3 : sel.shape2;
-        Rng rng;
-        rng.key = fit_key(prm.seed, prm.block_id_base + b,
-                          (uint32_t)mode | ((uint32_t)entry << 3) | ((uint32_t)subset << 5) |
-                              ((uint32_t)rot << 7) | ((uint32_t)im << 9));
-        rng.ctr = 0;
-        uint64_t cidx = 0, aidx = 0;
-        uint32_t err;
-        int pbc = 0;
-        if (kModes[mode].has_rot) {
-          // stale statistics of the unrotated 16-pixel cluster
-          float avg[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int i = 0; i < 16; i++)
 #pragma unroll
-          for (int i = 0; i < 16; i++)
+          for (int kk = 0; kk < 4; kk++) avg[kk] += (float)chan(px[i], kk);
 #pragma unroll
-            for (int k = 0; k < 4; k++) avg[k] += (float)chan(px[i], k);
-#pragma unroll
-          for (int k = 0; k < 4; k++) avg[k] = avg[k] / 16.0f;
-          err = fit_alpha(f, prm.quality, rng, sctab, px, avg, rec.px1, rec.px2, cidx, aidx);
-        } else {
-          const uint32_t smask = subset_mask(shape, ns, subset);
+        for (int kk = 0; kk < 4; kk++) avg[kk] = avg[kk] / 16.0f;
+        SubStart s0;
+        subset_setup(f, prm.quality, sctab, px, 0xFFFFu, k.rot + 1, 16, avg, false, s0);
+        rec.aidx_lo = (uint32_t)aidx;
+        rec.aidx_hi = (uint32_t)(aidx >> 32);
+        // Pack writes the RGB of the fit and the fitted alpha, no p-bit
+        rec.px1[0] = (to_pixel_int(s0.p1, f.qmask, -1) & 0x00FFFFFFu) | (a1q << 24);
+        rec.px2[0] = (to_pixel_int(s0.p2, f.qmask, -1) & 0x00FFFFFFu) | (a2q << 24);
+        st.p1[0] = s0.p1;
+        st.p2[0] = s0.p2;
+        st.err[0] = s0.err;
+        st.extra_err = aerr;
+        cidx = s0.idx;
+        total = s0.err + aerr;
+        active = s0.anneal ? 1u : 0u;
+      } else {
+        for (int s = 0; s < k.nsub; s++) {
+          const uint32_t smask = subset_mask(k.shape, k.nsub, s);
           const int n = __popc(smask);
           float avg[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-          uint32_t mnc = 0xFFFFFFFFu, mxc = 0;
           bool same = true;
           uint32_t firstp = 0;
           bool have_first = false;
@@ -1362,122 +1281,515 @@ __global__ void __launch_bounds__(64) encode_kernel(BlockRange r, Bc7Params prm,
             }
             same = same && px[i] == firstp;
 #pragma unroll
-            for (int k = 0; k < 4; k++) avg[k] += (float)chan(px[i], k);
+            for (int kk = 0; kk < 4; kk++) avg[kk] += (float)chan(px[i], kk);
           }
-          (void)mnc;
-          (void)mxc;
 #pragma unroll
-          for (int k = 0; k < 4; k++) avg[k] = avg[k] / (float)n;
-          uint32_t p1, p2;
-          err = fit_rgb(f, prm.quality, rng, sctab, px, smask, 0, n, avg, same, p1, p2, cidx, pbc);
-          // Pack's ToPixel of the chosen endpoints (idempotent on grid values)
-          if (f.pbit_type == PB_NONE) {
-            rec.px1 = to_pixel_int(p1, f.qmask, -1);
-            rec.px2 = to_pixel_int(p2, f.qmask, -1);
+          for (int kk = 0; kk < 4; kk++) avg[kk] = avg[kk] / (float)n;
+          SubStart ss;
+          subset_setup(f, prm.quality, sctab, px, smask, 0, n, avg, same, ss);
+          const int pb0 = f.pbit_type == PB_NONE ? -1 : pbc_bit(f.pbit_type, ss.pbc, 0);
+          const int pb1 = f.pbit_type == PB_NONE ? -1 : pbc_bit(f.pbit_type, ss.pbc, 1);
+          const uint32_t e1 = to_pixel_int(ss.p1, f.qmask, pb0);
+          const uint32_t e2 = to_pixel_int(ss.p2, f.qmask, pb1);
+          if (s == 0) {
+            rec.px1[0] = e1; rec.px2[0] = e2;
+            st.p1[0] = ss.p1; st.p2[0] = ss.p2; st.err[0] = ss.err;
+          } else if (s == 1) {
+            rec.px1[1] = e1; rec.px2[1] = e2;
+            st.p1[1] = ss.p1; st.p2[1] = ss.p2; st.err[1] = ss.err;
           } else {
-            rec.px1 = to_pixel_int(p1, f.qmask, pbc_bit(f.pbit_type, pbc, 0));
-            rec.px2 = to_pixel_int(p2, f.qmask, pbc_bit(f.pbit_type, pbc, 1));
+            rec.px1[2] = e1; rec.px2[2] = e2;
+            st.p1[2] = ss.p1; st.p2[2] = ss.p2; st.err[2] = ss.err;
           }
+          pbcs |= (uint32_t)ss.pbc << (2 * s);
+          cidx |= ss.idx;
+          total += ss.err;
+          if (ss.anneal) active |= 1u << s;
         }
-        rec.err = err;
-        rec.cidx_lo = (uint32_t)cidx;
-        rec.cidx_hi = (uint32_t)(cidx >> 32);
-        rec.aidx_lo = (uint32_t)aidx;
-        rec.aidx_hi = (uint32_t)(aidx >> 32);
-        rec.meta = (uint32_t)pbc | ((uint32_t)rot << 8) | ((uint32_t)im << 16);
+      }
+      rec.cidx_lo = (uint32_t)cidx;
+      rec.cidx_hi = (uint32_t)(cidx >> 32);
+      rec.meta = pbcs | ((uint32_t)k.rot << 8) | ((uint32_t)k.im << 10);
+      rec.err = active ? 0xFFFFFFFEu : total;  // final unless annealing follows
+      w.rec[(size_t)rel * 16 + c] = rec;
+      if (active) {
+        st.info = (uint32_t)k.mode | ((uint32_t)k.rot << 3) | ((uint32_t)k.im << 5) |
+                  ((uint32_t)k.nsub << 6) | ((uint32_t)k.shape << 8) | (active << 14) |
+                  ((uint32_t)k.entry << 17);
+        st.pbc = pbcs;
+        st.idx_lo = (uint32_t)cidx;
+        st.idx_hi = (uint32_t)(cidx >> 32);
+        const uint32_t slot = atomicAdd(&w.counters[16], 1u);
+        w.astate[slot] = st;
       }
     }
-    s_rec[blk][slot] = rec;
   }
-  __syncthreads();
+}
 
-  // phase 4: pick the mode and pack, one lane per block
-  if (lane < 4) {
-    const int blk = lane;
-    const uint32_t b = base + blk;
-    if (b < r.first_block + r.num_blocks) {
-      const BlockSel sel = s_sel[blk];
-      BitWriter w;
-      if (sel.cls == 0) {
-        const uint32_t rel = b - r.first_block;
-        const uint32_t wm = (scr.chunk_prefix[rel / 256] + scr.local_prefix[rel]) % 9;
-        pack_solid(s_px[blk][0], wm, w);
-      } else if (sel.cls == 1) {
-        w.put(1u << 6, 7);  // WriteTransparentBlock (Compressor.cpp:1416-1421)
-      } else {
-        uint64_t best = 0xFFFFFFFFFFFFFFFFull;
-        int bmode = -1, bfirst = 0, bshape = 0;
-        for (int oi = 0; oi < 8; oi++) {
-          const int mode = kOrder[oi];
-          if (!((sel.modes >> mode) & 1)) continue;
-          // candidates of this mode in the reference's evaluation order
-          for (int c = 0; c < 8; c++) {
-            int first = -1, nsub = 1, shape = 0;
-            if (sel.layout == 0) {
-              if (mode == 0 || mode == 2) {
-                if (c > 0) break;
-                first = mode == 0 ? 0 : 3;
-                nsub = 3;
-                shape = sel.shape3;
-              } else if (mode == 1 || mode == 3 || mode == 7) {
-                if (c > 0) break;
-                first = mode == 1 ? 6 : (mode == 3 ? 8 : 10);
-                nsub = 2;
-                shape = sel.shape2;
-              } else if (mode == 6) {
-                if (c > 1) break;
-                first = 12 + c;
-                shape = c == 0 ? sel.shape2 : sel.shape3;
-              } else {
-                break;
-              }
-            } else {
-              if (mode == 4) {
-                first = c;
-              } else if (mode == 5) {
-                if (c > 3) break;
-                first = 8 + c;
-              } else if (mode == 6) {
-                if (c > 0) break;
-                first = 12;
-                shape = sel.shape2;
-              } else if (mode == 7) {
-                if (c > 0) break;
-                first = 13;
-                nsub = 2;
-                shape = sel.shape2;
-              } else {
-                break;
-              }
-            }
-            uint64_t e = 0;
-            bool ok = true;
-            for (int s = 0; s < nsub; s++) {
-              const uint32_t se = s_rec[blk][first + s].err;
-              if (se == 0xFFFFFFFFu) ok = false;
-              e += se;
-            }
-            if (!ok) continue;
-            if (e < best) {
-              best = e;
-              bmode = mode;
-              bfirst = first;
-              bshape = shape;
-            }
-          }
-        }
-        if (bmode >= 0) pack_mode(bmode, bshape, &s_rec[blk][bfirst], w);
+// ---------------------------------------------------------------------------
+// Kernel 3: simulated annealing (OptimizeEndpointsForCluster,
+// Compressor.cpp:583-624), persistent.  A lane owns one candidate and
+// advances every still-annealing subset of it by one step per iteration; the
+// 16 pixels are scored in one pass, each against its own subset's
+// endpoints.  Lanes that finish take the next candidate from a wave-local
+// window of the global list, so the wave stays full.
+// ---------------------------------------------------------------------------
+struct Lane {
+  bool has;
+  uint32_t id;
+  Fit f;
+  int nsub, entry, shape;
+  uint32_t sub2;       // subset of pixel i in bits 2i..2i+1
+  int ptmode;
+  uint32_t px[16];
+  uint32_t active;
+  uint32_t p1[3], p2[3], bp1[3], bp2[3], cur[3], best[3];
+  int cpbc[3], bpbc[3], energy[3];
+  uint32_t key[3], ctr[3];
+  uint64_t idx;        // indices of all subsets (best so far)
+  uint32_t extra;
+};
+
+__device__ void lane_load(Lane &L, const BlockRange &r, const Bc7Params &prm, const AState &st) {
+  L.has = true;
+  L.id = st.id;
+  const int mode = st.info & 7, rot = (st.info >> 3) & 3, im = (st.info >> 5) & 1;
+  L.nsub = (st.info >> 6) & 3;
+  L.shape = (st.info >> 8) & 63;
+  L.active = (st.info >> 14) & 7;
+  L.entry = (st.info >> 17) & 1;
+  set_fit(L.f, mode, rot, im);
+  L.ptmode = kModes[mode].has_rot ? rot + 1 : 0;
+  L.sub2 = 0;
+  for (int s = 1; s < L.nsub; s++) {
+    const uint32_t m = subset_mask(L.shape, L.nsub, s);
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+      if ((m >> i) & 1) L.sub2 |= (uint32_t)s << (2 * i);
+  }
+  load_block(r, st.id >> 4, L.px);
+  const uint64_t bid = prm.block_id_base + r.first_block + (st.id >> 4);
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+    L.p1[s] = L.bp1[s] = st.p1[s];
+    L.p2[s] = L.bp2[s] = st.p2[s];
+    L.cur[s] = L.best[s] = st.err[s];
+    L.cpbc[s] = L.bpbc[s] = (st.pbc >> (2 * s)) & 3;
+    L.energy[s] = 0;
+    L.ctr[s] = 0;
+    L.key[s] = fit_key(prm.seed, bid, cand_tag(mode, L.entry, s, rot, im));
+  }
+  L.idx = (uint64_t)st.idx_lo | ((uint64_t)st.idx_hi << 32);
+  L.extra = st.extra_err;
+}
+
+// One annealing step of every active subset.  Returns true when the
+// candidate is finished.
+__device__ bool lane_step(Lane &L, uint32_t q) {
+  const Fit &f = L.f;
+  const int nb = 1 << f.nbits_idx;
+  const int prec = f.nbits_idx;
+  uint32_t np1[3], np2[3], e1[3], e2[3];
+  int npbc[3];
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+    np1[s] = np2[s] = e1[s] = e2[s] = 0;
+    npbc[s] = 0;
+    if (!((L.active >> s) & 1)) continue;
+    Rng rng;
+    rng.key = L.key[s];
+    rng.ctr = L.ctr[s];
+    pick_neighbor(f, rng, L.p1[s], L.p2[s], L.cpbc[s], np1[s], np2[s], npbc[s], L.bp1[s],
+                  L.bp2[s], L.bpbc[s]);
+    L.ctr[s] = rng.ctr;
+    e1[s] = to_pixel_int(np1[s], f.qmask, pbc_bit(f.pbit_type, npbc[s], 0));
+    e2[s] = to_pixel_int(np2[s], f.qmask, pbc_bit(f.pbit_type, npbc[s], 1));
+  }
+  // per-subset projection constants
+  int lsq[3];
+  float flsq[3];
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+    int l = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int d = (int)chan(e2[s], k) - (int)chan(e1[s], k);
+      l += d * d;
+    }
+    lsq[s] = l;
+    flsq[s] = (float)l;
+  }
+  uint32_t nerr[3] = {0, 0, 0};
+  uint64_t nidx = 0;
+  const float kf = (float)(nb - 1);
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int s = (L.sub2 >> (2 * i)) & 3;
+    if (!((L.active >> s) & 1)) continue;
+    const uint32_t qa = s == 0 ? e1[0] : (s == 1 ? e1[1] : e1[2]);
+    const uint32_t qb = s == 0 ? e2[0] : (s == 1 ? e2[1] : e2[2]);
+    const int l = s == 0 ? lsq[0] : (s == 1 ? lsq[1] : lsq[2]);
+    const float fl = s == 0 ? flsq[0] : (s == 1 ? flsq[1] : flsq[2]);
+    const uint32_t pix = L.px[i];
+    int a[4], b[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      a[k] = (int)chan(qa, k);
+      b[k] = (int)chan(qb, k);
+    }
+    uint32_t me;
+    int best = 0;
+    if (l == 0) {
+      me = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int d = (int)chan(pix, k) - a[k];
+        me += (uint32_t)(d * d);
       }
-      write_block(r.out + (size_t)b * 16, w);
+    } else {
+      const uint32_t pt = point_of(pix, L.ptmode);
+      int d = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) d += ((int)chan(pt, k) - a[k]) * (b[k] - a[k]);
+      const float pct = (float)d / fl;
+      const float t = pct * kf;
+      int j1 = (int)floorf(t);
+      int j2 = (int)ceilf(t);
+      j1 = j1 < 0 ? 0 : j1;
+      j1 = j1 > nb - 1 ? nb - 1 : j1;
+      j2 = j2 > nb - 1 ? nb - 1 : j2;
+      uint32_t e1v = 0;
+      {
+        const int w1 = kW[prec - 1][j1], w0 = 64 - w1;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int ip = (a[k] * w0 + b[k] * w1 + 32) >> 6;
+          const int dd = (int)chan(pix, k) - ip;
+          e1v += (uint32_t)(dd * dd);
+        }
+      }
+      best = j1;
+      me = e1v;
+      if (j2 > j1) {
+        const int w1 = kW[prec - 1][j2], w0 = 64 - w1;
+        uint32_t e2v = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int ip = (a[k] * w0 + b[k] * w1 + 32) >> 6;
+          const int dd = (int)chan(pix, k) - ip;
+          e2v += (uint32_t)(dd * dd);
+        }
+        if (e2v < e1v) {
+          best = j2;
+          me = e2v;
+        }
+      }
+    }
+    if (s == 0) nerr[0] += me;
+    else if (s == 1) nerr[1] += me;
+    else nerr[2] += me;
+    nidx |= (uint64_t)best << (4 * i);
+  }
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+    if (!((L.active >> s) & 1)) continue;
+    const uint32_t err = nerr[s];
+    const float temp = (float)L.energy[s] / (float)((int)q - 1);
+    bool accept = err < L.cur[s];
+    if (!accept) {
+      // AcceptNewEndpointError (Compressor.cpp:524-536)
+      Rng rng;
+      rng.key = L.key[s];
+      rng.ctr = L.ctr[s];
+      const double pr = exp(((double)0.1f * ((double)L.cur[s] - (double)err)) / (double)temp);
+      const double rr = (double)rng.frand();
+      L.ctr[s] = rng.ctr;
+      accept = rr < pr;
+    }
+    if (accept) {
+      L.cur[s] = err;
+      L.p1[s] = np1[s];
+      L.p2[s] = np2[s];
+      L.cpbc[s] = npbc[s];
+    }
+    if (err < L.best[s]) {
+      // indices of this subset's pixels
+      uint64_t m = 0;
+#pragma unroll
+      for (int i = 0; i < 16; i++)
+        if (((L.sub2 >> (2 * i)) & 3) == (uint32_t)s) m |= (uint64_t)0xF << (4 * i);
+      L.idx = (L.idx & ~m) | (nidx & m);
+      L.bp1[s] = np1[s];
+      L.bp2[s] = np2[s];
+      L.bpbc[s] = npbc[s];
+      L.best[s] = err;
+      L.energy[s] = 0;
+    }
+    L.energy[s]++;
+    if (!(L.best[s] > 0 && L.energy[s] < (int)q)) L.active &= ~(1u << s);
+  }
+  return L.active == 0;
+}
+
+__device__ void lane_finish(Lane &L, Work &w) {
+  CandRec *rc = &w.rec[(size_t)(L.id >> 4) * 16 + (L.id & 15)];
+  const Fit &f = L.f;
+  uint32_t total = L.extra;
+  uint32_t pbcs = rc->meta & 0x3Fu;
+  for (int s = 0; s < L.nsub; s++) {
+    const uint32_t bp1 = s == 0 ? L.bp1[0] : (s == 1 ? L.bp1[1] : L.bp1[2]);
+    const uint32_t bp2 = s == 0 ? L.bp2[0] : (s == 1 ? L.bp2[1] : L.bp2[2]);
+    const int pb = s == 0 ? L.bpbc[0] : (s == 1 ? L.bpbc[1] : L.bpbc[2]);
+    total += s == 0 ? L.best[0] : (s == 1 ? L.best[1] : L.best[2]);
+    const int pb0 = f.pbit_type == PB_NONE ? -1 : pbc_bit(f.pbit_type, pb, 0);
+    const int pb1 = f.pbit_type == PB_NONE ? -1 : pbc_bit(f.pbit_type, pb, 1);
+    uint32_t e1 = to_pixel_int(bp1, f.qmask, pb0);
+    uint32_t e2 = to_pixel_int(bp2, f.qmask, pb1);
+    if (kModes[f.mode].has_rot) {
+      e1 = (e1 & 0x00FFFFFFu) | (rc->px1[0] & 0xFF000000u);
+      e2 = (e2 & 0x00FFFFFFu) | (rc->px2[0] & 0xFF000000u);
+    }
+    rc->px1[s] = e1;
+    rc->px2[s] = e2;
+    pbcs = (pbcs & ~(3u << (2 * s))) | ((uint32_t)pb << (2 * s));
+  }
+  rc->cidx_lo = (uint32_t)L.idx;
+  rc->cidx_hi = (uint32_t)(L.idx >> 32);
+  rc->meta = (rc->meta & ~0x3Fu) | pbcs;
+  rc->err = total;
+}
+
+__global__ void __launch_bounds__(64) anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
+  const int lane = threadIdx.x;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  const uint32_t total = *(volatile uint32_t *)&w.counters[16];
+  Lane L;
+  L.has = false;
+  uint32_t wbase = 0, wend = 0;  // wave-local window of the list
+  bool exhausted = false;
+  for (;;) {
+    // refill idle lanes from the window, fetching a new window when needed
+    for (;;) {
+      const uint64_t need = __ballot(!L.has);
+      if (!need || exhausted) break;
+      if (wbase >= wend) {
+        uint32_t b0 = 0;
+        if (lane == 0) b0 = atomicAdd(&w.counters[17], 64u);
+        b0 = __shfl(b0, 0);
+        if (b0 >= total) {
+          exhausted = true;
+          break;
+        }
+        wbase = b0;
+        wend = min(b0 + 64u, total);
+      }
+      const uint32_t avail = wend - wbase;
+      const uint32_t rank = (uint32_t)__popcll(need & lt);
+      if (!L.has && rank < avail) lane_load(L, r, prm, w.astate[wbase + rank]);
+      wbase += min(avail, (uint32_t)__popcll(need));
+    }
+    if (!__ballot(L.has)) break;
+    if (L.has) {
+      if (lane_step(L, prm.quality)) {
+        lane_finish(L, w);
+        L.has = false;
+      }
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Kernel 4: choose the mode and write the block, one lane per block.
+// ---------------------------------------------------------------------------
+
+struct BitWriter {
+  uint64_t lo = 0, hi = 0;
+  int pos = 0;
+  __device__ __forceinline__ void put(uint32_t v, int n) {
+    if (n <= 0) return;
+    const uint64_t vv = (uint64_t)(n >= 32 ? v : (v & ((1u << n) - 1u)));
+    if (pos >= 64) {
+      hi |= vv << (pos - 64);
+    } else {
+      lo |= vv << pos;
+      if (pos + n > 64) hi |= vv >> (64 - pos);
+    }
+    pos += n;
+  }
+};
+
+__device__ __forceinline__ uint32_t idx_at(uint32_t lo, uint32_t hi, int i) {
+  return i < 8 ? (lo >> (4 * i)) & 0xFu : (hi >> (4 * (i - 8))) & 0xFu;
+}
+
+// CompressOptimalColorBC7 (Compressor.cpp:1424-1458).
+__device__ void pack_solid(uint32_t pixel, uint32_t wm, BitWriter &w) {
+  w.put(1u << 5, 6);
+  w.put(0, 2);
+#pragma unroll
+  for (int ch = 0; ch < 3; ch++) {
+    const uint32_t v = chan(pixel, ch);
+    w.put(kMode5Tab[v][0], 7);
+    w.put(kMode5Tab[v][1], 7);
+  }
+  w.put(pixel >> 24, 8);
+  w.put(pixel >> 24, 8);
+  w.put(0xaaaaaaabu, 31);
+  w.put(kWM[wm], 31);
+}
+
+__device__ __forceinline__ uint32_t invert_idx(uint32_t &lo, uint32_t &hi, uint32_t mask16,
+                                               uint32_t nv) {
+  uint32_t nl = 0, nh = 0;
+  for (int i = 0; i < 16; i++) {
+    uint32_t v = idx_at(lo, hi, i);
+    if ((mask16 >> i) & 1) v = nv - v;
+    if (i < 8) nl |= v << (4 * i);
+    else nh |= v << (4 * (i - 8));
+  }
+  lo = nl;
+  hi = nh;
+  return 0;
+}
+
+// CompressionMode::Pack (Compressor.cpp:1096-1298).  The p-bits are written
+// unswapped after an anchor fix-up swap, as the reference does
+// (:1149-1170 vs :1234-1241).
+__device__ void pack_mode(int mode, int shape, const CandRec &rec, BitWriter &w) {
+  const ModeAttr a = kModes[mode];
+  const int ns = a.subsets;
+  const int rot = (rec.meta >> 8) & 3;
+  const int im = (rec.meta >> 10) & 1;
+  w.put(1u << mode, mode + 1);
+  w.put((uint32_t)shape, a.part_bits);
+  if (a.has_rot) w.put((uint32_t)rot, 2);
+  if (a.has_idx_mode) w.put((uint32_t)im, 1);
+  uint32_t px1[3] = {rec.px1[0], rec.px1[1], rec.px1[2]};
+  uint32_t px2[3] = {rec.px2[0], rec.px2[1], rec.px2[2]};
+  uint32_t cl = rec.cidx_lo, ch = rec.cidx_hi;
+  uint32_t al = rec.aidx_lo, ah = rec.aidx_hi;
+  const int nib = (im == 0) ? a.idx_bits : a.alpha_idx_bits;
+  const int nab = (im == 0) ? a.alpha_idx_bits : a.idx_bits;
+  for (int s = 0; s < ns; s++) {
+    const uint32_t smask = subset_mask(shape, ns, s);
+    const int anchor = anchor_of(s, shape, ns);
+    if (idx_at(cl, ch, anchor) >> (nib - 1)) {
+      const uint32_t t = px1[s];
+      px1[s] = px2[s];
+      px2[s] = t;
+      invert_idx(cl, ch, smask, (1u << nib) - 1);
+      if (a.has_rot) invert_idx(al, ah, 0xFFFFu, (1u << nab) - 1);
+    }
+    if (a.has_rot && (idx_at(al, ah, anchor) >> (nab - 1)) > 0) {
+      const uint32_t a1 = px1[s] >> 24, a2 = px2[s] >> 24;
+      px1[s] = (px1[s] & 0x00FFFFFFu) | (a2 << 24);
+      px2[s] = (px2[s] & 0x00FFFFFFu) | (a1 << 24);
+      invert_idx(al, ah, 0xFFFFu, (1u << nab) - 1);
+    }
+  }
+  const int cp = a.color_prec, ap = a.alpha_prec;
+  for (int c = 0; c < 3; c++)
+    for (int s = 0; s < ns; s++) {
+      w.put(chan(px1[s], c) >> (8 - cp), cp);
+      w.put(chan(px2[s], c) >> (8 - cp), cp);
+    }
+  if (ap) {
+    for (int s = 0; s < ns; s++) {
+      w.put(chan(px1[s], 3) >> (8 - ap), ap);
+      w.put(chan(px2[s], 3) >> (8 - ap), ap);
+    }
+  }
+  if (a.pbit_type != PB_NONE) {
+    for (int s = 0; s < ns; s++) {
+      const int pbc = (rec.meta >> (2 * s)) & 3;
+      w.put((uint32_t)pbc_bit(a.pbit_type, pbc, 0), 1);
+      if (a.pbit_type != PB_SHARED) w.put((uint32_t)pbc_bit(a.pbit_type, pbc, 1), 1);
+    }
+  }
+  if (a.has_idx_mode && im == 1) {
+    for (int i = 0; i < 16; i++) w.put(idx_at(al, ah, i), i == 0 ? 1 : 2);
+    for (int i = 0; i < 16; i++) w.put(idx_at(cl, ch, i), i == 0 ? 2 : 3);
+  } else {
+    for (int i = 0; i < 16; i++) {
+      int s = 0;
+      for (int t = 1; t < ns; t++)
+        if ((subset_mask(shape, ns, t) >> i) & 1) s = t;
+      const int anchor = anchor_of(s, shape, ns);
+      w.put(idx_at(cl, ch, i), i == anchor ? nib - 1 : nib);
+    }
+    if (a.has_rot)
+      for (int i = 0; i < 16; i++) w.put(idx_at(al, ah, i), i == 0 ? nab - 1 : nab);
+  }
+}
+
+__global__ void __launch_bounds__(256) pack_kernel(BlockRange r, Bc7Scratch scr, Work w) {
+  const uint32_t rel = blockIdx.x * blockDim.x + threadIdx.x;
+  if (rel >= r.num_blocks) return;
+  const uint32_t b = r.first_block + rel;
+  const uint32_t sel = w.sel[rel];
+  BitWriter bw;
+  if (sel_cls(sel) == 0) {
+    const uint32_t bx = b % r.blocks_wide, by = b / r.blocks_wide;
+    const uint32_t pixel = r.img[(size_t)(by * 4) * r.width + bx * 4];
+    const uint32_t wm = (scr.chunk_prefix[rel / 256] + scr.local_prefix[rel]) % 9;
+    pack_solid(pixel, wm, bw);
+  } else if (sel_cls(sel) == 1) {
+    bw.put(1u << 6, 7);  // WriteTransparentBlock (Compressor.cpp:1416-1421)
+  } else {
+    // candidates in CompressClusters' order: modes 0,2,1,3,7,4,5,6, shape
+    // entries and rotations/index modes in loop order; first strict minimum
+    static constexpr int kL0[7] = {0, 1, 2, 3, 4, 5, 6};
+    static constexpr int kL1[14] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13};
+    const CandRec *rc = &w.rec[(size_t)rel * 16];
+    uint32_t best = 0xFFFFFFFFu;
+    int bc = -1;
+    if (sel_layout(sel) == 0) {
+      for (int i = 0; i < 7; i++) {
+        const uint32_t e = rc[kL0[i]].err;
+        if (e < best) {
+          best = e;
+          bc = kL0[i];
+        }
+      }
+    } else {
+      // layout 1 order: mode 7 (13) first, then 4 (0-7), 5 (8-11), 6 (12)
+      const uint32_t e7 = rc[13].err;
+      if (e7 < best) {
+        best = e7;
+        bc = 13;
+      }
+      for (int i = 0; i < 13; i++) {
+        const uint32_t e = rc[kL1[i]].err;
+        if (e < best) {
+          best = e;
+          bc = kL1[i];
+        }
+      }
+    }
+    if (bc >= 0) {
+      Cand k;
+      cand_of(sel, bc, 1, k);
+      pack_mode(k.mode, k.shape, rc[bc], bw);
+    }
+  }
+  uint4 v;
+  v.x = (uint32_t)bw.lo;
+  v.y = (uint32_t)(bw.lo >> 32);
+  v.z = (uint32_t)bw.hi;
+  v.w = (uint32_t)(bw.hi >> 32);
+  reinterpret_cast<uint4 *>(r.out)[b] = v;
+}
+
+__global__ void reset_kernel(uint32_t *counters, int n) {
+  const int i = threadIdx.x;
+  if (i < n) counters[i] = 0;
 }
 
 // Solid-block flags and in-chunk exclusive prefix (chunks of 256 blocks).
 __global__ void __launch_bounds__(256) solid_scan_kernel(BlockRange r, Bc7Scratch scr) {
   __shared__ uint32_t s_wave[4];
   const uint32_t t = threadIdx.x;
-  const uint32_t rel = blockIdx.x * 256 + t;  // relative to the job's first block
+  const uint32_t rel = blockIdx.x * 256 + t;
   const uint32_t b = r.first_block + rel;
   bool solid = false;
   if (rel < r.num_blocks) {
@@ -1540,6 +1852,7 @@ __global__ void __launch_bounds__(1024) chunk_scan_kernel(Bc7Scratch scr) {
 
 namespace {
 int g_tables_uploaded[64];
+int g_num_cus[64];
 }
 
 hipError_t bc7_upload_tables(int device) {
@@ -1553,11 +1866,21 @@ hipError_t bc7_upload_tables(int device) {
   if (e != hipSuccess) return e;
   e = hipMemcpyToSymbol(HIP_SYMBOL(bc7::kDxt6Tab), d6, sizeof(d6));
   if (e != hipSuccess) return e;
+  int cus = 0;
+  e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  if (e != hipSuccess) return e;
+  g_num_cus[device] = cus > 0 ? cus : 1;
   g_tables_uploaded[device] = 1;
   return hipSuccess;
 }
 
-hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch &s,
+size_t bc7_work_bytes(uint32_t num_blocks) {
+  const size_t n = num_blocks;
+  return 256 + n * 4 + n * 16 * sizeof(bc7::CandRec) + n * 19 * 4 + 64 * 4 +
+         n * 14 * sizeof(bc7::AState) + 4096;
+}
+
+hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch &s, void *work,
                       hipStream_t stream) {
   if (r.num_blocks == 0) return hipSuccess;
   int dev = 0;
@@ -1567,11 +1890,38 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   if (e != hipSuccess) return e;
   const uint32_t *sct = bc7_single_color_table(dev);
   if (!sct) return hipErrorOutOfMemory;
-  const uint32_t nchunks = (r.num_blocks + 255) / 256;
+  const uint32_t n = r.num_blocks;
+  // carve the work area
+  uint8_t *base = reinterpret_cast<uint8_t *>(work);
+  auto carve = [&](size_t bytes) {
+    uint8_t *q = base;
+    base += (bytes + 255) & ~(size_t)255;
+    return q;
+  };
+  bc7::Work w;
+  w.counters = reinterpret_cast<uint32_t *>(carve(64 * 4));
+  w.sel = reinterpret_cast<uint32_t *>(carve((size_t)n * 4));
+  w.rec = reinterpret_cast<bc7::CandRec *>(carve((size_t)n * 16 * sizeof(bc7::CandRec)));
+  w.queue = reinterpret_cast<uint32_t *>(carve((size_t)n * 19 * 4));
+  w.astate = reinterpret_cast<bc7::AState *>(carve((size_t)n * 14 * sizeof(bc7::AState)));
+  w.astate_cap = n * 14;
+  static const int cap[bc7::kNumTypes] = {1, 1, 1, 1, 1, 2, 8, 4};
+  uint32_t off = 0;
+  for (int t = 0; t < bc7::kNumTypes; t++) {
+    w.qoff[t] = off;
+    off += cap[t] * n;
+  }
+  const uint32_t nchunks = (n + 255) / 256;
+  const uint32_t ngroups = (n + 3) / 4;
+  hipLaunchKernelGGL(bc7::reset_kernel, dim3(1), dim3(64), 0, stream, w.counters, 64);
   hipLaunchKernelGGL(bc7::solid_scan_kernel, dim3(nchunks), dim3(256), 0, stream, r, s);
   hipLaunchKernelGGL(bc7::chunk_scan_kernel, dim3(1), dim3(1024), 0, stream, s);
-  const uint32_t ngroups = (r.num_blocks + 3) / 4;
-  hipLaunchKernelGGL(bc7::encode_kernel, dim3(ngroups), dim3(64), 0, stream, r, p, s, sct);
+  hipLaunchKernelGGL(bc7::select_kernel, dim3(ngroups), dim3(64), 0, stream, r, p, w);
+  const uint32_t persistent = (uint32_t)g_num_cus[dev] * 16;
+  hipLaunchKernelGGL(bc7::setup_kernel, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+  if (p.quality > 0)
+    hipLaunchKernelGGL(bc7::anneal_kernel, dim3(persistent), dim3(64), 0, stream, r, p, w);
+  hipLaunchKernelGGL(bc7::pack_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, r, s, w);
   return hipGetLastError();
 }
 

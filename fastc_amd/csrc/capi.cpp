@@ -170,16 +170,19 @@ fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8
   hipError_t e = hipSuccess;
   switch (j->format) {
     case FH_FORMAT_BPTC: {
-      const uint32_t nchunks = (count + 255) / 256;
-      const size_t need = (size_t)count + 256 + (size_t)nchunks * 4 + 256;
+      const uint32_t chunk = kBc7MaxBlocksPerCall;
+      const uint32_t per = count < chunk ? count : chunk;
+      const uint32_t nchunks = (per + 255) / 256;
+      const size_t prefix_bytes = ((size_t)per + 255) & ~(size_t)255;
+      const size_t work_off = (prefix_bytes + (size_t)nchunks * 4 + 255) & ~(size_t)255;
+      const size_t need = work_off + bc7_work_bytes(per) + 256;
       e = grow(&st->scratch, &st->scratch_size, need);
       if (e != hipSuccess) return fail_hip(e, "scratch allocation");
+      uint8_t *sbase = reinterpret_cast<uint8_t *>(st->scratch);
       Bc7Scratch s;
-      s.local_prefix = reinterpret_cast<uint8_t *>(st->scratch);
-      s.chunk_prefix =
-          reinterpret_cast<uint32_t *>(reinterpret_cast<uint8_t *>(st->scratch) +
-                                       ((count + 255) & ~255u));
-      s.num_chunks = nchunks;
+      s.local_prefix = sbase;
+      s.chunk_prefix = reinterpret_cast<uint32_t *>(sbase + prefix_bytes);
+      void *work = sbase + work_off;
       s.counter = st->counter;
       uint32_t caller_c = 0;
       if (j->flags & FH_FLAG_CALLER_WATERMARK) {
@@ -194,8 +197,16 @@ fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8
       p.block_modes = j->block_modes ? (j->block_modes & 0xFFu) : 0xFFu;
       p.seed = j->seed;
       p.block_id_base = 0;
-      e = bc7_encode(r, p, s, stream);
-      if (e != hipSuccess) return fail_hip(e, "BC7 encode launch");
+      // jobs larger than one call's work area run as consecutive calls on the
+      // stream; the watermark counter carries across them in order
+      for (uint32_t done = 0; done < count; done += chunk) {
+        BlockRange rc = r;
+        rc.first_block = first + done;
+        rc.num_blocks = count - done < chunk ? count - done : chunk;
+        s.num_chunks = (rc.num_blocks + 255) / 256;
+        e = bc7_encode(rc, p, s, work, stream);
+        if (e != hipSuccess) return fail_hip(e, "BC7 encode launch");
+      }
       if (j->flags & FH_FLAG_CALLER_WATERMARK) {
         e = hipMemcpyAsync(&caller_c, s.counter, 4, hipMemcpyDeviceToHost, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);

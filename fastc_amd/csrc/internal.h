@@ -36,7 +36,14 @@ struct Bc7Scratch {
 // Single-colour endpoint tables (uploaded once per device).
 const uint32_t *bc7_single_color_table(int device);
 
-hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch &s,
+// Device work area bc7_encode needs for num_blocks blocks (candidate
+// records, queues and annealing states).
+size_t bc7_work_bytes(uint32_t num_blocks);
+
+// Largest block count one bc7_encode call takes; callers split bigger jobs.
+constexpr uint32_t kBc7MaxBlocksPerCall = 1u << 20;
+
+hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch &s, void *work,
                       hipStream_t stream);
 
 hipError_t dxt_encode(const BlockRange &r, int dxt5, hipStream_t stream);
