@@ -1,0 +1,237 @@
+// C++ mirror of FasTC's encode API on top of the fasthip C ABI.
+//
+// Same names, argument meaning and error behaviour as the reference:
+//   fasthip::FasTC::ECompressionFormat   Base/include/FasTC/CompressionFormat.h:26-55
+//   fasthip::FasTC::CompressionJob       Base/include/FasTC/CompressionJob.h:40-140
+//   fasthip::FasTC::DecompressionJob     Base/include/FasTC/CompressionJob.h:144-169
+//   fasthip::SCompressionSettings        Core/include/FasTC/TexComp.h:30-76
+//   fasthip::CompressImageData           Core/include/FasTC/TexComp.h:81-88
+//   fasthip::BPTCC::CompressionSettings  BPTCEncoder/include/FasTC/BPTCCompressor.h:123-158
+//   fasthip::BPTCC::Compress/Decompress  BPTCCompressor.h:166-167, 216
+//   fasthip::DXTC::CompressImageDXT1/5   DXTEncoder/include/FasTC/DXTCompressor.h:24-25
+//   fasthip::ETCC::Compress_RG           ETCEncoder/include/FasTC/ETCCompressor.h:35
+// Everything lives in namespace fasthip so it can be linked next to FasTC;
+// the only new setting is SCompressionSettings::bUseGPU, mirroring bUseNVTT.
+//
+// CompressImageData returns false and prints "TexComp -- <message>" on
+// stderr like the reference (Core/src/TexComp.cpp:157-159).  The codec entry
+// points return void in the reference and have no error channel; here they
+// throw fasthip::Error instead of silently producing nothing.  There is no
+// CPU fallback: without a HIP device every entry point reports an error.
+#ifndef FASTHIP_FASTC_HPP_
+#define FASTHIP_FASTC_HPP_
+
+#include <stdint.h>
+#include <stdio.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "../fasthip.h"
+
+namespace fasthip {
+
+class Error : public std::runtime_error {
+ public:
+  Error(fh_status s, const std::string &what) : std::runtime_error(what), status(s) {}
+  fh_status status;
+};
+
+namespace FasTC {
+
+enum ECompressionFormat {
+  eCompressionFormat_DXT1 = FH_FORMAT_DXT1,
+  eCompressionFormat_DXT5 = FH_FORMAT_DXT5,
+  eCompressionFormat_ETC1 = FH_FORMAT_ETC1,
+  eCompressionFormat_BPTC = FH_FORMAT_BPTC
+};
+
+class CompressionJob {
+ public:
+  CompressionJob(ECompressionFormat fmt, const uint8_t *in, uint8_t *out, uint32_t w, uint32_t h)
+      : m_Format(fmt), m_InBuf(in), m_OutBuf(out), m_Width(w), m_Height(h), m_XStart(0),
+        m_XEnd(w), m_YStart(0), m_YEnd(h) {}
+  CompressionJob(ECompressionFormat fmt, const uint8_t *in, uint8_t *out, uint32_t w, uint32_t h,
+                 uint32_t x0, uint32_t y0)
+      : m_Format(fmt), m_InBuf(in), m_OutBuf(out), m_Width(w), m_Height(h), m_XStart(x0),
+        m_XEnd(w), m_YStart(y0), m_YEnd(h) {}
+  CompressionJob(ECompressionFormat fmt, const uint8_t *in, uint8_t *out, uint32_t w, uint32_t h,
+                 uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1)
+      : m_Format(fmt), m_InBuf(in), m_OutBuf(out), m_Width(w), m_Height(h), m_XStart(x0),
+        m_XEnd(x1), m_YStart(y0), m_YEnd(y1) {}
+  ECompressionFormat Format() const { return m_Format; }
+  const uint8_t *InBuf() const { return m_InBuf; }
+  uint8_t *OutBuf() const { return m_OutBuf; }
+  uint32_t Width() const { return m_Width; }
+  uint32_t Height() const { return m_Height; }
+  uint32_t XStart() const { return m_XStart; }
+  uint32_t XEnd() const { return m_XEnd; }
+  uint32_t YStart() const { return m_YStart; }
+  uint32_t YEnd() const { return m_YEnd; }
+  uint32_t CoordsToBlockIdx(uint32_t x, uint32_t y) const { return (y / 4) * (m_Width / 4) + x / 4; }
+
+ private:
+  ECompressionFormat m_Format;
+  const uint8_t *m_InBuf;
+  uint8_t *m_OutBuf;
+  uint32_t m_Width, m_Height;
+  uint32_t m_XStart, m_XEnd, m_YStart, m_YEnd;
+};
+
+class DecompressionJob {
+ public:
+  DecompressionJob(ECompressionFormat fmt, const uint8_t *in, uint8_t *out, uint32_t w, uint32_t h)
+      : m_Format(fmt), m_InBuf(in), m_OutBuf(out), m_Width(w), m_Height(h) {}
+  const uint8_t *InBuf() const { return m_InBuf; }
+  uint8_t *OutBuf() const { return m_OutBuf; }
+  uint32_t Width() const { return m_Width; }
+  uint32_t Height() const { return m_Height; }
+  ECompressionFormat Format() const { return m_Format; }
+
+ private:
+  ECompressionFormat m_Format;
+  const uint8_t *m_InBuf;
+  uint8_t *m_OutBuf;
+  uint32_t m_Width, m_Height;
+};
+
+// CompressedImage::GetCompressedSize (Core/src/CompressedImage.cpp:136-149).
+inline size_t GetCompressedSize(uint32_t w, uint32_t h, ECompressionFormat fmt) {
+  size_t n = 0;
+  return fh_compressed_size((uint32_t)fmt, w, h, &n) == FH_OK ? n : 0;
+}
+
+}  // namespace FasTC
+
+namespace detail {
+
+inline void check(fh_status s) {
+  if (s != FH_OK) throw Error(s, fh_last_error());
+}
+
+inline fh_job make_job(const FasTC::CompressionJob &cj) {
+  fh_job j = fh_job();
+  j.format = (uint32_t)cj.Format();
+  j.in = cj.InBuf();
+  j.out = cj.OutBuf();
+  j.out_size = FasTC::GetCompressedSize(cj.Width(), cj.Height(), cj.Format());
+  j.width = cj.Width();
+  j.height = cj.Height();
+  j.x_start = cj.XStart();
+  j.y_start = cj.YStart();
+  j.x_end = cj.XEnd();
+  j.y_end = cj.YEnd();
+  j.quality = 50;
+  j.block_modes = 0xFF;
+  return j;
+}
+
+}  // namespace detail
+
+namespace BPTCC {
+
+enum ErrorMetric { eErrorMetric_Uniform, eErrorMetric_Nonuniform, kNumErrorMetrics };
+
+struct CompressionSettings {
+  uint32_t m_BlockModes = 0xFF;
+  ErrorMetric m_ErrorMetric = eErrorMetric_Uniform;
+  uint32_t m_NumSimulatedAnnealingSteps = 50;
+  // Key of the annealing random stream (the reference seeds a process-wide
+  // LCG with time(NULL), Compressor.cpp:358).
+  uint64_t m_Seed = 0;
+};
+
+inline void Compress(const FasTC::CompressionJob &cj,
+                     CompressionSettings settings = CompressionSettings()) {
+  fh_job j = detail::make_job(cj);
+  j.quality = settings.m_NumSimulatedAnnealingSteps;
+  j.block_modes = settings.m_BlockModes;
+  j.error_metric = (uint32_t)settings.m_ErrorMetric;
+  j.seed = settings.m_Seed;
+  detail::check(fh_compress(&j));
+}
+
+inline void Decompress(const FasTC::DecompressionJob &dj) {
+  const size_t n = FasTC::GetCompressedSize(dj.Width(), dj.Height(), dj.Format());
+  detail::check(fh_decompress((uint32_t)dj.Format(), dj.InBuf(), n, dj.OutBuf(), dj.Width(),
+                              dj.Height()));
+}
+
+}  // namespace BPTCC
+
+namespace DXTC {
+inline void CompressImageDXT1(const FasTC::CompressionJob &cj) {
+  fh_job j = detail::make_job(cj);
+  j.format = FH_FORMAT_DXT1;
+  detail::check(fh_compress(&j));
+}
+inline void CompressImageDXT5(const FasTC::CompressionJob &cj) {
+  fh_job j = detail::make_job(cj);
+  j.format = FH_FORMAT_DXT5;
+  detail::check(fh_compress(&j));
+}
+}  // namespace DXTC
+
+namespace ETCC {
+inline void Compress_RG(const FasTC::CompressionJob &cj) {
+  fh_job j = detail::make_job(cj);
+  j.format = FH_FORMAT_ETC1;
+  detail::check(fh_compress(&j));
+}
+}  // namespace ETCC
+
+// SCompressionSettings (Core/include/FasTC/TexComp.h:30-76) with the same
+// defaults as its constructor (Core/src/TexComp.cpp:77-85) plus bUseGPU.
+struct SCompressionSettings {
+  FasTC::ECompressionFormat format = FasTC::eCompressionFormat_BPTC;
+  bool bUseSIMD = false;
+  int iNumThreads = 1;   // ignored: the GPU path is one launch
+  int iQuality = 50;     // BPTC annealing steps, clamped to [0, 256]
+  int iNumCompressions = 1;
+  int iJobSize = 0;      // ignored
+  bool bUseAtomics = false;
+  bool bUsePVRTexLib = false;
+  bool bUseNVTT = false;
+  bool bUseGPU = true;   // route the encode to the MI355X kernels
+  uint64_t seed = 0;     // annealing stream key
+};
+
+// CompressImageData (Core/src/TexComp.cpp:427-525): validates, encodes the
+// whole image and returns false with a "TexComp -- ..." message on failure.
+inline bool CompressImageData(const unsigned char *data, unsigned int width, unsigned int height,
+                              unsigned char *cmpData, unsigned int cmpDataSz,
+                              const SCompressionSettings &settings) {
+  if (!settings.bUseGPU) {
+    fprintf(stderr, "TexComp -- %s\n", "this library only implements the GPU path (bUseGPU)");
+    return false;
+  }
+  if (settings.bUseSIMD) {
+    fprintf(stderr, "TexComp -- %s\n", "Platform does not support SIMD!\n");
+    return false;
+  }
+  fh_job j = fh_job();
+  j.format = (uint32_t)settings.format;
+  j.in = data;
+  j.out = cmpData;
+  j.out_size = cmpDataSz;
+  j.width = width;
+  j.height = height;
+  j.x_end = width;
+  j.y_end = height;
+  int q = settings.iQuality;
+  j.quality = (uint32_t)(q < 0 ? 0 : (q > 256 ? 256 : q));
+  j.block_modes = 0xFF;
+  j.seed = settings.seed;
+  for (int i = 0; i < (settings.iNumCompressions > 0 ? settings.iNumCompressions : 1); i++) {
+    const fh_status s = fh_compress(&j);
+    if (s != FH_OK) {
+      fprintf(stderr, "TexComp -- %s\n", fh_last_error());
+      return false;
+    }
+  }
+  return true;
+}
+
+}  // namespace fasthip
+
+#endif  // FASTHIP_FASTC_HPP_

@@ -1,0 +1,84 @@
+// Drives the C++ FasTC-style API (include/fasthip/fastc.hpp) the way a FasTC
+// caller would.  Usage:
+//   facade_test validate                 -> checks error behaviour (no GPU needed)
+//   facade_test encode in.raw w h outdir -> BPTC(q=0), DXT1, DXT5, ETC1 via the
+//                                           codec entry points and CompressImageData
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "fasthip/fastc.hpp"
+
+using namespace fasthip;
+
+static int validate() {
+  std::vector<uint8_t> img(8 * 8 * 4, 0), out(8 * 8, 0);
+  SCompressionSettings s;
+  int fails = 0;
+  // width not a multiple of the block size (TexComp.cpp:474-476)
+  if (CompressImageData(img.data(), 6, 8, out.data(), (unsigned)out.size(), s)) fails++;
+  // output buffer too small (TexComp.cpp:493-495)
+  if (CompressImageData(img.data(), 8, 8, out.data(), 10, s)) fails++;
+  // no data (TexComp.cpp:454-457)
+  if (CompressImageData(nullptr, 8, 8, out.data(), (unsigned)out.size(), s)) fails++;
+  SCompressionSettings cpu = s;
+  cpu.bUseGPU = false;
+  if (CompressImageData(img.data(), 8, 8, out.data(), (unsigned)out.size(), cpu)) fails++;
+  if (FasTC::GetCompressedSize(8, 8, FasTC::eCompressionFormat_DXT1) != 32) fails++;
+  if (!fh_device_available()) {
+    // a valid job must report the missing device, never fall back to CPU
+    FasTC::CompressionJob cj(FasTC::eCompressionFormat_BPTC, img.data(), out.data(), 8, 8);
+    bool threw = false;
+    try {
+      BPTCC::Compress(cj);
+    } catch (const Error &e) {
+      threw = e.status == FH_ERR_NO_DEVICE;
+    }
+    if (!threw) fails++;
+  }
+  printf("validate: %d failures\n", fails);
+  return fails ? 1 : 0;
+}
+
+static void save(const std::string &path, const std::vector<uint8_t> &v) {
+  FILE *f = fopen(path.c_str(), "wb");
+  fwrite(v.data(), 1, v.size(), f);
+  fclose(f);
+}
+
+static int encode(const char *in, unsigned w, unsigned h, const std::string &dir) {
+  std::vector<uint8_t> img((size_t)w * h * 4);
+  FILE *f = fopen(in, "rb");
+  if (!f || fread(img.data(), 1, img.size(), f) != img.size()) return 2;
+  fclose(f);
+  const size_t n16 = (size_t)w * h, n8 = n16 / 2;
+  std::vector<uint8_t> bptc(n16), d1(n8), d5(n16), e1(n8), via(n16);
+  BPTCC::CompressionSettings bs;
+  bs.m_NumSimulatedAnnealingSteps = 0;
+  BPTCC::Compress(FasTC::CompressionJob(FasTC::eCompressionFormat_BPTC, img.data(), bptc.data(), w, h), bs);
+  DXTC::CompressImageDXT1(FasTC::CompressionJob(FasTC::eCompressionFormat_DXT1, img.data(), d1.data(), w, h));
+  DXTC::CompressImageDXT5(FasTC::CompressionJob(FasTC::eCompressionFormat_DXT5, img.data(), d5.data(), w, h));
+  ETCC::Compress_RG(FasTC::CompressionJob(FasTC::eCompressionFormat_ETC1, img.data(), e1.data(), w, h));
+  SCompressionSettings s;
+  s.format = FasTC::eCompressionFormat_BPTC;
+  s.iQuality = 0;
+  if (!CompressImageData(img.data(), w, h, via.data(), (unsigned)via.size(), s)) return 3;
+  save(dir + "/bptc.bin", bptc);
+  save(dir + "/dxt1.bin", d1);
+  save(dir + "/dxt5.bin", d5);
+  save(dir + "/etc1.bin", e1);
+  save(dir + "/bptc_via_cid.bin", via);
+  printf("encode ok\n");
+  return 0;
+}
+
+int main(int argc, char **argv) {
+  if (argc >= 2 && !strcmp(argv[1], "validate")) return validate();
+  if (argc >= 6 && !strcmp(argv[1], "encode"))
+    return encode(argv[2], (unsigned)atoi(argv[3]), (unsigned)atoi(argv[4]), argv[5]);
+  fprintf(stderr, "usage: facade_test validate | encode in.raw w h outdir\n");
+  return 2;
+}
