@@ -26,6 +26,7 @@ BLOCK_BYTES = {0: 8, 1: 16, 2: 8, 3: 16}
 
 FH_OK = 0
 FH_FLAG_CALLER_WATERMARK = 1
+FH_FLAG_NO_Q0_GUARD = 2
 STATUS_NAMES = {0: "FH_OK", 1: "FH_ERR_INVALID_ARGUMENT", 2: "FH_ERR_NO_DATA",
                 3: "FH_ERR_BLOCK_DIMS", 4: "FH_ERR_UNKNOWN_FORMAT",
                 5: "FH_ERR_BUFFER_TOO_SMALL", 6: "FH_ERR_UNSUPPORTED",
@@ -77,12 +78,18 @@ def lib():
         L.fh_compressed_size.restype = ctypes.c_int
         L.fh_last_error.restype = ctypes.c_char_p
         L.fh_device_available.restype = ctypes.c_int
+        L.fh_set_kernel_timing.argtypes = [ctypes.c_int]
+        L.fh_set_kernel_timing.restype = ctypes.c_int
+        L.fh_kernel_time.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_uint32)]
+        L.fh_kernel_time.restype = ctypes.c_int
         _lib = L
     return _lib
 
 
 EXPORTED_SYMBOLS = ["fh_compress", "fh_compress_device", "fh_decompress",
-                    "fh_compressed_size", "fh_last_error", "fh_device_available"]
+                    "fh_compressed_size", "fh_last_error", "fh_device_available",
+                    "fh_set_kernel_timing", "fh_kernel_time"]
 
 
 def _check(status):
@@ -101,7 +108,7 @@ def compressed_size(fmt, width, height):
 
 
 def _job(fmt, inp, out, out_size, width, height, rect, quality, seed, block_modes,
-         wm_counter):
+         wm_counter, q0_guard=True):
     x0, y0, x1, y1 = rect if rect is not None else (0, 0, width, height)
     j = FhJob()
     j.format = fmt
@@ -114,39 +121,56 @@ def _job(fmt, inp, out, out_size, width, height, rect, quality, seed, block_mode
     j.block_modes = block_modes
     j.error_metric = 0
     j.seed = seed
+    j.flags = 0 if q0_guard else FH_FLAG_NO_Q0_GUARD
     if wm_counter is not None:
         j.wm_counter = ctypes.pointer(wm_counter)
-        j.flags = FH_FLAG_CALLER_WATERMARK
+        j.flags |= FH_FLAG_CALLER_WATERMARK
     return j
 
 
 def compress(img, fmt=eCompressionFormat_BPTC, quality=0, seed=0, block_modes=0xFF,
-             rect=None, out=None, wm_counter=None):
+             rect=None, out=None, wm_counter=None, q0_guard=True):
     """Encode a (h, w) uint32 RGBA8 image (R in the low byte) on the GPU.
 
     Mirrors CompressImageData: returns the compressed bytes (uint8 array of
     CompressedImage::GetCompressedSize bytes).  `wm_counter` is a
     ctypes.c_uint32 holding the BC7 solid-block watermark counter (the
     reference's gWMVal); it is updated in place.  Without it the library's
-    process-wide counter is used, like the reference."""
+    process-wide counter is used, like the reference.  `q0_guard=False`
+    drops the BPTC quality guard (FH_FLAG_NO_Q0_GUARD)."""
     img = np.ascontiguousarray(img, dtype=np.uint32)
     h, w = img.shape
     size = compressed_size(fmt, w, h)
     if out is None:
         out = np.zeros(size, dtype=np.uint8)
     j = _job(fmt, img.ctypes.data, out.ctypes.data, out.nbytes, w, h, rect, quality, seed,
-             block_modes, wm_counter)
+             block_modes, wm_counter, q0_guard)
     _check(lib().fh_compress(ctypes.byref(j)))
     return out
 
 
 def compress_device(in_ptr, out_ptr, out_size, width, height, fmt=eCompressionFormat_BPTC,
-                    quality=0, seed=0, block_modes=0xFF, stream=0, wm_counter=None):
+                    quality=0, seed=0, block_modes=0xFF, stream=0, wm_counter=None,
+                    q0_guard=True):
     """Encode device-resident data (raw device pointers, e.g. torch
     tensor.data_ptr()) asynchronously on `stream` (a hipStream_t as int)."""
     j = _job(fmt, in_ptr, out_ptr, out_size, width, height, None, quality, seed, block_modes,
-             wm_counter)
+             wm_counter, q0_guard)
     _check(lib().fh_compress_device(ctypes.byref(j), ctypes.c_void_p(stream)))
+
+
+def set_kernel_timing(enable):
+    """Bracket every library kernel with HIP events on its stream (clears the
+    totals)."""
+    _check(lib().fh_set_kernel_timing(1 if enable else 0))
+
+
+def kernel_time(kernel):
+    """(total milliseconds, launches) of one kernel since timing was enabled."""
+    ms = ctypes.c_double(0)
+    n = ctypes.c_uint32(0)
+    _check(lib().fh_kernel_time(kernel.encode(), ctypes.byref(ms), ctypes.byref(n)))
+    return ms.value, n.value
 
 
 def decompress(data, width, height, fmt=eCompressionFormat_BPTC):

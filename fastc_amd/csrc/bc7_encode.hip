@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "bc7_block_decode.h"
 #include "internal.h"
 
 namespace fh {
@@ -1310,7 +1311,7 @@ __global__ void __launch_bounds__(64) setup_kernel(BlockRange r, Bc7Params prm, 
       rec.cidx_lo = (uint32_t)cidx;
       rec.cidx_hi = (uint32_t)(cidx >> 32);
       rec.meta = pbcs | ((uint32_t)k.rot << 8) | ((uint32_t)k.im << 10);
-      rec.err = active ? 0xFFFFFFFEu : total;  // final unless annealing follows
+      rec.err = total;  // the quality-0 result; annealing overwrites active ones
       w.rec[(size_t)rel * 16 + c] = rec;
       if (active) {
         st.info = (uint32_t)k.mode | ((uint32_t)k.rot << 3) | ((uint32_t)k.im << 5) |
@@ -1722,7 +1723,10 @@ __device__ void pack_mode(int mode, int shape, const CandRec &rec, BitWriter &w)
   }
 }
 
-__global__ void __launch_bounds__(256) pack_kernel(BlockRange r, Bc7Scratch scr, Work w) {
+// dst == nullptr writes the block into the job's output; otherwise into
+// dst[rel] (the quality-0 copy the guard compares against).
+__global__ void __launch_bounds__(256) pack_kernel(BlockRange r, Bc7Scratch scr, Work w,
+                                                   uint4 *dst) {
   const uint32_t rel = blockIdx.x * blockDim.x + threadIdx.x;
   if (rel >= r.num_blocks) return;
   const uint32_t b = r.first_block + rel;
@@ -1777,7 +1781,41 @@ __global__ void __launch_bounds__(256) pack_kernel(BlockRange r, Bc7Scratch scr,
   v.y = (uint32_t)(bw.lo >> 32);
   v.z = (uint32_t)bw.hi;
   v.w = (uint32_t)(bw.hi >> 32);
-  reinterpret_cast<uint4 *>(r.out)[b] = v;
+  if (dst) dst[rel] = v;
+  else reinterpret_cast<uint4 *>(r.out)[b] = v;
+}
+
+// Quality guard (no counterpart in the reference): where the annealed block
+// decodes (reference decoder) to a larger premultiplied RGB error than the
+// quality-0 block, the quality-0 block is written instead.  The error is
+// sum((a*c - a'*c')^2) over pixels and RGB, Image::ComputePSNR's per-pixel
+// term (Base/src/Image.cpp:205-255) scaled by 255^2 to stay an exact integer.
+__device__ __forceinline__ uint64_t premul_err(const uint32_t src[16], const uint32_t dec[16]) {
+  uint64_t e = 0;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int a = (int)(src[i] >> 24), b = (int)(dec[i] >> 24);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const int d = a * (int)chan(src[i], c) - b * (int)chan(dec[i], c);
+      e += (uint64_t)((int64_t)d * d);
+    }
+  }
+  return e;
+}
+
+__global__ void __launch_bounds__(256) guard_kernel(BlockRange r, const uint4 *q0) {
+  const uint32_t rel = blockIdx.x * blockDim.x + threadIdx.x;
+  if (rel >= r.num_blocks) return;
+  const uint32_t b = r.first_block + rel;
+  uint4 *out = reinterpret_cast<uint4 *>(r.out);
+  const uint4 a = out[b], z = q0[rel];
+  if (a.x == z.x && a.y == z.y && a.z == z.z && a.w == z.w) return;
+  uint32_t px[16], da[16], dz[16];
+  load_block(r, rel, px);
+  bc7dec::decode_block(a, da);
+  bc7dec::decode_block(z, dz);
+  if (premul_err(px, dz) < premul_err(px, da)) out[b] = z;
 }
 
 __global__ void reset_kernel(uint32_t *counters, int n) {
@@ -1877,7 +1915,7 @@ hipError_t bc7_upload_tables(int device) {
 size_t bc7_work_bytes(uint32_t num_blocks) {
   const size_t n = num_blocks;
   return 256 + n * 4 + n * 16 * sizeof(bc7::CandRec) + n * 19 * 4 + 64 * 4 +
-         n * 14 * sizeof(bc7::AState) + 4096;
+         n * 14 * sizeof(bc7::AState) + n * 16 + 4096;
 }
 
 hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch &s, void *work,
@@ -1905,6 +1943,7 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   w.queue = reinterpret_cast<uint32_t *>(carve((size_t)n * 19 * 4));
   w.astate = reinterpret_cast<bc7::AState *>(carve((size_t)n * 14 * sizeof(bc7::AState)));
   w.astate_cap = n * 14;
+  uint4 *q0 = reinterpret_cast<uint4 *>(carve((size_t)n * 16));
   static const int cap[bc7::kNumTypes] = {1, 1, 1, 1, 1, 2, 8, 4};
   uint32_t off = 0;
   for (int t = 0; t < bc7::kNumTypes; t++) {
@@ -1913,15 +1952,39 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   }
   const uint32_t nchunks = (n + 255) / 256;
   const uint32_t ngroups = (n + 3) / 4;
+  const uint32_t persistent = (uint32_t)g_num_cus[dev] * 16;
+  const dim3 pack_grid((n + 255) / 256);
+  const bool guard = p.quality > 0 && p.q0_guard;
+  int t = timing_begin(stream);
   hipLaunchKernelGGL(bc7::reset_kernel, dim3(1), dim3(64), 0, stream, w.counters, 64);
   hipLaunchKernelGGL(bc7::solid_scan_kernel, dim3(nchunks), dim3(256), 0, stream, r, s);
   hipLaunchKernelGGL(bc7::chunk_scan_kernel, dim3(1), dim3(1024), 0, stream, s);
+  timing_end(t, kTagBc7Scan, stream);
+  t = timing_begin(stream);
   hipLaunchKernelGGL(bc7::select_kernel, dim3(ngroups), dim3(64), 0, stream, r, p, w);
-  const uint32_t persistent = (uint32_t)g_num_cus[dev] * 16;
+  timing_end(t, kTagBc7Select, stream);
+  t = timing_begin(stream);
   hipLaunchKernelGGL(bc7::setup_kernel, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
-  if (p.quality > 0)
+  timing_end(t, kTagBc7Setup, stream);
+  if (guard) {
+    t = timing_begin(stream);
+    hipLaunchKernelGGL(bc7::pack_kernel, pack_grid, dim3(256), 0, stream, r, s, w, q0);
+    timing_end(t, kTagBc7Pack, stream);
+  }
+  if (p.quality > 0) {
+    t = timing_begin(stream);
     hipLaunchKernelGGL(bc7::anneal_kernel, dim3(persistent), dim3(64), 0, stream, r, p, w);
-  hipLaunchKernelGGL(bc7::pack_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, r, s, w);
+    timing_end(t, kTagBc7Anneal, stream);
+  }
+  t = timing_begin(stream);
+  hipLaunchKernelGGL(bc7::pack_kernel, pack_grid, dim3(256), 0, stream, r, s, w,
+                     (uint4 *)nullptr);
+  timing_end(t, kTagBc7Pack, stream);
+  if (guard) {
+    t = timing_begin(stream);
+    hipLaunchKernelGGL(bc7::guard_kernel, pack_grid, dim3(256), 0, stream, r, q0);
+    timing_end(t, kTagBc7Guard, stream);
+  }
   return hipGetLastError();
 }
 

@@ -197,6 +197,7 @@ fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8
       p.block_modes = j->block_modes ? (j->block_modes & 0xFFu) : 0xFFu;
       p.seed = j->seed;
       p.block_id_base = 0;
+      p.q0_guard = (j->flags & FH_FLAG_NO_Q0_GUARD) ? 0u : 1u;
       // jobs larger than one call's work area run as consecutive calls on the
       // stream; the watermark counter carries across them in order
       for (uint32_t done = 0; done < count; done += chunk) {

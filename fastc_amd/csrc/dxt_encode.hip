@@ -443,10 +443,12 @@ hipError_t dxt_encode(const BlockRange &r, int dxt5, hipStream_t stream) {
     g_dxt_tables[dev] = 1;
   }
   const uint32_t grid = (r.num_blocks + 255) / 256;
+  const int t = timing_begin(stream);
   if (dxt5)
     hipLaunchKernelGGL(dxt::encode_kernel<1>, dim3(grid), dim3(256), 0, stream, r);
   else
     hipLaunchKernelGGL(dxt::encode_kernel<0>, dim3(grid), dim3(256), 0, stream, r);
+  timing_end(t, kTagDxt, stream);
   return hipGetLastError();
 }
 

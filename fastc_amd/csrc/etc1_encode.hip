@@ -514,7 +514,9 @@ hipError_t etc1_encode(const BlockRange &r, hipStream_t stream) {
     g_tables[dev] = d;
   }
   const uint32_t grid = (r.num_blocks + 255) / 256;
+  const int t = timing_begin(stream);
   hipLaunchKernelGGL(etc::encode_kernel, dim3(grid), dim3(256), 0, stream, r, g_tables[dev]);
+  timing_end(t, kTagEtc1, stream);
   return hipGetLastError();
 }
 

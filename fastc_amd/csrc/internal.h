@@ -23,6 +23,7 @@ struct Bc7Params {
   uint32_t block_modes;  // mask of allowed BC7 modes
   uint64_t seed;         // annealing stream key
   uint64_t block_id_base;  // added to the raster block index to key the stream
+  uint32_t q0_guard;     // keep the quality-0 block where annealing decodes worse
 };
 
 // Device scratch needed by the BC7 watermark prefix scan.
@@ -59,5 +60,23 @@ void build_bc7_dxt6_table(uint8_t out[256][2][3]);
 void build_bc7_single_color_table(uint32_t *out);  // kSingleColorEntries entries
 
 constexpr int kSingleColorEntries = 5 * 5 * 4 * 256;
+
+// Kernel timing (timing.cpp).  timing_begin records a start event on the
+// stream when timing is enabled and returns its slot (-1 otherwise);
+// timing_end records the matching stop event for kernel `tag`.
+enum KernelTag {
+  kTagBc7Scan,
+  kTagBc7Select,
+  kTagBc7Setup,
+  kTagBc7Anneal,
+  kTagBc7Pack,
+  kTagBc7Guard,
+  kTagDxt,
+  kTagEtc1,
+  kTagBc7Decode,
+  kNumKernelTags
+};
+int timing_begin(hipStream_t stream);
+void timing_end(int slot, int tag, hipStream_t stream);
 
 }  // namespace fh

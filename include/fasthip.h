@@ -25,6 +25,8 @@
  *                        (Core/src/TexComp.cpp:157-159) before returning false.
  *   fh_device_available  no reference counterpart (the reference is CPU only);
  *                        lets callers choose this backend.
+ *   fh_set_kernel_timing, fh_kernel_time
+ *                        per-kernel timing for benchmarks (no counterpart).
  *
  * All functions are thread-safe.  Buffers are owned by the caller; no pointer
  * is retained after a call returns.
@@ -66,7 +68,12 @@ enum {
 enum {
   /* Keep the solid-block watermark counter in *wm_counter (caller-owned)
    * instead of the library's process-wide counter. */
-  FH_FLAG_CALLER_WATERMARK = 1u << 0
+  FH_FLAG_CALLER_WATERMARK = 1u << 0,
+  /* BPTC with quality > 0: skip the quality guard.  By default a block whose
+   * annealed encoding decodes to a larger (premultiplied-alpha RGB) error
+   * than its quality-0 encoding is written as the quality-0 encoding, so no
+   * block is worse than at quality 0; the reference has no such guard. */
+  FH_FLAG_NO_Q0_GUARD = 1u << 1
 };
 
 /* One compression job: FasTC::CompressionJob plus BPTC settings. */
@@ -98,6 +105,16 @@ fh_status fh_compressed_size(uint32_t format, uint32_t width, uint32_t height, s
 
 const char *fh_last_error(void);
 int fh_device_available(void);
+
+/* Kernel timing for benchmarks; no reference counterpart (the reference's
+ * CompressImageData reports only a whole-job wall time, TexComp.cpp:517).
+ * While enabled, every kernel the library launches is bracketed by HIP
+ * events on its own stream.  Enabling or disabling clears the totals.
+ * fh_kernel_time waits for the recorded events and returns the summed time
+ * and launch count of one kernel: "bc7_scan", "bc7_select", "bc7_setup",
+ * "bc7_anneal", "bc7_pack", "bc7_guard", "dxt", "etc1", "bc7_decode". */
+fh_status fh_set_kernel_timing(int enable);
+fh_status fh_kernel_time(const char *kernel, double *total_ms, uint32_t *launches);
 
 #ifdef __cplusplus
 }

@@ -171,6 +171,15 @@ static void build_tables(void) {
 const uint8_t *ora_table_mode5(void) { build_tables(); return &gMode5[0][0]; }
 const uint8_t *ora_table_dxt6(void) { build_tables(); return &gDxt6[0][0][0]; }
 
+void ora_table_partitions(uint16_t shape2[64], uint16_t shape3[128], uint8_t anchor2[64],
+                          uint8_t anchor3[128], uint32_t interp[128]) {
+  memcpy(shape2, kShape2, sizeof(kShape2));
+  memcpy(shape3, kShape3, sizeof(kShape3));
+  memcpy(anchor2, kAnchor2, sizeof(kAnchor2));
+  memcpy(anchor3, kAnchor3, sizeof(kAnchor3));
+  memcpy(interp, kInterp, sizeof(kInterp));
+}
+
 /* ------------------------------------------------------------------------ */
 /* Vector helpers: FasTC::VectorBase<float,4> semantics (VectorBase.h).     */
 /* ------------------------------------------------------------------------ */
@@ -718,6 +727,25 @@ static double single_color(const cmode *m, v4 p, v4 *p1, v4 *p2, uint8_t *best_p
     }
   }
   return best;
+}
+
+double ora_single_color(int mode, int idx_mode, const float p[4], float p1[4], float p2[4],
+                        int *pbc) {
+  cmode m;
+  memset(&m, 0, sizeof(m));
+  m.a = &kModes[mode & 7];
+  m.idx_mode = idx_mode;
+  m.metric = v4_splat(1.0f);
+  v4 pv, a, b;
+  for (int k = 0; k < 4; k++) pv.v[k] = p[k];
+  uint8_t bp = 0;
+  const double e = single_color(&m, pv, &a, &b, &bp);
+  for (int k = 0; k < 4; k++) {
+    p1[k] = a.v[k];
+    p2[k] = b.v[k];
+  }
+  *pbc = bp;
+  return e;
 }
 
 /* ChangePointForDir{Without,With}PbitChange (Compressor.cpp:364-418). */
@@ -1431,6 +1459,8 @@ static void compress_clusters(const shape_sel *sel, const uint32_t px[16],
   }
 }
 
+static void decode_block(const uint8_t blk[16], uint32_t px[16], int spec);
+
 void ora_bc7_compress_block(const uint32_t px[16], uint8_t out[16], const ora_bc7_settings *s,
                             uint32_t wm_index, uint64_t block_id, ora_bc7_block_stats *st) {
   if (st) {
@@ -1459,6 +1489,23 @@ void ora_bc7_compress_block(const uint32_t px[16], uint8_t out[16], const ora_bc
   shape_sel sel = box_selection(px);
   sel.modes &= s->block_modes;
   compress_clusters(&sel, px, s, block_id, out, st);
+  if (st) st->q0_kept = 0;
+  if (s->q0_guard && s->quality > 0) {
+    ora_bc7_settings s0 = *s;
+    s0.quality = 0;
+    uint8_t out0[16];
+    memset(out0, 0, 16);
+    compress_clusters(&sel, px, &s0, block_id, out0, NULL);
+    if (memcmp(out0, out, 16) != 0) {
+      uint32_t d0[16], d1[16];
+      decode_block(out0, d0, 0);
+      decode_block(out, d1, 0);
+      if (ora_block_premul_err(px, d0) < ora_block_premul_err(px, d1)) {
+        memcpy(out, out0, 16);
+        if (st) st->q0_kept = 1;
+      }
+    }
+  }
 }
 
 /* BPTCC::Compress (Compressor.cpp:1473-1520) with GetBlock (1460-1466). */
@@ -1646,6 +1693,19 @@ double ora_premul_sse(const uint8_t *a, const uint8_t *b, uint32_t npix) {
     }
   }
   return mse;
+}
+
+uint64_t ora_block_premul_err(const uint32_t src[16], const uint32_t dec[16]) {
+  uint64_t e = 0;
+  for (int i = 0; i < 16; i++) {
+    const int64_t a = (int64_t)(src[i] >> 24), b = (int64_t)(dec[i] >> 24);
+    for (int c = 0; c < 3; c++) {
+      const int64_t d = a * (int64_t)((src[i] >> (8 * c)) & 0xFF) -
+                        b * (int64_t)((dec[i] >> (8 * c)) & 0xFF);
+      e += (uint64_t)(d * d);
+    }
+  }
+  return e;
 }
 
 double ora_psnr(const uint8_t *a, const uint8_t *b, uint32_t width, uint32_t height) {

@@ -37,6 +37,10 @@ typedef struct {
   int      rng_kind;      /* ORA_RNG_* */
   uint32_t *lcg_state;    /* ORA_RNG_LCG: the shared g_seed */
   uint64_t seed;          /* ORA_RNG_COUNTER: job seed */
+  /* Quality guard (not in the reference): with quality > 0, a block whose
+   * annealed encoding decodes (reference decoder) to a larger premultiplied
+   * RGB error than its quality-0 encoding is written as the quality-0 one. */
+  int      q0_guard;
 } ora_bc7_settings;
 
 /* Per-block statistics (mode chosen, path) for tests. */
@@ -45,6 +49,7 @@ typedef struct {
   int mode;          /* mode written */
   double best_error; /* internal error of the chosen mode (searched path) */
   double mode_error[8]; /* min internal error per mode, -1 if not tried */
+  int q0_kept;       /* the quality guard replaced the annealed encoding */
 } ora_bc7_block_stats;
 
 /* Compress one 4x4 block (16 RGBA8 pixels, R in the low byte).
@@ -83,6 +88,11 @@ double ora_psnr(const uint8_t *a, const uint8_t *b, uint32_t width,
  * the numerator of ComputePSNR's mse (Image.cpp:244-248). */
 double ora_premul_sse(const uint8_t *a, const uint8_t *b, uint32_t npix);
 
+/* Premultiplied RGB error of one block, scaled by 255^2 so that it is an
+ * exact integer: sum over pixels and RGB of (a*c - a'*c')^2.  Orders blocks
+ * the way Image::ComputePSNR's per-pixel term does. */
+uint64_t ora_block_premul_err(const uint32_t src[16], const uint32_t dec[16]);
+
 /* Building blocks exported for the function-level tests. */
 uint8_t ora_quantize_channel(uint8_t val, uint8_t mask, int pbit);
 uint32_t ora_to_pixel(const float v[4], uint32_t mask, int pbit);
@@ -96,6 +106,15 @@ void ora_bc7_box_selection(const uint32_t px[16], uint32_t *nshapes,
                            uint32_t *modes);
 const uint8_t *ora_table_mode5(void);   /* [256][2] 7-bit endpoint pairs */
 const uint8_t *ora_table_dxt6(void);    /* [256][2][3] */
+/* Partition masks (Shapes.h:23-50), anchors (AnchorTables.h:22-45) and
+ * interpolation weights (Compressor.cpp:159-168), copied out flat. */
+void ora_table_partitions(uint16_t shape2[64], uint16_t shape3[128], uint8_t anchor2[64],
+                          uint8_t anchor3[128], uint32_t interp[128]);
+/* CompressionMode::CompressSingleColor (Compressor.cpp:252-353) for one
+ * mode and index mode (rotation 0): endpoints p1/p2 (8-bit values as
+ * floats), p-bit combination and the per-pixel error. */
+double ora_single_color(int mode, int idx_mode, const float p[4], float p1[4], float p2[4],
+                        int *pbc);
 uint32_t ora_counter_rng(uint32_t key, uint32_t n);
 uint32_t ora_fit_key(uint64_t seed, uint64_t block_id, uint32_t fit_tag);
 

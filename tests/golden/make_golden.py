@@ -3,12 +3,15 @@
 Run in a container that has the FasTC tree at /root/reference and the built
 oracle/_ref library (make -C oracle/ref):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [--tables-only]
 
 Fixtures written (data only, no reference source text):
-  bc7_tables.json  SHA-256 of the reference's two BC7 lookup tables
-                   (BPTCEncoder/src/BCLookupTables.h: Optimal7CompressBC7Mode5,
-                   Optimal6CompressDXT1) as flat uint8 arrays.
+  bc7_tables.json  SHA-256 of the reference's BC7 tables as flat arrays: the
+                   two lookup tables (BPTCEncoder/src/BCLookupTables.h:
+                   Optimal7CompressBC7Mode5, Optimal6CompressDXT1, uint8), the
+                   partition masks (Shapes.h, uint16), anchors
+                   (AnchorTables.h, uint8) and interpolation weights
+                   (Compressor.cpp kInterpolationValues, uint32).
   dxt_etc.npz      DXT1 / DXT5 / ETC1 output of the reference encoders
                    (stb_dxt.h, rg_etc1.cpp built from the reference tree) for a
                    64x64 generator image (seed 2, hard edges).
@@ -30,10 +33,25 @@ sys.path.insert(0, ROOT)
 REF = os.environ.get("FASTC_REF", "/root/reference")
 
 
-def parse_table(text, name):
+def parse_table(text, name, dtype=np.uint8):
     i = text.index(name)
     body = text[text.index("=", i):text.index("};", i)]
-    return np.array([int(x, 0) for x in re.findall(r"0x[0-9a-fA-F]+|\d+", body)], dtype=np.uint8)
+    return np.array([int(x, 0) for x in re.findall(r"0x[0-9a-fA-F]+|\d+", body)], dtype=dtype)
+
+
+def partition_tables(ref):
+    """The reference's partition masks, anchors and interpolation weights as
+    flat arrays (Shapes.h, AnchorTables.h, Compressor.cpp)."""
+    shapes = open(os.path.join(ref, "BPTCEncoder", "include", "FasTC", "Shapes.h")).read()
+    anchors = open(os.path.join(ref, "BPTCEncoder", "src", "AnchorTables.h")).read()
+    comp = open(os.path.join(ref, "BPTCEncoder", "src", "Compressor.cpp")).read()
+    return {
+        "shape2": parse_table(shapes, "kShapeMask2[kNumShapes2]", np.uint16),
+        "shape3": parse_table(shapes, "kShapeMask3[kNumShapes3][2]", np.uint16),
+        "anchor2": parse_table(anchors, "kAnchorIdx2[BPTCC::kNumShapes2]"),
+        "anchor3": parse_table(anchors, "kAnchorIdx3[2][BPTCC::kNumShapes3]"),
+        "interp": parse_table(comp, "kInterpolationValues[4][16][2]", np.uint32),
+    }
 
 
 def ref_lib():
@@ -64,9 +82,15 @@ def main():
     m5 = parse_table(text, "Optimal7CompressBC7Mode5[256][2]")
     d6 = parse_table(text, "Optimal6CompressDXT1[256][2][3]")
     assert m5.size == 512 and d6.size == 1536
+    tables = {"mode5_sha256": hashlib.sha256(m5.tobytes()).hexdigest(),
+              "dxt6_sha256": hashlib.sha256(d6.tobytes()).hexdigest()}
+    for name, arr in partition_tables(REF).items():
+        tables[name + "_sha256"] = hashlib.sha256(arr.tobytes()).hexdigest()
     with open(os.path.join(HERE, "bc7_tables.json"), "w") as f:
-        json.dump({"mode5_sha256": hashlib.sha256(m5.tobytes()).hexdigest(),
-                   "dxt6_sha256": hashlib.sha256(d6.tobytes()).hexdigest()}, f, indent=1)
+        json.dump(tables, f, indent=1)
+    if "--tables-only" in sys.argv:
+        print("table fixture written")
+        return
 
     L = ref_lib()
     img = synth.generate(64, 64, 2, edges=True)

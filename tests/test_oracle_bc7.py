@@ -296,3 +296,140 @@ def test_psnr_closed_form(oracle):
     c[:] = 0x00808080  # alpha 0 hides RGB entirely
     d = np.full((4, 4), 0x00000000, dtype=np.uint32)
     assert oracle.premul_sse(c, d) == 0.0
+
+
+def test_block_premul_err_closed_form(oracle):
+    """The guard's block error is Image::ComputePSNR's per-pixel term
+    (Base/src/Image.cpp:205-255) times 255^2, exactly."""
+    rng = np.random.default_rng(8)
+    for _ in range(50):
+        a = rng.integers(0, 2**32, 16, dtype=np.uint64).astype(np.uint32)
+        b = rng.integers(0, 2**32, 16, dtype=np.uint64).astype(np.uint32)
+        want = 0
+        for i in range(16):
+            pa, pb = int(a[i]) >> 24, int(b[i]) >> 24
+            for c in range(3):
+                d = pa * ((int(a[i]) >> (8 * c)) & 255) - pb * ((int(b[i]) >> (8 * c)) & 255)
+                want += d * d
+        assert oracle.block_premul_err(a, b) == want
+        assert abs(oracle.premul_sse(a, b) * 255 * 255 - want) <= 1e-6 * max(want, 1)
+
+
+def _dec_ref(oracle, blk):
+    """One block through the reference decoder (BPTCC::Decompress)."""
+    return oracle.bc7_decompress(blk, 4, 4).reshape(16)
+
+
+def test_quality_guard(oracle):
+    """With the guard, no block decodes worse than its quality-0 encoding;
+    blocks the guard keeps are the unguarded annealed blocks."""
+    img = synth.generate(64, 64, 13)
+    blks = synth.blocks(img)
+    kept = 0
+    for b in range(blks.shape[0]):
+        o0, _ = _encode_block(oracle, blks[b], quality=0)
+        og, sg = _encode_block(oracle, blks[b], quality=50, seed=2, block_id=b, q0_guard=True)
+        on, _ = _encode_block(oracle, blks[b], quality=50, seed=2, block_id=b)
+        e0 = oracle.block_premul_err(blks[b], _dec_ref(oracle, o0))
+        eg = oracle.block_premul_err(blks[b], _dec_ref(oracle, og))
+        en = oracle.block_premul_err(blks[b], _dec_ref(oracle, on))
+        assert eg <= e0
+        if sg.q0_kept:
+            assert np.array_equal(og, o0) and en > e0
+            kept += 1
+        else:
+            assert np.array_equal(og, on)
+    assert kept > 0
+
+
+def test_partition_tables_match_reference(oracle):
+    """Partition masks, anchors and interpolation weights equal the
+    reference's (Shapes.h:23-50, AnchorTables.h:22-45,
+    Compressor.cpp:159-168): hash fixture made from the reference text, and
+    element by element when the tree is present."""
+    L = oracle.lib()
+    s2 = np.zeros(64, np.uint16)
+    s3 = np.zeros(128, np.uint16)
+    a2 = np.zeros(64, np.uint8)
+    a3 = np.zeros(128, np.uint8)
+    iv = np.zeros(128, np.uint32)
+    L.ora_table_partitions(s2.ctypes.data, s3.ctypes.data, a2.ctypes.data, a3.ctypes.data,
+                           iv.ctypes.data)
+    got = {"shape2": s2, "shape3": s3, "anchor2": a2, "anchor3": a3, "interp": iv}
+    fx = json.load(open(os.path.join(HERE, "golden", "bc7_tables.json")))
+    for k, v in got.items():
+        assert hashlib.sha256(v.tobytes()).hexdigest() == fx[k + "_sha256"], k
+    if os.path.exists(REF):
+        from tests.golden.make_golden import partition_tables
+        for k, v in partition_tables(REF).items():
+            assert np.array_equal(got[k], v), k
+    # BC7 specification facts independent of both: subset 0 always holds
+    # pixel 0 (the implicit anchor), and each anchor lies in its subset
+    s2, s3 = s2.astype(int), s3.astype(int)
+    a2, a3 = a2.astype(int), a3.astype(int)
+    for sh in range(64):
+        assert not (s2[sh] & 1) and (s2[sh] >> a2[sh]) & 1
+        m1, m2 = int(s3[2 * sh]), int(s3[2 * sh + 1])
+        assert not (m1 & 1) and (m2 & ~m1) == 0
+        assert (m1 >> a3[sh]) & 1 and not (m2 >> a3[sh]) & 1
+        assert (m2 >> a3[64 + sh]) & 1
+
+
+# (color bits, alpha bits, p-bit type 0 none / 1 shared / 2 per endpoint,
+#  colour index bits, alpha index bits) of the eight BC7 modes (BC7 spec)
+_MODES = {0: (4, 0, 2, 3, 0), 1: (6, 0, 1, 3, 0), 2: (5, 0, 0, 2, 0), 3: (7, 0, 2, 2, 0),
+          4: (5, 6, 0, 2, 3), 5: (7, 8, 0, 2, 2), 6: (7, 7, 2, 4, 0), 7: (5, 5, 2, 2, 0)}
+_W1 = {2: 21, 3: 9, 4: 4}  # weight of index 1 (BC7 spec interpolation tables)
+
+
+def test_single_color_brute_force(oracle):
+    """CompressSingleColor (Compressor.cpp:252-353) against exhaustive
+    enumeration: per channel the distance is the minimum over every grid
+    pair of |interp(index 1) - value|, the p-bit combination minimises the
+    sum of squares (first one on ties) and the endpoints reach the minimum."""
+    import ctypes
+    L = oracle.lib()
+    rng = np.random.default_rng(4)
+    cols = [[0, 0, 0, 0], [255, 255, 255, 255], [1, 128, 254, 7]] + \
+        [list(rng.integers(0, 256, 4)) for _ in range(25)]
+    for mode, (cb, ab, pbt, ib, aib) in _MODES.items():
+        for im in ([0, 1] if mode == 4 else [0]):
+            nbits_idx = ib if im == 0 else aib
+            w = _W1[nbits_idx]
+            combos = [(0, 0)] if pbt == 0 else ([(0, 0), (1, 1)] if pbt == 1 else
+                                                [(0, 0), (0, 1), (1, 0), (1, 1)])
+            for col in cols:
+                p = np.array(col, np.float32)
+                p1 = np.zeros(4, np.float32)
+                p2 = np.zeros(4, np.float32)
+                pbc = ctypes.c_int(0)
+                err = L.ora_single_color(mode, im, p.ctypes.data, p1.ctypes.data,
+                                         p2.ctypes.data, ctypes.byref(pbc))
+                best, best_c, best_d = None, None, None
+                for ci, (pl, ph) in enumerate(combos):
+                    d = []
+                    for ch in range(4):
+                        bits = cb if ch < 3 else ab
+                        val = int(col[ch])
+                        if bits == 0:
+                            d.append(255 - val)
+                            continue
+                        pbl, pbh = (pl, ph) if pbt else (-1, -1)
+                        lo = np.array([_expand(k, bits, pbl) for k in range(1 << bits)])
+                        hi = np.array([_expand(k, bits, pbh) for k in range(1 << bits)])
+                        ip = ((64 - w) * lo[:, None] + w * hi[None, :] + 32) >> 6
+                        d.append(int(np.abs(ip - val).min()))
+                    e = float(sum(x * x for x in d))
+                    if best is None or e < best:
+                        best, best_c, best_d = e, ci, d
+                assert err == best and pbc.value == best_c, (mode, im, col)
+                pl, ph = combos[best_c]
+                for ch in range(4):
+                    bits = cb if ch < 3 else ab
+                    if bits == 0:
+                        assert p1[ch] == 255 and p2[ch] == 255
+                        continue
+                    a, b = int(p1[ch]), int(p2[ch])
+                    assert a in [_expand(k, bits, pl if pbt else -1) for k in range(1 << bits)]
+                    assert b in [_expand(k, bits, ph if pbt else -1) for k in range(1 << bits)]
+                    assert abs((((64 - w) * a + w * b + 32) >> 6) - int(col[ch])) == best_d[ch]
