@@ -478,7 +478,8 @@ int ora_power_method(const float m[16], float eig[4], float *eigval) {
 /* RGBACluster::GetPrincipalAxis with eigOne = eigTwo = NULL
  * (RGBAEndpoints.cpp:327-428).  Reproduces the read of default-valued
  * (-1,-1,-1,-1) entries past the unique-point count in the collinearity
- * test (RGBAEndpoints.cpp:364-371). */
+ * test (RGBAEndpoints.cpp:364-371).  Parity unpinned (float trajectory); the
+ * power method inside it is pinned on its own. */
 static void principal_axis(const clus *c, v4 *axis) {
   v4 to[16];
   for (uint32_t i = 0; i < c->n; i++) to[i] = v4_sub(clus_pt(c, i), c->avg);
@@ -629,7 +630,8 @@ static void clamp_endpoints(v4 *p1, v4 *p2) {
   }
 }
 
-/* ClampEndpointsToGrid (Compressor.cpp:212-250). */
+/* ClampEndpointsToGrid (Compressor.cpp:212-250).  Parity unpinned (float
+ * trajectory; only checked through whole-block GPU/oracle agreement). */
 static void clamp_to_grid(const cmode *m, v4 *p1, v4 *p2, uint8_t *best_pbc) {
   const int npbc = m_npbc(m);
   const int has_pb = npbc > 1;
@@ -865,7 +867,9 @@ static double optimize_endpoints(cmode *m, const clus *c, v4 *p1, v4 *p2, uint8_
   return best;
 }
 
-/* CompressCluster, RGB variant (Compressor.cpp:921-1094). */
+/* CompressCluster, RGB variant (Compressor.cpp:921-1094).  Parity unpinned:
+ * the k-means and least-squares float trajectories can only be pinned by
+ * running the reference, whose BC7 encoder does not build without CMake. */
 static double compress_cluster_rgb(cmode *m, const clus *c, v4 *p1, v4 *p2, uint8_t *best_idx,
                                    uint8_t *best_pbc) {
   if (v4_eq(c->mx, c->mn)) {
@@ -965,7 +969,8 @@ static float clamp_alpha(float x) {
   return t < 255.0f ? t : 255.0f;
 }
 
-/* CompressCluster, alpha/rotation variant (Compressor.cpp:632-919). */
+/* CompressCluster, alpha/rotation variant (Compressor.cpp:632-919).  Parity
+ * unpinned, including the stale-statistics quirk (see the RGB variant). */
 static double compress_cluster_alpha(cmode *m, const clus *c, v4 *p1, v4 *p2, uint8_t *best_idx,
                                      uint8_t *alpha_idx) {
   if (v4_eq(c->mx, c->mn)) {
@@ -1204,7 +1209,8 @@ static void bw_write(bitw *w, uint32_t val, int n) {
   }
 }
 
-/* CompressionMode::Pack (Compressor.cpp:1096-1298).  The p-bits are written
+/* CompressionMode::Pack (Compressor.cpp:1096-1298).  Parity unpinned for the
+ * multi-subset field order (covered only by decoding what it writes).  The p-bits are written
  * from the combo as chosen, also when the anchor fix-up swapped the two
  * endpoints (Compressor.cpp:1149-1170 vs 1234-1241). */
 static void mode_pack(const cmode *m, params *pr, uint8_t out[16]) {

@@ -11,7 +11,10 @@
  * Every exported function cites the reference lines it follows.  Functions
  * whose results are not pinned by a golden vector, a closed form, an
  * invariant or a brute-force check are marked "parity unpinned" in their
- * comment.
+ * comment.  Parity unpinned: the cluster fits of CompressCluster (RGB and
+ * alpha/rotation variants, with GetPrincipalAxis and ClampEndpointsToGrid)
+ * and Pack's multi-subset field order; their float trajectories need the
+ * reference BC7 encoder, which cannot be built without its CMake step.
  */
 #ifndef FASTC_ORACLE_H_
 #define FASTC_ORACLE_H_
