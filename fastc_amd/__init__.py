@@ -83,13 +83,16 @@ def lib():
         L.fh_kernel_time.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_uint32)]
         L.fh_kernel_time.restype = ctypes.c_int
+        L.fh_bc7_exhaustive_labels.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_void_p]
+        L.fh_bc7_exhaustive_labels.restype = ctypes.c_int
         _lib = L
     return _lib
 
 
 EXPORTED_SYMBOLS = ["fh_compress", "fh_compress_device", "fh_decompress",
                     "fh_compressed_size", "fh_last_error", "fh_device_available",
-                    "fh_set_kernel_timing", "fh_kernel_time"]
+                    "fh_set_kernel_timing", "fh_kernel_time", "fh_bc7_exhaustive_labels"]
 
 
 def _check(status):
@@ -171,6 +174,16 @@ def kernel_time(kernel):
     n = ctypes.c_uint32(0)
     _check(lib().fh_kernel_time(kernel.encode(), ctypes.byref(ms), ctypes.byref(n)))
     return ms.value, n.value
+
+
+def exhaustive_labels_device(blocks_ptr, n, labels_ptr, errors_ptr=0, stream=0):
+    """Training labels of the BC7 candidate ranker on device buffers
+    (fh_bc7_exhaustive_labels): blocks n x 16 uint32, labels n int32,
+    errors n uint64 (optional)."""
+    _check(lib().fh_bc7_exhaustive_labels(ctypes.c_void_p(blocks_ptr), n,
+                                          ctypes.c_void_p(labels_ptr),
+                                          ctypes.c_void_p(errors_ptr or None),
+                                          ctypes.c_void_p(stream)))
 
 
 def decompress(data, width, height, fmt=eCompressionFormat_BPTC):

@@ -933,89 +933,72 @@ __device__ uint32_t alpha_fit(const Fit &f, const uint32_t px[16], uint64_t &aid
 // ---------------------------------------------------------------------------
 
 // Candidates of a block: one per (mode, shape entry) of CompressClusters
-// (Compressor.cpp:1771-1809), with the rotation modes split per
-// (rotation, index mode).  Layout 0 (all but non-opaque blocks):
-//   0 mode 0, 1 mode 2, 2 mode 1, 3 mode 3, 4 mode 7, 5 mode 6 (shape entry 0),
-//   6 mode 6 (shape entry 1).
-// Layout 1 (non-opaque): 0-7 mode 4 (rotation = c/2, index mode = c%2),
-//   8-11 mode 5 (rotation), 12 mode 6, 13 mode 7.
+// (Compressor.cpp:1771-1809), with the rotation modes split per (rotation,
+// index mode), stored in the order the reference tries them (modes 0, 2, 1,
+// 3, 7, 4, 5, 6; shape entries; rotations; index modes), so that the first
+// strict minimum over the slots is the reference's choice.  Up to 16 slots
+// per block, each a 16-bit descriptor:
+//   mode 3 | shape 6 << 3 | rotation 2 << 9 | index mode 1 << 11 | entry 1 << 12
+// (entry = the BoxSelection shape entry, which keys the annealing stream).
 // Candidate types (work queues, sorted so that a wave runs one code path):
 //   0 mode 0, 1 mode 2, 2 mode 1, 3 mode 3, 4 mode 7, 5 mode 6, 6 mode 4, 7 mode 5.
 constexpr int kNumTypes = 8;
-__constant__ int kTypeCap[kNumTypes] = {1, 1, 1, 1, 1, 2, 8, 4};
+constexpr int kMaxCands = 16;
 __constant__ int kTypeOfMode[8] = {0, 2, 1, 3, 6, 7, 5, 4};
 
-// Block selection, packed: cls 2 | layout 1 | nshapes 2 | shape2 6 | shape3 6 | modes 8.
-__device__ __forceinline__ uint32_t sel_pack(int cls, int layout, int nshapes, int s2, int s3,
-                                             uint32_t modes) {
-  return (uint32_t)cls | ((uint32_t)layout << 2) | ((uint32_t)nshapes << 3) |
-         ((uint32_t)s2 << 5) | ((uint32_t)s3 << 11) | (modes << 17);
+// Block class and candidate count, packed: cls 2 | count 5 << 2.
+__device__ __forceinline__ uint32_t sel_pack(int cls, int count) {
+  return (uint32_t)cls | ((uint32_t)count << 2);
 }
 __device__ __forceinline__ int sel_cls(uint32_t s) { return s & 3; }
-__device__ __forceinline__ int sel_layout(uint32_t s) { return (s >> 2) & 1; }
-__device__ __forceinline__ int sel_nshapes(uint32_t s) { return (s >> 3) & 3; }
-__device__ __forceinline__ int sel_shape2(uint32_t s) { return (s >> 5) & 63; }
-__device__ __forceinline__ int sel_shape3(uint32_t s) { return (s >> 11) & 63; }
-__device__ __forceinline__ uint32_t sel_modes(uint32_t s) { return (s >> 17) & 0xFF; }
+__device__ __forceinline__ int sel_count(uint32_t s) { return (s >> 2) & 31; }
 
 struct Cand {
   int mode, entry, rot, im, shape, nsub;
 };
 
-__device__ __forceinline__ bool cand_of(uint32_t sel, int c, uint32_t q, Cand &k) {
-  k.rot = 0;
-  k.im = 0;
-  k.entry = 0;
-  k.nsub = 1;
-  k.shape = 0;
-  const uint32_t modes = sel_modes(sel);
-  if (sel_layout(sel) == 0) {
-    switch (c) {
-      case 0: k.mode = 0; break;
-      case 1: k.mode = 2; break;
-      case 2: k.mode = 1; break;
-      case 3: k.mode = 3; break;
-      case 4: k.mode = 7; break;
-      case 5: k.mode = 6; break;
-      case 6: k.mode = 6; k.entry = 1; break;
-      default: return false;
+__device__ __forceinline__ uint32_t cdesc_pack(int mode, int shape, int rot, int im, int entry) {
+  return (uint32_t)mode | ((uint32_t)shape << 3) | ((uint32_t)rot << 9) | ((uint32_t)im << 11) |
+         ((uint32_t)entry << 12);
+}
+
+__device__ __forceinline__ void cand_from_desc(uint32_t d, Cand &k) {
+  k.mode = d & 7;
+  k.shape = (d >> 3) & 63;
+  k.rot = (d >> 9) & 3;
+  k.im = (d >> 11) & 1;
+  k.entry = (d >> 12) & 1;
+  k.nsub = kModes[k.mode].subsets;
+}
+
+// The candidate list of a searched block after BoxSelection, in the
+// reference's order (see above).  Returns the count.
+__device__ int box_candidates(bool opaque_layout, int nshapes, int s2, int s3, uint32_t modes,
+                              uint32_t q, uint16_t out[kMaxCands]) {
+  int n = 0;
+  if (opaque_layout) {
+    if (nshapes >= 2) {
+      if (((modes >> 0) & 1) && s3 < 16) out[n++] = (uint16_t)cdesc_pack(0, s3, 0, 0, 1);
+      if ((modes >> 2) & 1) out[n++] = (uint16_t)cdesc_pack(2, s3, 0, 0, 1);
     }
-    if (k.mode == 0 || k.mode == 2) {
-      if (sel_nshapes(sel) < 2) return false;
-      k.entry = 1;
-      k.nsub = 3;
-      k.shape = sel_shape3(sel);
-      if (k.mode == 0 && k.shape >= 16) return false;
-    } else if (k.mode != 6) {
-      k.nsub = 2;
-      k.shape = sel_shape2(sel);
-    } else {
-      k.shape = k.entry ? sel_shape3(sel) : sel_shape2(sel);
-      if (k.entry >= sel_nshapes(sel)) return false;
+    if ((modes >> 1) & 1) out[n++] = (uint16_t)cdesc_pack(1, s2, 0, 0, 0);
+    if ((modes >> 3) & 1) out[n++] = (uint16_t)cdesc_pack(3, s2, 0, 0, 0);
+    if ((modes >> 7) & 1) out[n++] = (uint16_t)cdesc_pack(7, s2, 0, 0, 0);
+    if ((modes >> 6) & 1) {
+      out[n++] = (uint16_t)cdesc_pack(6, s2, 0, 0, 0);
       // both mode-6 entries fit the same 16 pixels; without annealing the
       // second repeats the first exactly and cannot win (strict <)
-      if (k.entry == 1 && q == 0) return false;
+      if (nshapes >= 2 && q > 0) out[n++] = (uint16_t)cdesc_pack(6, s3, 0, 0, 1);
     }
   } else {
-    if (c < 8) {
-      k.mode = 4;
-      k.rot = c >> 1;
-      k.im = c & 1;
-    } else if (c < 12) {
-      k.mode = 5;
-      k.rot = c - 8;
-    } else if (c == 12) {
-      k.mode = 6;
-      k.shape = sel_shape2(sel);
-    } else if (c == 13) {
-      k.mode = 7;
-      k.nsub = 2;
-      k.shape = sel_shape2(sel);
-    } else {
-      return false;
-    }
+    if ((modes >> 7) & 1) out[n++] = (uint16_t)cdesc_pack(7, s2, 0, 0, 0);
+    if ((modes >> 4) & 1)
+      for (int c = 0; c < 8; c++) out[n++] = (uint16_t)cdesc_pack(4, 0, c >> 1, c & 1, 0);
+    if ((modes >> 5) & 1)
+      for (int c = 0; c < 4; c++) out[n++] = (uint16_t)cdesc_pack(5, 0, c, 0, 0);
+    if ((modes >> 6) & 1) out[n++] = (uint16_t)cdesc_pack(6, s2, 0, 0, 0);
   }
-  return (modes >> k.mode) & 1;
+  return n;
 }
 
 // Result of one candidate, read by the pack kernel.
@@ -1041,11 +1024,12 @@ struct AState {
 
 struct Work {
   uint32_t *sel;          // per block
+  uint16_t *cdesc;        // per block x 16 candidate descriptors
   CandRec *rec;           // per block x 16
-  uint32_t *queue;        // per-type candidate queues
+  uint32_t *queue;        // per-type candidate queues, 16 entries per block each
   uint32_t *counters;     // [0..7] queue sizes, [8..15] set-up fetch, [16] anneal size, [17] anneal fetch
   AState *astate;
-  uint32_t qoff[kNumTypes];
+  uint32_t qcap;          // entries per type queue
   uint32_t astate_cap;
 };
 
@@ -1068,6 +1052,22 @@ __device__ __forceinline__ uint32_t cand_tag(int mode, int entry, int subset, in
          ((uint32_t)im << 9);
 }
 
+// Appends the wave's valid candidates to their type queues (one atomic per
+// type per wave).
+__device__ __forceinline__ void enqueue_candidates(Work &w, bool valid, int type, uint32_t id) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  for (int t = 0; t < kNumTypes; t++) {
+    const uint64_t m = __ballot(valid && type == t);
+    if (!m) continue;
+    const int leader = (int)(__ffsll((unsigned long long)m) - 1);
+    uint32_t b0 = 0;
+    if (lane == leader) b0 = atomicAdd(&w.counters[t], (uint32_t)__popcll(m));
+    b0 = __shfl(b0, leader);
+    if (valid && type == t) w.queue[(size_t)t * w.qcap + b0 + __popcll(m & lt)] = id;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Kernel 1: classification and BoxSelection (Compressor.cpp:1670-1750,
 // 1819-1860); one wave per four blocks, lane i scores shape i.  Emits the
@@ -1076,6 +1076,7 @@ __device__ __forceinline__ uint32_t cand_tag(int mode, int entry, int subset, in
 __global__ void __launch_bounds__(64) select_kernel(BlockRange r, Bc7Params prm, Work w) {
   __shared__ uint32_t s_px[4][16];
   __shared__ uint32_t s_sel[4];
+  __shared__ uint16_t s_cd[4][kMaxCands];
   const int lane = threadIdx.x;
   const uint32_t base = blockIdx.x * 4;  // relative block index
   {
@@ -1103,10 +1104,12 @@ __global__ void __launch_bounds__(64) select_kernel(BlockRange r, Bc7Params prm,
       opaque = opaque && (px[i] >> 24) >= 250;
     }
     uint32_t sel;
+    uint16_t cd[kMaxCands];
+    int count = 0;
     if (solid) {
-      sel = sel_pack(0, 0, 1, 0, 0, 0);
+      sel = sel_pack(0, 0);
     } else if (transparent) {
-      sel = sel_pack(1, 0, 1, 0, 0, 0);
+      sel = sel_pack(1, 0);
     } else {
       double err2 = 0.0, err3 = 0.0;
       for (int ns = 2; ns <= 3; ns++) {
@@ -1156,55 +1159,144 @@ __global__ void __launch_bounds__(64) select_kernel(BlockRange r, Bc7Params prm,
         }
       }
       uint32_t modes;
-      int layout = 0, nshapes = 1, s3 = 0;
+      bool opaque_layout = true;
+      int nshapes = 1, s3 = 0;
       if (v2 < 1e-9) {
         modes = (1u << 1) | (1u << 3) | (1u << 7);
       } else if (!opaque) {
         modes = (1u << 4) | (1u << 5) | (1u << 6) | (1u << 7);
-        layout = 1;
+        opaque_layout = false;
       } else {
         modes = 0xFFu & ~((1u << 4) | (1u << 5));
         nshapes = 2;
         s3 = i3;
         if (v3 < 1e-9) modes = (1u << 0) | (1u << 2);
       }
-      sel = sel_pack(2, layout, nshapes, i2, s3, modes & prm.block_modes);
+      count = box_candidates(opaque_layout, nshapes, i2, s3, modes & prm.block_modes,
+                             prm.quality, cd);
+      sel = sel_pack(2, count);
     }
     if (lane == 0) {
       s_sel[blk] = sel;
       w.sel[rel] = sel;
     }
+    // slot c of this block's descriptors goes out from lane c
+    if (lane < count) w.cdesc[(size_t)rel * kMaxCands + lane] = cd[lane];
+    if (lane < kMaxCands) s_cd[blk][lane] = lane < count ? cd[lane] : (uint16_t)0;
   }
   __syncthreads();
-  // enqueue: lane = (block, candidate)
+  // enqueue: lane = (block, candidate slot)
   const int blk = lane >> 4, c = lane & 15;
   const uint32_t rel = base + blk;
   bool valid = false;
   int type = 0;
-  if (rel < r.num_blocks) {
-    const uint32_t sel = s_sel[blk];
-    Cand k;
-    if (sel_cls(sel) == 2 && cand_of(sel, c, prm.quality, k)) {
-      valid = true;
-      type = kTypeOfMode[k.mode];
-    }
-    CandRec *rc = &w.rec[(size_t)rel * 16 + c];
-    rc->err = 0xFFFFFFFFu;
+  if (rel < r.num_blocks && c < sel_count(s_sel[blk])) {
+    valid = true;
+    type = kTypeOfMode[s_cd[blk][c] & 7];
   }
-  const uint64_t lt = (1ull << lane) - 1ull;
-  for (int t = 0; t < kNumTypes; t++) {
-    const uint64_t m = __ballot(valid && type == t);
-    if (!m) continue;
-    uint32_t b0 = 0;
-    if (lane == (int)(__ffsll((unsigned long long)m) - 1)) b0 = atomicAdd(&w.counters[t], (uint32_t)__popcll(m));
-    b0 = __shfl(b0, (int)(__ffsll((unsigned long long)m) - 1));
-    if (valid && type == t) w.queue[w.qoff[t] + b0 + __popcll(m & lt)] = (rel << 4) | (uint32_t)c;
-  }
+  enqueue_candidates(w, valid, type, (rel << 4) | (uint32_t)c);
 }
 
 // ---------------------------------------------------------------------------
 // Kernel 2: candidate set-up, persistent over the type queues in order.
 // ---------------------------------------------------------------------------
+// Quality-0 fit of one candidate: every subset's endpoints, indices and
+// error (CompressionMode::Compress, Compressor.cpp:1300-1371, without the
+// annealing), the record the pack kernel reads, and the annealing start
+// state.  Returns the mask of subsets that annealing continues (quality > 0).
+__device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t px[16], uint32_t quality,
+                                  const uint32_t *sctab, CandRec &rec, AState &st,
+                                  uint32_t &total) {
+  Fit f;
+  set_fit(f, k.mode, k.rot, k.im);
+  rec.err = 0;
+  rec.px1[0] = rec.px1[1] = rec.px1[2] = 0;
+  rec.px2[0] = rec.px2[1] = rec.px2[2] = 0;
+  rec.aidx_lo = rec.aidx_hi = 0;
+  st.extra_err = 0;
+  st.pad = 0;
+  uint32_t active = 0, pbcs = 0;
+  total = 0;
+  uint64_t cidx = 0;
+  if (kModes[k.mode].has_rot) {
+    uint64_t aidx;
+    uint32_t a1q, a2q;
+    const uint32_t aerr = alpha_fit(f, px, aidx, a1q, a2q);
+    // the RGB fit sees the stale statistics of the unrotated cluster
+    float avg[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+#pragma unroll
+      for (int kk = 0; kk < 4; kk++) avg[kk] += (float)chan(px[i], kk);
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) avg[kk] = avg[kk] / 16.0f;
+    SubStart s0;
+    subset_setup(f, quality, sctab, px, 0xFFFFu, k.rot + 1, 16, avg, false, s0);
+    rec.aidx_lo = (uint32_t)aidx;
+    rec.aidx_hi = (uint32_t)(aidx >> 32);
+    // Pack writes the RGB of the fit and the fitted alpha, no p-bit
+    rec.px1[0] = (to_pixel_int(s0.p1, f.qmask, -1) & 0x00FFFFFFu) | (a1q << 24);
+    rec.px2[0] = (to_pixel_int(s0.p2, f.qmask, -1) & 0x00FFFFFFu) | (a2q << 24);
+    st.p1[0] = s0.p1;
+    st.p2[0] = s0.p2;
+    st.err[0] = s0.err;
+    st.extra_err = aerr;
+    cidx = s0.idx;
+    total = s0.err + aerr;
+    active = s0.anneal ? 1u : 0u;
+  } else {
+    for (int s = 0; s < k.nsub; s++) {
+      const uint32_t smask = subset_mask(k.shape, k.nsub, s);
+      const int n = __popc(smask);
+      float avg[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      bool same = true;
+      uint32_t firstp = 0;
+      bool have_first = false;
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        if (!((smask >> i) & 1)) continue;
+        if (!have_first) {
+          firstp = px[i];
+          have_first = true;
+        }
+        same = same && px[i] == firstp;
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++) avg[kk] += (float)chan(px[i], kk);
+      }
+#pragma unroll
+      for (int kk = 0; kk < 4; kk++) avg[kk] = avg[kk] / (float)n;
+      SubStart ss;
+      subset_setup(f, quality, sctab, px, smask, 0, n, avg, same, ss);
+      const int pb0 = f.pbit_type == PB_NONE ? -1 : pbc_bit(f.pbit_type, ss.pbc, 0);
+      const int pb1 = f.pbit_type == PB_NONE ? -1 : pbc_bit(f.pbit_type, ss.pbc, 1);
+      const uint32_t e1 = to_pixel_int(ss.p1, f.qmask, pb0);
+      const uint32_t e2 = to_pixel_int(ss.p2, f.qmask, pb1);
+      if (s == 0) {
+        rec.px1[0] = e1; rec.px2[0] = e2;
+        st.p1[0] = ss.p1; st.p2[0] = ss.p2; st.err[0] = ss.err;
+      } else if (s == 1) {
+        rec.px1[1] = e1; rec.px2[1] = e2;
+        st.p1[1] = ss.p1; st.p2[1] = ss.p2; st.err[1] = ss.err;
+      } else {
+        rec.px1[2] = e1; rec.px2[2] = e2;
+        st.p1[2] = ss.p1; st.p2[2] = ss.p2; st.err[2] = ss.err;
+      }
+      pbcs |= (uint32_t)ss.pbc << (2 * s);
+      cidx |= ss.idx;
+      total += ss.err;
+      if (ss.anneal) active |= 1u << s;
+    }
+  }
+  rec.cidx_lo = (uint32_t)cidx;
+  rec.cidx_hi = (uint32_t)(cidx >> 32);
+  rec.meta = pbcs | ((uint32_t)k.rot << 8) | ((uint32_t)k.im << 10);
+  rec.err = total;  // the quality-0 result; annealing overwrites active ones
+  st.pbc = pbcs;
+  st.idx_lo = (uint32_t)cidx;
+  st.idx_hi = (uint32_t)(cidx >> 32);
+  return active;
+}
+
 __global__ void __launch_bounds__(64) setup_kernel(BlockRange r, Bc7Params prm, Work w,
                                                    const uint32_t *sctab) {
   const int lane = threadIdx.x;
@@ -1217,109 +1309,23 @@ __global__ void __launch_bounds__(64) setup_kernel(BlockRange r, Bc7Params prm, 
       if (b0 >= count) break;
       const uint32_t item = b0 + lane;
       if (item >= count) continue;
-      const uint32_t id = w.queue[w.qoff[t] + item];
+      const uint32_t id = w.queue[(size_t)t * w.qcap + item];
       const uint32_t rel = id >> 4;
       const int c = id & 15;
-      const uint32_t sel = w.sel[rel];
       Cand k;
-      cand_of(sel, c, prm.quality, k);
+      cand_from_desc(w.cdesc[(size_t)rel * kMaxCands + c], k);
       uint32_t px[16];
       load_block(r, rel, px);
-      Fit f;
-      set_fit(f, k.mode, k.rot, k.im);
-      const uint64_t bid = prm.block_id_base + r.first_block + rel;
       CandRec rec;
-      rec.err = 0;
-      rec.px1[0] = rec.px1[1] = rec.px1[2] = 0;
-      rec.px2[0] = rec.px2[1] = rec.px2[2] = 0;
-      rec.aidx_lo = rec.aidx_hi = 0;
       AState st;
+      uint32_t total = 0;
+      const uint32_t active = fit_candidate(k, px, prm.quality, sctab, rec, st, total);
       st.id = id;
-      st.extra_err = 0;
-      st.pad = 0;
-      uint32_t active = 0, pbcs = 0, total = 0;
-      uint64_t cidx = 0;
-      if (kModes[k.mode].has_rot) {
-        uint64_t aidx;
-        uint32_t a1q, a2q;
-        const uint32_t aerr = alpha_fit(f, px, aidx, a1q, a2q);
-        // the RGB fit sees the stale statistics of the unrotated cluster
-        float avg[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int i = 0; i < 16; i++)
-#pragma unroll
-          for (int kk = 0; kk < 4; kk++) avg[kk] += (float)chan(px[i], kk);
-#pragma unroll
-        for (int kk = 0; kk < 4; kk++) avg[kk] = avg[kk] / 16.0f;
-        SubStart s0;
-        subset_setup(f, prm.quality, sctab, px, 0xFFFFu, k.rot + 1, 16, avg, false, s0);
-        rec.aidx_lo = (uint32_t)aidx;
-        rec.aidx_hi = (uint32_t)(aidx >> 32);
-        // Pack writes the RGB of the fit and the fitted alpha, no p-bit
-        rec.px1[0] = (to_pixel_int(s0.p1, f.qmask, -1) & 0x00FFFFFFu) | (a1q << 24);
-        rec.px2[0] = (to_pixel_int(s0.p2, f.qmask, -1) & 0x00FFFFFFu) | (a2q << 24);
-        st.p1[0] = s0.p1;
-        st.p2[0] = s0.p2;
-        st.err[0] = s0.err;
-        st.extra_err = aerr;
-        cidx = s0.idx;
-        total = s0.err + aerr;
-        active = s0.anneal ? 1u : 0u;
-      } else {
-        for (int s = 0; s < k.nsub; s++) {
-          const uint32_t smask = subset_mask(k.shape, k.nsub, s);
-          const int n = __popc(smask);
-          float avg[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-          bool same = true;
-          uint32_t firstp = 0;
-          bool have_first = false;
-#pragma unroll
-          for (int i = 0; i < 16; i++) {
-            if (!((smask >> i) & 1)) continue;
-            if (!have_first) {
-              firstp = px[i];
-              have_first = true;
-            }
-            same = same && px[i] == firstp;
-#pragma unroll
-            for (int kk = 0; kk < 4; kk++) avg[kk] += (float)chan(px[i], kk);
-          }
-#pragma unroll
-          for (int kk = 0; kk < 4; kk++) avg[kk] = avg[kk] / (float)n;
-          SubStart ss;
-          subset_setup(f, prm.quality, sctab, px, smask, 0, n, avg, same, ss);
-          const int pb0 = f.pbit_type == PB_NONE ? -1 : pbc_bit(f.pbit_type, ss.pbc, 0);
-          const int pb1 = f.pbit_type == PB_NONE ? -1 : pbc_bit(f.pbit_type, ss.pbc, 1);
-          const uint32_t e1 = to_pixel_int(ss.p1, f.qmask, pb0);
-          const uint32_t e2 = to_pixel_int(ss.p2, f.qmask, pb1);
-          if (s == 0) {
-            rec.px1[0] = e1; rec.px2[0] = e2;
-            st.p1[0] = ss.p1; st.p2[0] = ss.p2; st.err[0] = ss.err;
-          } else if (s == 1) {
-            rec.px1[1] = e1; rec.px2[1] = e2;
-            st.p1[1] = ss.p1; st.p2[1] = ss.p2; st.err[1] = ss.err;
-          } else {
-            rec.px1[2] = e1; rec.px2[2] = e2;
-            st.p1[2] = ss.p1; st.p2[2] = ss.p2; st.err[2] = ss.err;
-          }
-          pbcs |= (uint32_t)ss.pbc << (2 * s);
-          cidx |= ss.idx;
-          total += ss.err;
-          if (ss.anneal) active |= 1u << s;
-        }
-      }
-      rec.cidx_lo = (uint32_t)cidx;
-      rec.cidx_hi = (uint32_t)(cidx >> 32);
-      rec.meta = pbcs | ((uint32_t)k.rot << 8) | ((uint32_t)k.im << 10);
-      rec.err = total;  // the quality-0 result; annealing overwrites active ones
-      w.rec[(size_t)rel * 16 + c] = rec;
+      w.rec[(size_t)rel * kMaxCands + c] = rec;
       if (active) {
         st.info = (uint32_t)k.mode | ((uint32_t)k.rot << 3) | ((uint32_t)k.im << 5) |
                   ((uint32_t)k.nsub << 6) | ((uint32_t)k.shape << 8) | (active << 14) |
                   ((uint32_t)k.entry << 17);
-        st.pbc = pbcs;
-        st.idx_lo = (uint32_t)cidx;
-        st.idx_hi = (uint32_t)(cidx >> 32);
         const uint32_t slot = atomicAdd(&w.counters[16], 1u);
         w.astate[slot] = st;
       }
@@ -1740,39 +1746,22 @@ __global__ void __launch_bounds__(256) pack_kernel(BlockRange r, Bc7Scratch scr,
   } else if (sel_cls(sel) == 1) {
     bw.put(1u << 6, 7);  // WriteTransparentBlock (Compressor.cpp:1416-1421)
   } else {
-    // candidates in CompressClusters' order: modes 0,2,1,3,7,4,5,6, shape
-    // entries and rotations/index modes in loop order; first strict minimum
-    static constexpr int kL0[7] = {0, 1, 2, 3, 4, 5, 6};
-    static constexpr int kL1[14] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13};
-    const CandRec *rc = &w.rec[(size_t)rel * 16];
+    // the slots hold the candidates in CompressClusters' order: keep the
+    // first strict minimum
+    const CandRec *rc = &w.rec[(size_t)rel * kMaxCands];
+    const int count = sel_count(sel);
     uint32_t best = 0xFFFFFFFFu;
     int bc = -1;
-    if (sel_layout(sel) == 0) {
-      for (int i = 0; i < 7; i++) {
-        const uint32_t e = rc[kL0[i]].err;
-        if (e < best) {
-          best = e;
-          bc = kL0[i];
-        }
-      }
-    } else {
-      // layout 1 order: mode 7 (13) first, then 4 (0-7), 5 (8-11), 6 (12)
-      const uint32_t e7 = rc[13].err;
-      if (e7 < best) {
-        best = e7;
-        bc = 13;
-      }
-      for (int i = 0; i < 13; i++) {
-        const uint32_t e = rc[kL1[i]].err;
-        if (e < best) {
-          best = e;
-          bc = kL1[i];
-        }
+    for (int i = 0; i < count; i++) {
+      const uint32_t e = rc[i].err;
+      if (e < best) {
+        best = e;
+        bc = i;
       }
     }
     if (bc >= 0) {
       Cand k;
-      cand_of(sel, bc, 1, k);
+      cand_from_desc(w.cdesc[(size_t)rel * kMaxCands + bc], k);
       pack_mode(k.mode, k.shape, rc[bc], bw);
     }
   }
@@ -1816,6 +1805,74 @@ __global__ void __launch_bounds__(256) guard_kernel(BlockRange r, const uint4 *q
   bc7dec::decode_block(a, da);
   bc7dec::decode_block(z, dz);
   if (premul_err(px, dz) < premul_err(px, da)) out[b] = z;
+}
+
+// ---------------------------------------------------------------------------
+// Exhaustive quality-0 search, the training labels of the candidate ranker:
+// every mode with every partition shape, rotation and index mode (the 285
+// valid outputs of the ranking network) for every block.  Each candidate is
+// fitted, packed and decoded with the reference decoder, and the one with
+// the smallest decoded error (premul_err) wins: the encoder's internal error
+// is not the decoded error for the rotation modes (stale statistics, the
+// reference decoder's mode-4 index-mode-1 reading) and the single-colour
+// estimate.  Ties go to the mode tried first by CompressClusters
+// (0,2,1,3,7,4,5,6), then to the lower slot.  One launch per mode, so that a
+// wave runs one code path; lanes take consecutive slots of one block.
+// ---------------------------------------------------------------------------
+__constant__ int kModePos[8] = {0, 2, 1, 3, 5, 6, 7, 4};
+
+__device__ __forceinline__ void slot_cand(int mode, int slot, Cand &k) {
+  k.mode = mode;
+  k.entry = 0;
+  k.nsub = kModes[mode].subsets;
+  k.shape = k.nsub > 1 ? slot : 0;
+  k.rot = mode == 4 ? (slot >> 1) : (mode == 5 ? slot : 0);
+  k.im = mode == 4 ? (slot & 1) : 0;
+}
+
+__global__ void __launch_bounds__(256) label_fill_kernel(unsigned long long *best, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) best[i] = ~0ull;
+}
+
+__global__ void __launch_bounds__(64) label_kernel(BlockRange r, int mode, uint32_t nslots,
+                                                   const uint32_t *sctab,
+                                                   unsigned long long *best) {
+  const uint64_t items = (uint64_t)r.num_blocks * nslots;
+  for (uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; item < items;
+       item += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t rel = (uint32_t)(item / nslots);
+    const int slot = (int)(item % nslots);
+    Cand k;
+    slot_cand(mode, slot, k);
+    uint32_t px[16];
+    load_block(r, rel, px);
+    CandRec rec;
+    AState st;
+    uint32_t total = 0;
+    fit_candidate(k, px, 0, sctab, rec, st, total);
+    BitWriter bw;
+    pack_mode(k.mode, k.shape, rec, bw);
+    const uint4 blk = make_uint4((uint32_t)bw.lo, (uint32_t)(bw.lo >> 32), (uint32_t)bw.hi,
+                                 (uint32_t)(bw.hi >> 32));
+    uint32_t dec[16];
+    bc7dec::decode_block(blk, dec);
+    const unsigned long long err = premul_err(px, dec);  // < 2^38
+    const unsigned long long key =
+        (err << 16) | (unsigned long long)(kModePos[mode] * 64 + slot);
+    atomicMin(&best[rel], key);
+  }
+}
+
+__global__ void __launch_bounds__(256) label_finish_kernel(const unsigned long long *best,
+                                                           uint32_t n, int32_t *labels,
+                                                           unsigned long long *errors) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long key = best[i];
+  const int rank = (int)(key & 0xFFFFu);
+  labels[i] = kOrder[rank >> 6] * 64 + (rank & 63);
+  if (errors) errors[i] = key >> 16;
 }
 
 __global__ void reset_kernel(uint32_t *counters, int n) {
@@ -1914,8 +1971,10 @@ hipError_t bc7_upload_tables(int device) {
 
 size_t bc7_work_bytes(uint32_t num_blocks) {
   const size_t n = num_blocks;
-  return 256 + n * 4 + n * 16 * sizeof(bc7::CandRec) + n * 19 * 4 + 64 * 4 +
-         n * 14 * sizeof(bc7::AState) + n * 16 + 4096;
+  // counters, sel, cdesc, rec, queues, astate, q0, alignment slack
+  return 64 * 4 + n * 4 + n * bc7::kMaxCands * 2 + n * bc7::kMaxCands * sizeof(bc7::CandRec) +
+         n * bc7::kNumTypes * bc7::kMaxCands * 4 + n * bc7::kMaxCands * sizeof(bc7::AState) +
+         n * 16 + 8 * 256;
 }
 
 hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch &s, void *work,
@@ -1937,19 +1996,16 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
     return q;
   };
   bc7::Work w;
+  const size_t nc = (size_t)n * bc7::kMaxCands;
   w.counters = reinterpret_cast<uint32_t *>(carve(64 * 4));
   w.sel = reinterpret_cast<uint32_t *>(carve((size_t)n * 4));
-  w.rec = reinterpret_cast<bc7::CandRec *>(carve((size_t)n * 16 * sizeof(bc7::CandRec)));
-  w.queue = reinterpret_cast<uint32_t *>(carve((size_t)n * 19 * 4));
-  w.astate = reinterpret_cast<bc7::AState *>(carve((size_t)n * 14 * sizeof(bc7::AState)));
-  w.astate_cap = n * 14;
+  w.cdesc = reinterpret_cast<uint16_t *>(carve(nc * 2));
+  w.rec = reinterpret_cast<bc7::CandRec *>(carve(nc * sizeof(bc7::CandRec)));
+  w.qcap = (uint32_t)nc;
+  w.queue = reinterpret_cast<uint32_t *>(carve(nc * bc7::kNumTypes * 4));
+  w.astate = reinterpret_cast<bc7::AState *>(carve(nc * sizeof(bc7::AState)));
+  w.astate_cap = (uint32_t)nc;
   uint4 *q0 = reinterpret_cast<uint4 *>(carve((size_t)n * 16));
-  static const int cap[bc7::kNumTypes] = {1, 1, 1, 1, 1, 2, 8, 4};
-  uint32_t off = 0;
-  for (int t = 0; t < bc7::kNumTypes; t++) {
-    w.qoff[t] = off;
-    off += cap[t] * n;
-  }
   const uint32_t nchunks = (n + 255) / 256;
   const uint32_t ngroups = (n + 3) / 4;
   const uint32_t persistent = (uint32_t)g_num_cus[dev] * 16;
@@ -1985,6 +2041,36 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
     hipLaunchKernelGGL(bc7::guard_kernel, pack_grid, dim3(256), 0, stream, r, q0);
     timing_end(t, kTagBc7Guard, stream);
   }
+  return hipGetLastError();
+}
+
+hipError_t bc7_exhaustive_labels(const BlockRange &r, unsigned long long *best,
+                                 int32_t *labels, unsigned long long *errors,
+                                 hipStream_t stream) {
+  const uint32_t n = r.num_blocks;
+  if (n == 0) return hipSuccess;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  e = bc7_upload_tables(dev);
+  if (e != hipSuccess) return e;
+  const uint32_t *sct = bc7_single_color_table(dev);
+  if (!sct) return hipErrorOutOfMemory;
+  hipLaunchKernelGGL(bc7::label_fill_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, best,
+                     n);
+  static const uint32_t kSlots[8] = {16, 64, 64, 64, 8, 4, 1, 64};
+  const int t = timing_begin(stream);
+  for (int mode = 0; mode < 8; mode++) {
+    const uint64_t items = (uint64_t)n * kSlots[mode];
+    uint64_t groups = (items + 63) / 64;
+    const uint64_t cap = (uint64_t)g_num_cus[dev] * 64;
+    if (groups > cap) groups = cap;
+    hipLaunchKernelGGL(bc7::label_kernel, dim3((uint32_t)groups), dim3(64), 0, stream, r, mode,
+                       kSlots[mode], sct, best);
+  }
+  timing_end(t, kTagBc7Labels, stream);
+  hipLaunchKernelGGL(bc7::label_finish_kernel, dim3((n + 255) / 256), dim3(256), 0, stream,
+                     (const unsigned long long *)best, n, labels, errors);
   return hipGetLastError();
 }
 

@@ -343,6 +343,33 @@ fh_status fh_compressed_size(uint32_t format, uint32_t width, uint32_t height, s
   return FH_OK;
 }
 
+fh_status fh_bc7_exhaustive_labels(const uint32_t *blocks, uint32_t n, int32_t *labels,
+                                   unsigned long long *errors, void *hip_stream) {
+  if (!blocks || !labels) return fail(FH_ERR_INVALID_ARGUMENT, "null buffer");
+  if (n == 0) return FH_OK;
+  fh_status s;
+  const int dev = current_device(&s);
+  if (dev < 0) return s;
+  DeviceState *st = device_state(dev);
+  std::lock_guard<std::mutex> g(st->mu);
+  hipError_t e = ensure_init(st);
+  if (e != hipSuccess) return fail_hip(e, "device initialisation");
+  e = grow(&st->scratch, &st->scratch_size, (size_t)n * 8 + 256);
+  if (e != hipSuccess) return fail_hip(e, "scratch allocation");
+  // n blocks of 16 pixels, seen as a 4 x 4n image one block wide
+  BlockRange r;
+  r.img = blocks;
+  r.out = nullptr;
+  r.width = 4;
+  r.blocks_wide = 1;
+  r.first_block = 0;
+  r.num_blocks = n;
+  e = bc7_exhaustive_labels(r, reinterpret_cast<unsigned long long *>(st->scratch), labels,
+                            errors, reinterpret_cast<hipStream_t>(hip_stream));
+  if (e != hipSuccess) return fail_hip(e, "label search launch");
+  return FH_OK;
+}
+
 const char *fh_last_error(void) { return t_last_error.c_str(); }
 
 int fh_device_available(void) {

@@ -47,6 +47,13 @@ constexpr uint32_t kBc7MaxBlocksPerCall = 1u << 20;
 hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch &s, void *work,
                       hipStream_t stream);
 
+// Exhaustive quality-0 search over all 285 (mode, slot) candidates of the
+// candidate ranker: labels[i] = mode * 64 + slot of the smallest decoded
+// premultiplied error (errors[i], optional; Image::ComputePSNR's per-pixel
+// term times 255^2).  best: n words of scratch.
+hipError_t bc7_exhaustive_labels(const BlockRange &r, unsigned long long *best, int32_t *labels,
+                                 unsigned long long *errors, hipStream_t stream);
+
 hipError_t dxt_encode(const BlockRange &r, int dxt5, hipStream_t stream);
 hipError_t etc1_encode(const BlockRange &r, hipStream_t stream);
 
@@ -74,6 +81,7 @@ enum KernelTag {
   kTagDxt,
   kTagEtc1,
   kTagBc7Decode,
+  kTagBc7Labels,
   kNumKernelTags
 };
 int timing_begin(hipStream_t stream);

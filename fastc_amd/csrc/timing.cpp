@@ -16,9 +16,9 @@
 namespace fh {
 namespace {
 
-const char *const kTagNames[kNumKernelTags] = {"bc7_scan",  "bc7_select", "bc7_setup",
-                                               "bc7_anneal", "bc7_pack",  "bc7_guard",
-                                               "dxt",       "etc1",      "bc7_decode"};
+const char *const kTagNames[kNumKernelTags] = {
+    "bc7_scan", "bc7_select", "bc7_setup", "bc7_anneal", "bc7_pack", "bc7_guard",
+    "dxt",      "etc1",       "bc7_decode", "bc7_labels"};
 
 struct Pending {
   int tag;

@@ -106,6 +106,19 @@ fh_status fh_compressed_size(uint32_t format, uint32_t width, uint32_t height, s
 const char *fh_last_error(void);
 int fh_device_available(void);
 
+/* Training labels of the BC7 candidate ranker (no reference counterpart).
+ * Device buffers: blocks = n x 16 RGBA8 pixels (block-major, raster order
+ * inside a block), labels = n int32, errors = n uint64 or NULL.  For every
+ * block, all 285 (mode, slot) candidates are fitted at quality 0 (slot =
+ * partition shape for modes 0-3 and 7, rotation * 2 + index mode for mode 4,
+ * rotation for mode 5, 0 for mode 6), packed and decoded with the reference
+ * decoder; labels[i] = mode * 64 + slot of the smallest premultiplied RGB
+ * error (sum of (a*c - a'*c')^2, written to errors[i]).  Ties go to the mode
+ * BPTCC::Compress tries first (0,2,1,3,7,4,5,6), then the lower slot.
+ * Asynchronous on hip_stream. */
+fh_status fh_bc7_exhaustive_labels(const uint32_t *blocks, uint32_t n, int32_t *labels,
+                                   unsigned long long *errors, void *hip_stream);
+
 /* Kernel timing for benchmarks; no reference counterpart (the reference's
  * CompressImageData reports only a whole-job wall time, TexComp.cpp:517).
  * While enabled, every kernel the library launches is bracketed by HIP

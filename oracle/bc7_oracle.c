@@ -1453,12 +1453,17 @@ static void compress_clusters(const shape_sel *sel, const uint32_t px[16],
       }
     }
   }
-  cmode pm;
-  memset(&pm, 0, sizeof(pm));
-  pm.a = &kModes[best_mode < 8 ? best_mode : 0];
-  pm.metric = v4_splat(1.0f);
   memset(out, 0, 16);
-  mode_pack(&pm, &best_pr, out);
+  /* No mode tried (the block-mode mask removed every candidate): the
+   * reference asserts (Compressor.cpp:1811) and is undefined without
+   * asserts; an all-zero block (reserved mode, decodes to zero) is written. */
+  if (best_mode < 8) {
+    cmode pm;
+    memset(&pm, 0, sizeof(pm));
+    pm.a = &kModes[best_mode];
+    pm.metric = v4_splat(1.0f);
+    mode_pack(&pm, &best_pr, out);
+  }
   if (st) {
     st->mode = best_mode;
     st->best_error = best_err;
@@ -1466,6 +1471,75 @@ static void compress_clusters(const shape_sel *sel, const uint32_t px[16],
 }
 
 static void decode_block(const uint8_t blk[16], uint32_t px[16], int spec);
+
+/* One candidate at quality 0, packed: CompressionMode::Compress and Pack for
+ * a fixed (mode, slot) as in ora_bc7_exhaustive_label.  Returns the internal
+ * error. */
+double ora_bc7_encode_candidate(const uint32_t px[16], int mode, int slot, uint8_t out[16]) {
+  clus c;
+  clus_init(&c, px);
+  cmode m;
+  memset(&m, 0, sizeof(m));
+  m.a = &kModes[mode];
+  m.metric = v4_splat(1.0f);
+  params pr;
+  double e;
+  if (m.a->has_rot) {
+    memset(&pr, 0, sizeof(pr));
+    memset(pr.idx, 0xFF, sizeof(pr.idx));
+    memset(pr.pbc, 0xFF, sizeof(pr.pbc));
+    clus_set_partition(&c, 0);
+    m.rot = mode == 4 ? slot >> 1 : slot;
+    m.idx_mode = mode == 4 ? slot & 1 : 0;
+    e = compress_cluster_alpha(&m, &c, &pr.p1[0], &pr.p2[0], pr.idx[0], pr.aidx);
+    pr.rot = m.rot;
+    pr.idx_mode = m.idx_mode;
+    pr.shape = 0;
+  } else {
+    c.shape = (uint32_t)(m.a->subsets > 1 ? slot : 0);
+    c.nparts = (uint32_t)m.a->subsets;
+    e = mode_compress(&m, &pr, (int)c.shape, 0, &c);
+  }
+  memset(out, 0, 16);
+  cmode pm;
+  memset(&pm, 0, sizeof(pm));
+  pm.a = &kModes[mode];
+  pm.metric = v4_splat(1.0f);
+  mode_pack(&pm, &pr, out);
+  return e;
+}
+
+/* Exhaustive quality-0 search over every (mode, slot) the candidate ranker
+ * scores (no reference counterpart; built from CompressionMode::Compress,
+ * Compressor.cpp:1300-1371, one rotation / index mode at a time): slot =
+ * shape for modes 0-3 and 7, rotation * 2 + index mode for mode 4, rotation
+ * for mode 5, 0 for mode 6.  Each candidate is packed and decoded with the
+ * reference decoder; returns mode * 64 + slot of the smallest decoded
+ * premultiplied error (ora_block_premul_err); ties go to the earlier mode in
+ * CompressClusters' order, then the lower slot. */
+int ora_bc7_exhaustive_label(const uint32_t px[16], uint64_t *err_out) {
+  static const int order[8] = {0, 2, 1, 3, 7, 4, 5, 6};
+  static const int nslots[8] = {16, 64, 64, 64, 8, 4, 1, 64};
+  uint64_t best = UINT64_MAX;
+  int label = -1;
+  for (int mi = 0; mi < 8; mi++) {
+    const int mode = order[mi];
+    for (int slot = 0; slot < nslots[mode]; slot++) {
+      uint8_t blk[16];
+      uint32_t dec[16];
+      ora_bc7_encode_candidate(px, mode, slot, blk);
+      decode_block(blk, dec, 0);
+      const uint64_t e = ora_block_premul_err(px, dec);
+      if (e < best) {
+        best = e;
+        label = mode * 64 + slot;
+      }
+    }
+  }
+  if (err_out) *err_out = best;
+  return label;
+}
+
 
 void ora_bc7_compress_block(const uint32_t px[16], uint8_t out[16], const ora_bc7_settings *s,
                             uint32_t wm_index, uint64_t block_id, ora_bc7_block_stats *st) {

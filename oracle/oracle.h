@@ -55,6 +55,15 @@ typedef struct {
   int q0_kept;       /* the quality guard replaced the annealed encoding */
 } ora_bc7_block_stats;
 
+/* Exhaustive quality-0 search over all 285 (mode, slot) candidates (the
+ * candidate ranker's outputs): returns mode * 64 + slot with the smallest
+ * decoded premultiplied error (ora_block_premul_err), written to *err. */
+int ora_bc7_exhaustive_label(const uint32_t px[16], uint64_t *err);
+
+/* One (mode, slot) candidate at quality 0, packed into out; returns its
+ * internal error. */
+double ora_bc7_encode_candidate(const uint32_t px[16], int mode, int slot, uint8_t out[16]);
+
 /* Compress one 4x4 block (16 RGBA8 pixels, R in the low byte).
  * Follows CompressBC7Block (Compressor.cpp:1819-1860).
  * wm_index: the watermark table entry for a solid block (the reference's

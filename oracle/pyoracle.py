@@ -76,6 +76,10 @@ def lib():
                                        ctypes.POINTER(ctypes.c_int)]
         L.ora_single_color.restype = ctypes.c_double
         L.ora_table_partitions.argtypes = [vp] * 5
+        L.ora_bc7_exhaustive_label.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
+        L.ora_bc7_exhaustive_label.restype = ctypes.c_int
+        L.ora_bc7_encode_candidate.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+        L.ora_bc7_encode_candidate.restype = ctypes.c_double
         L.ora_counter_rng.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
         L.ora_counter_rng.restype = ctypes.c_uint32
         L.ora_fit_key.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32]
@@ -137,6 +141,22 @@ def psnr(a, b):
     return lib().ora_psnr(_p(a), _p(b), a.shape[1], a.shape[0])
 
 
+def bc7_exhaustive_label(px16):
+    """(label = mode * 64 + slot, error) of the exhaustive quality-0 search."""
+    px = np.ascontiguousarray(px16, dtype=np.uint32)
+    e = ctypes.c_uint64(0)
+    lab = lib().ora_bc7_exhaustive_label(_p(px), ctypes.byref(e))
+    return lab, e.value
+
+
+def bc7_encode_candidate(px16, mode, slot):
+    """(16-byte block, internal error) of one quality-0 candidate."""
+    px = np.ascontiguousarray(px16, dtype=np.uint32)
+    out = np.zeros(16, dtype=np.uint8)
+    e = lib().ora_bc7_encode_candidate(_p(px), mode, slot, _p(out))
+    return out, e
+
+
 def block_premul_err(src16, dec16):
     a = np.ascontiguousarray(src16, dtype=np.uint32)
     b = np.ascontiguousarray(dec16, dtype=np.uint32)
@@ -147,3 +167,77 @@ def premul_sse(a, b):
     a = np.ascontiguousarray(a, dtype=np.uint32)
     b = np.ascontiguousarray(b, dtype=np.uint32)
     return lib().ora_premul_sse(_p(a), _p(b), a.size)
+
+
+# --- candidate-ranking MLP oracle (mlp_oracle.c) ----------------------------
+
+MLP_IN, MLP_H, MLP_OUT = 64, 256, 512
+MLP_NPARAMS = 64 * 256 + 256 + 256 * 256 + 256 + 256 * 512 + 512
+
+
+def _mlp_lib():
+    L = lib()
+    if not getattr(L, "_mlp_ready", False):
+        vp = ctypes.c_void_p
+        L.ora_bf16_round.argtypes = [ctypes.c_float]
+        L.ora_bf16_round.restype = ctypes.c_float
+        L.ora_mlp_valid.argtypes = [ctypes.c_int]
+        L.ora_mlp_features.argtypes = [vp, ctypes.c_int, vp]
+        L.ora_mlp_forward.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]
+        L.ora_mlp_xent.argtypes = [vp, vp, ctypes.c_int, vp]
+        L.ora_mlp_xent.restype = ctypes.c_double
+        L.ora_mlp_loss_grad.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]
+        L.ora_mlp_loss_grad.restype = ctypes.c_double
+        L.ora_adam_step.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                    ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        L._mlp_ready = True
+    return L
+
+
+def mlp_valid_mask():
+    L = _mlp_lib()
+    return np.array([L.ora_mlp_valid(j) for j in range(MLP_OUT)], dtype=bool)
+
+
+def mlp_features(blocks):
+    """(n, 16) uint32 blocks -> (n, 64) float32 inputs (RGBA / 255, bf16)."""
+    b = np.ascontiguousarray(blocks, dtype=np.uint32)
+    x = np.zeros((b.shape[0], MLP_IN), dtype=np.float32)
+    _mlp_lib().ora_mlp_features(_p(b), b.shape[0], _p(x))
+    return x
+
+
+def mlp_forward(params, x, round_bf16=True):
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    B = x.shape[0]
+    h1 = np.zeros((B, MLP_H), np.float32)
+    h2 = np.zeros((B, MLP_H), np.float32)
+    lg = np.zeros((B, MLP_OUT), np.float32)
+    _mlp_lib().ora_mlp_forward(_p(params), _p(x), B, int(round_bf16), _p(h1), _p(h2), _p(lg))
+    return h1, h2, lg
+
+
+def mlp_xent(logits, labels, want_grad=True):
+    logits = np.ascontiguousarray(logits, dtype=np.float32)
+    labels = np.ascontiguousarray(labels, dtype=np.int32)
+    B = logits.shape[0]
+    d = np.zeros_like(logits) if want_grad else None
+    loss = _mlp_lib().ora_mlp_xent(_p(logits), _p(labels), B, _p(d) if want_grad else None)
+    return loss, d
+
+
+def mlp_loss_grad(params, x, labels, round_bf16=True, want_grad=True):
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    labels = np.ascontiguousarray(labels, dtype=np.int32)
+    g = np.zeros(MLP_NPARAMS, np.float32) if want_grad else None
+    loss = _mlp_lib().ora_mlp_loss_grad(_p(params), _p(x), _p(labels), x.shape[0],
+                                        int(round_bf16), _p(g) if want_grad else None)
+    return loss, g
+
+
+def adam_step(w, m, v, g, t, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+    """In place on float32 arrays w, m, v."""
+    g = np.ascontiguousarray(g, dtype=np.float32)
+    _mlp_lib().ora_adam_step(_p(w), _p(m), _p(v), _p(g), w.size, t, lr, beta1, beta2, eps)

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_functions():
     text = open(os.path.join(ROOT, "include", "fasthip.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_ ]+\*?\s*(fh_[a-z_]+)\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_ ]+\*?\s*(fh_[a-z0-9_]+)\(", text, re.M)))
 
 
 def test_library_exports_header_symbols():
