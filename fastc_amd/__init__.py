@@ -27,6 +27,7 @@ BLOCK_BYTES = {0: 8, 1: 16, 2: 8, 3: 16}
 FH_OK = 0
 FH_FLAG_CALLER_WATERMARK = 1
 FH_FLAG_NO_Q0_GUARD = 2
+FH_FLAG_LEARNED = 4
 STATUS_NAMES = {0: "FH_OK", 1: "FH_ERR_INVALID_ARGUMENT", 2: "FH_ERR_NO_DATA",
                 3: "FH_ERR_BLOCK_DIMS", 4: "FH_ERR_UNKNOWN_FORMAT",
                 5: "FH_ERR_BUFFER_TOO_SMALL", 6: "FH_ERR_UNSUPPORTED",
@@ -46,7 +47,20 @@ class FhJob(ctypes.Structure):
                 ("error_metric", ctypes.c_uint32),
                 ("seed", ctypes.c_uint64),
                 ("wm_counter", ctypes.POINTER(ctypes.c_uint32)),
-                ("flags", ctypes.c_uint32)]
+                ("flags", ctypes.c_uint32),
+                ("ranker", ctypes.c_void_p),
+                ("ranker_topk", ctypes.c_uint32),
+                ("ranker_threshold", ctypes.c_float)]
+
+
+class FhTrainConfig(ctypes.Structure):
+    _fields_ = [("epochs", ctypes.c_uint32), ("batch", ctypes.c_uint32), ("lr", ctypes.c_float),
+                ("shuffle_seed", ctypes.c_uint64)]
+
+
+class FhTrainReport(ctypes.Structure):
+    _fields_ = [("first_loss", ctypes.c_double), ("epoch_loss", ctypes.c_double * 16),
+                ("steps", ctypes.c_uint32), ("ms", ctypes.c_float)]
 
 
 class FastHipError(RuntimeError):
@@ -86,13 +100,32 @@ def lib():
         L.fh_bc7_exhaustive_labels.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
                                                ctypes.c_void_p, ctypes.c_void_p]
         L.fh_bc7_exhaustive_labels.restype = ctypes.c_int
+        vp = ctypes.c_void_p
+        L.fh_ranker_create.argtypes = [ctypes.POINTER(vp)]
+        L.fh_ranker_destroy.argtypes = [vp]
+        L.fh_ranker_num_params.restype = ctypes.c_size_t
+        L.fh_ranker_init.argtypes = [vp, ctypes.c_uint64]
+        L.fh_ranker_set_params.argtypes = [vp, vp, ctypes.c_size_t]
+        L.fh_ranker_get_params.argtypes = [vp, vp, ctypes.c_size_t]
+        L.fh_ranker_forward.argtypes = [vp, vp, ctypes.c_uint32, vp, vp]
+        L.fh_ranker_gradients.argtypes = [vp, vp, vp, ctypes.c_uint32, vp,
+                                          ctypes.POINTER(ctypes.c_double)]
+        L.fh_ranker_train.argtypes = [vp, vp, vp, ctypes.c_uint32, ctypes.POINTER(FhTrainConfig),
+                                      ctypes.POINTER(FhTrainReport)]
+        for name in ("fh_ranker_create", "fh_ranker_destroy", "fh_ranker_init",
+                     "fh_ranker_set_params", "fh_ranker_get_params", "fh_ranker_forward",
+                     "fh_ranker_gradients", "fh_ranker_train"):
+            getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
 
 
 EXPORTED_SYMBOLS = ["fh_compress", "fh_compress_device", "fh_decompress",
                     "fh_compressed_size", "fh_last_error", "fh_device_available",
-                    "fh_set_kernel_timing", "fh_kernel_time", "fh_bc7_exhaustive_labels"]
+                    "fh_set_kernel_timing", "fh_kernel_time", "fh_bc7_exhaustive_labels",
+                    "fh_ranker_create", "fh_ranker_destroy", "fh_ranker_num_params",
+                    "fh_ranker_init", "fh_ranker_set_params", "fh_ranker_get_params",
+                    "fh_ranker_forward", "fh_ranker_gradients", "fh_ranker_train"]
 
 
 def _check(status):
@@ -111,7 +144,7 @@ def compressed_size(fmt, width, height):
 
 
 def _job(fmt, inp, out, out_size, width, height, rect, quality, seed, block_modes,
-         wm_counter, q0_guard=True):
+         wm_counter, q0_guard=True, ranker=None, topk=4, threshold=0.6):
     x0, y0, x1, y1 = rect if rect is not None else (0, 0, width, height)
     j = FhJob()
     j.format = fmt
@@ -125,6 +158,11 @@ def _job(fmt, inp, out, out_size, width, height, rect, quality, seed, block_mode
     j.error_metric = 0
     j.seed = seed
     j.flags = 0 if q0_guard else FH_FLAG_NO_Q0_GUARD
+    if ranker is not None:
+        j.flags |= FH_FLAG_LEARNED
+        j.ranker = ranker.handle
+        j.ranker_topk = topk
+        j.ranker_threshold = threshold
     if wm_counter is not None:
         j.wm_counter = ctypes.pointer(wm_counter)
         j.flags |= FH_FLAG_CALLER_WATERMARK
@@ -132,7 +170,8 @@ def _job(fmt, inp, out, out_size, width, height, rect, quality, seed, block_mode
 
 
 def compress(img, fmt=eCompressionFormat_BPTC, quality=0, seed=0, block_modes=0xFF,
-             rect=None, out=None, wm_counter=None, q0_guard=True):
+             rect=None, out=None, wm_counter=None, q0_guard=True, ranker=None, topk=4,
+             threshold=0.6):
     """Encode a (h, w) uint32 RGBA8 image (R in the low byte) on the GPU.
 
     Mirrors CompressImageData: returns the compressed bytes (uint8 array of
@@ -140,25 +179,27 @@ def compress(img, fmt=eCompressionFormat_BPTC, quality=0, seed=0, block_modes=0x
     ctypes.c_uint32 holding the BC7 solid-block watermark counter (the
     reference's gWMVal); it is updated in place.  Without it the library's
     process-wide counter is used, like the reference.  `q0_guard=False`
-    drops the BPTC quality guard (FH_FLAG_NO_Q0_GUARD)."""
+    drops the BPTC quality guard (FH_FLAG_NO_Q0_GUARD).  With a `Ranker`, BPTC
+    runs the learned candidate pruning (FH_FLAG_LEARNED) with `topk` and
+    `threshold`."""
     img = np.ascontiguousarray(img, dtype=np.uint32)
     h, w = img.shape
     size = compressed_size(fmt, w, h)
     if out is None:
         out = np.zeros(size, dtype=np.uint8)
     j = _job(fmt, img.ctypes.data, out.ctypes.data, out.nbytes, w, h, rect, quality, seed,
-             block_modes, wm_counter, q0_guard)
+             block_modes, wm_counter, q0_guard, ranker, topk, threshold)
     _check(lib().fh_compress(ctypes.byref(j)))
     return out
 
 
 def compress_device(in_ptr, out_ptr, out_size, width, height, fmt=eCompressionFormat_BPTC,
                     quality=0, seed=0, block_modes=0xFF, stream=0, wm_counter=None,
-                    q0_guard=True):
+                    q0_guard=True, ranker=None, topk=4, threshold=0.6):
     """Encode device-resident data (raw device pointers, e.g. torch
     tensor.data_ptr()) asynchronously on `stream` (a hipStream_t as int)."""
     j = _job(fmt, in_ptr, out_ptr, out_size, width, height, None, quality, seed, block_modes,
-             wm_counter, q0_guard)
+             wm_counter, q0_guard, ranker, topk, threshold)
     _check(lib().fh_compress_device(ctypes.byref(j), ctypes.c_void_p(stream)))
 
 
@@ -184,6 +225,62 @@ def exhaustive_labels_device(blocks_ptr, n, labels_ptr, errors_ptr=0, stream=0):
                                           ctypes.c_void_p(labels_ptr),
                                           ctypes.c_void_p(errors_ptr or None),
                                           ctypes.c_void_p(stream)))
+
+
+class Ranker:
+    """The BC7 candidate ranker (fh_ranker_*): a 64-256-256-512 MLP on the
+    GPU.  Device buffers are raw pointers (e.g. torch tensor.data_ptr())."""
+
+    def __init__(self, seed=None):
+        self._h = ctypes.c_void_p()
+        _check(lib().fh_ranker_create(ctypes.byref(self._h)))
+        self.num_params = lib().fh_ranker_num_params()
+        if seed is not None:
+            self.init(seed)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.fh_ranker_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def init(self, seed):
+        _check(lib().fh_ranker_init(self._h, seed))
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        _check(lib().fh_ranker_set_params(self._h, p.ctypes.data, p.size))
+
+    def get_params(self):
+        p = np.zeros(self.num_params, dtype=np.float32)
+        _check(lib().fh_ranker_get_params(self._h, p.ctypes.data, p.size))
+        return p
+
+    def forward(self, blocks_ptr, n, logits_ptr, stream=0):
+        _check(lib().fh_ranker_forward(self._h, ctypes.c_void_p(blocks_ptr), n,
+                                       ctypes.c_void_p(logits_ptr), ctypes.c_void_p(stream)))
+
+    def gradients(self, blocks_ptr, labels_ptr, n):
+        g = np.zeros(self.num_params, dtype=np.float32)
+        loss = ctypes.c_double(0)
+        _check(lib().fh_ranker_gradients(self._h, ctypes.c_void_p(blocks_ptr),
+                                         ctypes.c_void_p(labels_ptr), n, g.ctypes.data,
+                                         ctypes.byref(loss)))
+        return loss.value, g
+
+    def train(self, blocks_ptr, labels_ptr, n, epochs, batch, lr, shuffle_seed=0):
+        cfg = FhTrainConfig(epochs, batch, lr, shuffle_seed)
+        rep = FhTrainReport()
+        _check(lib().fh_ranker_train(self._h, ctypes.c_void_p(blocks_ptr),
+                                     ctypes.c_void_p(labels_ptr), n, ctypes.byref(cfg),
+                                     ctypes.byref(rep)))
+        return {"first_loss": rep.first_loss,
+                "epoch_loss": list(rep.epoch_loss)[:min(epochs, 16)],
+                "steps": rep.steps, "ms": rep.ms}
 
 
 def decompress(data, width, height, fmt=eCompressionFormat_BPTC):

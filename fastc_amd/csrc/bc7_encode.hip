@@ -1073,17 +1073,25 @@ __device__ __forceinline__ void enqueue_candidates(Work &w, bool valid, int type
 // 1819-1860); one wave per four blocks, lane i scores shape i.  Emits the
 // candidates into per-type queues (wave-aggregated appends).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(64) select_kernel(BlockRange r, Bc7Params prm, Work w) {
+// list, if given, holds the blocks to select (list_count of them, a device
+// word); otherwise all blocks of the range.
+__global__ void __launch_bounds__(64) select_kernel(BlockRange r, Bc7Params prm, Work w,
+                                                    const uint32_t *list,
+                                                    const uint32_t *list_count) {
   __shared__ uint32_t s_px[4][16];
   __shared__ uint32_t s_sel[4];
   __shared__ uint16_t s_cd[4][kMaxCands];
   const int lane = threadIdx.x;
-  const uint32_t base = blockIdx.x * 4;  // relative block index
+  const uint32_t base = blockIdx.x * 4;  // position in the list or the range
+  const uint32_t total = list ? *list_count : r.num_blocks;
+  if (base >= total) return;  // whole workgroup
+  auto rel_of = [&](uint32_t i) { return list ? list[i] : i; };
   {
     const int row = lane >> 4, x = lane & 15;
     const int blk = x >> 2;
-    const uint32_t rel = base + blk;
-    if (rel < r.num_blocks) {
+    const uint32_t i = base + blk;
+    if (i < total) {
+      const uint32_t rel = rel_of(i);
       const uint32_t b = r.first_block + rel;
       const uint32_t bx = b % r.blocks_wide, by = b / r.blocks_wide;
       s_px[blk][row * 4 + (x & 3)] = r.img[(size_t)(by * 4 + row) * r.width + bx * 4 + (x & 3)];
@@ -1091,8 +1099,8 @@ __global__ void __launch_bounds__(64) select_kernel(BlockRange r, Bc7Params prm,
   }
   __syncthreads();
   for (int blk = 0; blk < 4; blk++) {
-    const uint32_t rel = base + blk;
-    if (rel >= r.num_blocks) break;  // uniform
+    if (base + blk >= total) break;  // uniform
+    const uint32_t rel = rel_of(base + blk);
     uint32_t px[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) px[i] = s_px[blk][i];
@@ -1187,14 +1195,121 @@ __global__ void __launch_bounds__(64) select_kernel(BlockRange r, Bc7Params prm,
   __syncthreads();
   // enqueue: lane = (block, candidate slot)
   const int blk = lane >> 4, c = lane & 15;
-  const uint32_t rel = base + blk;
+  const uint32_t rel = base + blk < total ? rel_of(base + blk) : 0;
   bool valid = false;
   int type = 0;
-  if (rel < r.num_blocks && c < sel_count(s_sel[blk])) {
+  if (base + blk < total && c < sel_count(s_sel[blk])) {
     valid = true;
     type = kTypeOfMode[s_cd[blk][c] & 7];
   }
   enqueue_candidates(w, valid, type, (rel << 4) | (uint32_t)c);
+}
+
+// ---------------------------------------------------------------------------
+// Kernel 1b (learned candidate pruning): one wave per block reads the
+// ranker's 512 logits (logit = mode * 64 + slot) for the chunk of blocks
+// starting at rel0.  Solid and transparent blocks are classified as above.
+// If the top softmax probability over the real candidates of the allowed
+// modes reaches the threshold, the top-k candidates (ties to the lower
+// logit) become the block's slots; otherwise the block goes to the fallback
+// list, which select_kernel then searches with BoxSelection.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool logit_real(int j) {
+  const int mode = j >> 6, slot = j & 63;
+  switch (mode) {
+    case 0: return slot < 16;
+    case 4: return slot < 8;
+    case 5: return slot < 4;
+    case 6: return slot == 0;
+    default: return true;
+  }
+}
+
+__device__ __forceinline__ uint32_t cdesc_of_logit(int j) {
+  const int mode = j >> 6, slot = j & 63;
+  const int shape = kModes[mode].subsets > 1 ? slot : 0;
+  const int rot = mode == 4 ? (slot >> 1) : (mode == 5 ? slot : 0);
+  const int im = mode == 4 ? (slot & 1) : 0;
+  return cdesc_pack(mode, shape, rot, im, 0);
+}
+
+__global__ void __launch_bounds__(256) learned_select_kernel(BlockRange r, Bc7Params prm, Work w,
+                                                             uint32_t rel0, uint32_t nchunk,
+                                                             const float *logits,
+                                                             uint32_t *fallback,
+                                                             uint32_t *fallback_count) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t k = blockIdx.x * 4 + wave;
+  if (k >= nchunk) return;  // whole wave
+  const uint32_t rel = rel0 + k;
+  const uint32_t b = r.first_block + rel;
+  const uint32_t bx = b % r.blocks_wide, by = b / r.blocks_wide;
+  const uint32_t mine =
+      r.img[(size_t)(by * 4 + ((lane & 15) >> 2)) * r.width + bx * 4 + (lane & 3)];
+  const uint32_t p0 = __shfl(mine, 0);
+  const bool solid = !__ballot((lane < 16) && mine != p0);
+  const bool transparent = !__ballot((lane < 16) && (mine >> 24) != 0);
+  if (solid || transparent) {
+    if (lane == 0) w.sel[rel] = sel_pack(solid ? 0 : 1, 0);
+    return;
+  }
+  float z[8];
+  bool ok[8];
+  float mx = -INFINITY;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const int j = lane + 64 * i;
+    ok[i] = logit_real(j) && ((prm.block_modes >> (j >> 6)) & 1);
+    z[i] = logits[(size_t)k * 512 + j];
+    if (ok[i]) mx = fmaxf(mx, z[i]);
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+  float sum = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+    if (ok[i]) sum += expf(z[i] - mx);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) sum += __shfl_xor(sum, off);
+  // top probability = exp(0) / sum
+  if (!(mx > -INFINITY) || 1.0f / sum < prm.threshold) {
+    if (lane == 0) {
+      const uint32_t at = atomicAdd(fallback_count, 1u);
+      fallback[at] = rel;
+    }
+    return;
+  }
+  const int kk = (int)(prm.topk < 1 ? 1 : (prm.topk > 8 ? 8 : prm.topk));
+  uint32_t my_desc = 0;
+  int count = 0;
+  for (int t = 0; t < kk; t++) {
+    float bv = -INFINITY;
+    int bj = 1 << 30;
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+      if (ok[i] && z[i] > bv) {  // i ascending: the lower logit wins ties
+        bv = z[i];
+        bj = lane + 64 * i;
+      }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      const float ov = __shfl_xor(bv, off);
+      const int oj = __shfl_xor(bj, off);
+      if (ov > bv || (ov == bv && oj < bj)) {
+        bv = ov;
+        bj = oj;
+      }
+    }
+    if (!(bv > -INFINITY)) break;  // uniform
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+      if (lane + 64 * i == bj) ok[i] = false;
+    if (lane == t) my_desc = cdesc_of_logit(bj);
+    count++;
+  }
+  if (lane < count) w.cdesc[(size_t)rel * kMaxCands + lane] = (uint16_t)my_desc;
+  if (lane == 0) w.sel[rel] = sel_pack(2, count);
+  enqueue_candidates(w, lane < count, kTypeOfMode[my_desc & 7], (rel << 4) | (uint32_t)lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -1807,6 +1922,64 @@ __global__ void __launch_bounds__(256) guard_kernel(BlockRange r, const uint4 *q
   if (premul_err(px, dz) < premul_err(px, da)) out[b] = z;
 }
 
+// Final selection of the learned search: every candidate's block is packed
+// and decoded with the reference decoder, and the smallest premul_err wins
+// (ties to the lower slot).  Pass 0 (before annealing) keeps the quality-0
+// winner and its error; the final pass writes its winner unless the
+// quality-0 one decodes strictly better, so no block is worse than at
+// quality 0.  Solid and transparent blocks are written as pack_kernel does.
+__global__ void __launch_bounds__(256) pack_decoded_kernel(BlockRange r, Bc7Scratch scr, Work w,
+                                                           uint4 *q0blk,
+                                                           unsigned long long *q0err,
+                                                           int final_pass) {
+  const uint32_t rel = blockIdx.x * blockDim.x + threadIdx.x;
+  if (rel >= r.num_blocks) return;
+  const uint32_t b = r.first_block + rel;
+  const uint32_t sel = w.sel[rel];
+  uint4 best = make_uint4(0, 0, 0, 0);
+  if (sel_cls(sel) != 2) {
+    if (!final_pass) return;
+    BitWriter bw;
+    if (sel_cls(sel) == 0) {
+      const uint32_t bx = b % r.blocks_wide, by = b / r.blocks_wide;
+      const uint32_t pixel = r.img[(size_t)(by * 4) * r.width + bx * 4];
+      const uint32_t wm = (scr.chunk_prefix[rel / 256] + scr.local_prefix[rel]) % 9;
+      pack_solid(pixel, wm, bw);
+    } else {
+      bw.put(1u << 6, 7);  // WriteTransparentBlock (Compressor.cpp:1416-1421)
+    }
+    best = make_uint4((uint32_t)bw.lo, (uint32_t)(bw.lo >> 32), (uint32_t)bw.hi,
+                      (uint32_t)(bw.hi >> 32));
+  } else {
+    uint32_t px[16];
+    load_block(r, rel, px);
+    unsigned long long best_err = ~0ull;
+    const int count = sel_count(sel);
+    for (int i = 0; i < count; i++) {
+      Cand k;
+      cand_from_desc(w.cdesc[(size_t)rel * kMaxCands + i], k);
+      BitWriter bw;
+      pack_mode(k.mode, k.shape, w.rec[(size_t)rel * kMaxCands + i], bw);
+      const uint4 blk = make_uint4((uint32_t)bw.lo, (uint32_t)(bw.lo >> 32), (uint32_t)bw.hi,
+                                   (uint32_t)(bw.hi >> 32));
+      uint32_t dec[16];
+      bc7dec::decode_block(blk, dec);
+      const unsigned long long e = premul_err(px, dec);
+      if (e < best_err) {
+        best_err = e;
+        best = blk;
+      }
+    }
+    if (!final_pass) {
+      q0blk[rel] = best;
+      q0err[rel] = best_err;
+      return;
+    }
+    if (q0err && q0err[rel] < best_err) best = q0blk[rel];
+  }
+  reinterpret_cast<uint4 *>(r.out)[b] = best;
+}
+
 // ---------------------------------------------------------------------------
 // Exhaustive quality-0 search, the training labels of the candidate ranker:
 // every mode with every partition shape, rotation and index mode (the 285
@@ -1969,12 +2142,20 @@ hipError_t bc7_upload_tables(int device) {
   return hipSuccess;
 }
 
-size_t bc7_work_bytes(uint32_t num_blocks) {
+size_t bc7_work_bytes(uint32_t num_blocks, bool learned) {
   const size_t n = num_blocks;
   // counters, sel, cdesc, rec, queues, astate, q0, alignment slack
-  return 64 * 4 + n * 4 + n * bc7::kMaxCands * 2 + n * bc7::kMaxCands * sizeof(bc7::CandRec) +
-         n * bc7::kNumTypes * bc7::kMaxCands * 4 + n * bc7::kMaxCands * sizeof(bc7::AState) +
-         n * 16 + 8 * 256;
+  size_t bytes = 64 * 4 + n * 4 + n * bc7::kMaxCands * 2 +
+                 n * bc7::kMaxCands * sizeof(bc7::CandRec) +
+                 n * bc7::kNumTypes * bc7::kMaxCands * 4 +
+                 n * bc7::kMaxCands * sizeof(bc7::AState) + n * 16 + 8 * 256;
+  if (learned) {
+    // q0 errors, fallback list, one chunk of network activations and logits
+    const size_t c = n < kBc7MlpChunk ? n : kBc7MlpChunk;
+    const size_t rows = (c + 127) & ~(size_t)127;
+    bytes += n * 8 + n * 4 + rows * (64 * 2 + 256 * 2 * 2 + 512 * 4) + 8 * 256;
+  }
+  return bytes;
 }
 
 hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch &s, void *work,
@@ -2006,6 +2187,22 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   w.astate = reinterpret_cast<bc7::AState *>(carve(nc * sizeof(bc7::AState)));
   w.astate_cap = (uint32_t)nc;
   uint4 *q0 = reinterpret_cast<uint4 *>(carve((size_t)n * 16));
+  const bool learned = p.ranker != nullptr;
+  unsigned long long *q0err = nullptr;
+  uint32_t *fallback = nullptr;
+  uint16_t *mX = nullptr, *mh1 = nullptr, *mh2 = nullptr;
+  float *mlogits = nullptr;
+  const uint32_t mchunk = n < kBc7MlpChunk ? n : kBc7MlpChunk;
+  if (learned) {
+    const size_t rows = ((size_t)mchunk + 127) & ~(size_t)127;
+    q0err = reinterpret_cast<unsigned long long *>(carve((size_t)n * 8));
+    fallback = reinterpret_cast<uint32_t *>(carve((size_t)n * 4));
+    mX = reinterpret_cast<uint16_t *>(carve(rows * 64 * 2));
+    mh1 = reinterpret_cast<uint16_t *>(carve(rows * 256 * 2));
+    mh2 = reinterpret_cast<uint16_t *>(carve(rows * 256 * 2));
+    mlogits = reinterpret_cast<float *>(carve(rows * 512 * 4));
+  }
+  uint32_t *fallback_count = w.counters + 20;
   const uint32_t nchunks = (n + 255) / 256;
   const uint32_t ngroups = (n + 3) / 4;
   const uint32_t persistent = (uint32_t)g_num_cus[dev] * 16;
@@ -2016,12 +2213,52 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   hipLaunchKernelGGL(bc7::solid_scan_kernel, dim3(nchunks), dim3(256), 0, stream, r, s);
   hipLaunchKernelGGL(bc7::chunk_scan_kernel, dim3(1), dim3(1024), 0, stream, s);
   timing_end(t, kTagBc7Scan, stream);
-  t = timing_begin(stream);
-  hipLaunchKernelGGL(bc7::select_kernel, dim3(ngroups), dim3(64), 0, stream, r, p, w);
-  timing_end(t, kTagBc7Select, stream);
+  if (learned) {
+    // rank each chunk's candidates, keep the confident top-k, then run
+    // BoxSelection over the blocks the ranker was not sure about
+    for (uint32_t c0 = 0; c0 < n; c0 += mchunk) {
+      const uint32_t cn = n - c0 < mchunk ? n - c0 : mchunk;
+      BlockRange rc = r;
+      rc.first_block = r.first_block + c0;
+      rc.num_blocks = cn;
+      t = timing_begin(stream);
+      e = ranker_logits(*p.ranker, rc, cn, mX, mh1, mh2, mlogits, stream);
+      timing_end(t, kTagBc7Rank, stream);
+      if (e != hipSuccess) return e;
+      t = timing_begin(stream);
+      hipLaunchKernelGGL(bc7::learned_select_kernel, dim3((cn + 3) / 4), dim3(256), 0, stream, r,
+                         p, w, c0, cn, (const float *)mlogits, fallback, fallback_count);
+      timing_end(t, kTagBc7Select, stream);
+    }
+    t = timing_begin(stream);
+    hipLaunchKernelGGL(bc7::select_kernel, dim3(ngroups), dim3(64), 0, stream, r, p, w,
+                       (const uint32_t *)fallback, (const uint32_t *)fallback_count);
+    timing_end(t, kTagBc7Select, stream);
+  } else {
+    t = timing_begin(stream);
+    hipLaunchKernelGGL(bc7::select_kernel, dim3(ngroups), dim3(64), 0, stream, r, p, w,
+                       (const uint32_t *)nullptr, (const uint32_t *)nullptr);
+    timing_end(t, kTagBc7Select, stream);
+  }
   t = timing_begin(stream);
   hipLaunchKernelGGL(bc7::setup_kernel, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
   timing_end(t, kTagBc7Setup, stream);
+  if (learned) {
+    if (p.quality > 0) {
+      t = timing_begin(stream);
+      hipLaunchKernelGGL(bc7::pack_decoded_kernel, pack_grid, dim3(256), 0, stream, r, s, w, q0,
+                         q0err, 0);
+      timing_end(t, kTagBc7Pack, stream);
+      t = timing_begin(stream);
+      hipLaunchKernelGGL(bc7::anneal_kernel, dim3(persistent), dim3(64), 0, stream, r, p, w);
+      timing_end(t, kTagBc7Anneal, stream);
+    }
+    t = timing_begin(stream);
+    hipLaunchKernelGGL(bc7::pack_decoded_kernel, pack_grid, dim3(256), 0, stream, r, s, w, q0,
+                       p.quality > 0 ? q0err : (unsigned long long *)nullptr, 1);
+    timing_end(t, kTagBc7Pack, stream);
+    return hipGetLastError();
+  }
   if (guard) {
     t = timing_begin(stream);
     hipLaunchKernelGGL(bc7::pack_kernel, pack_grid, dim3(256), 0, stream, r, s, w, q0);

@@ -153,6 +153,13 @@ fh_status validate(const fh_job *j, bool device) {
     return fail(FH_ERR_UNSUPPORTED, "only the uniform error metric is implemented");
   if ((j->flags & FH_FLAG_CALLER_WATERMARK) && !j->wm_counter)
     return fail(FH_ERR_INVALID_ARGUMENT, "FH_FLAG_CALLER_WATERMARK needs wm_counter");
+  if ((j->flags & FH_FLAG_LEARNED) && j->format == FH_FORMAT_BPTC) {
+    if (!j->ranker) return fail(FH_ERR_INVALID_ARGUMENT, "FH_FLAG_LEARNED needs a ranker");
+    if (j->ranker_topk > 8) return fail(FH_ERR_INVALID_ARGUMENT, "ranker_topk must be 1..8");
+    int dev = -1;
+    if (hipGetDevice(&dev) == hipSuccess && dev != j->ranker->net.device)
+      return fail(FH_ERR_INVALID_ARGUMENT, "ranker belongs to another device");
+  }
   return FH_OK;
 }
 
@@ -175,7 +182,8 @@ fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8
       const uint32_t nchunks = (per + 255) / 256;
       const size_t prefix_bytes = ((size_t)per + 255) & ~(size_t)255;
       const size_t work_off = (prefix_bytes + (size_t)nchunks * 4 + 255) & ~(size_t)255;
-      const size_t need = work_off + bc7_work_bytes(per) + 256;
+      const bool learned = (j->flags & FH_FLAG_LEARNED) != 0;
+      const size_t need = work_off + bc7_work_bytes(per, learned) + 256;
       e = grow(&st->scratch, &st->scratch_size, need);
       if (e != hipSuccess) return fail_hip(e, "scratch allocation");
       uint8_t *sbase = reinterpret_cast<uint8_t *>(st->scratch);
@@ -198,6 +206,9 @@ fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8
       p.seed = j->seed;
       p.block_id_base = 0;
       p.q0_guard = (j->flags & FH_FLAG_NO_Q0_GUARD) ? 0u : 1u;
+      p.ranker = learned ? &j->ranker->net : nullptr;
+      p.topk = j->ranker_topk ? j->ranker_topk : 4;
+      p.threshold = j->ranker_threshold;
       // jobs larger than one call's work area run as consecutive calls on the
       // stream; the watermark counter carries across them in order
       for (uint32_t done = 0; done < count; done += chunk) {
@@ -245,6 +256,8 @@ int current_device(fh_status *s) {
 }
 
 }  // namespace
+
+void set_error(const char *msg) { t_last_error = msg; }
 
 const uint32_t *bc7_single_color_table(int device) {
   DeviceState *st = device_state(device);

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 namespace fh {
 
 // Launch parameters shared by the block encoders.  Blocks are addressed in
@@ -18,12 +20,18 @@ struct BlockRange {
   uint32_t num_blocks;
 };
 
+struct Ranker;
+
 struct Bc7Params {
   uint32_t quality;      // annealing steps
   uint32_t block_modes;  // mask of allowed BC7 modes
   uint64_t seed;         // annealing stream key
   uint64_t block_id_base;  // added to the raster block index to key the stream
   uint32_t q0_guard;     // keep the quality-0 block where annealing decodes worse
+  // learned candidate pruning (null = the reference's BoxSelection search)
+  const Ranker *ranker;
+  uint32_t topk;         // candidates searched per confident block (1..8)
+  float threshold;       // below this top softmax probability: BoxSelection
 };
 
 // Device scratch needed by the BC7 watermark prefix scan.
@@ -38,8 +46,12 @@ struct Bc7Scratch {
 const uint32_t *bc7_single_color_table(int device);
 
 // Device work area bc7_encode needs for num_blocks blocks (candidate
-// records, queues and annealing states).
-size_t bc7_work_bytes(uint32_t num_blocks);
+// records, queues and annealing states, and with the ranker its
+// activations).
+size_t bc7_work_bytes(uint32_t num_blocks, bool learned);
+
+// Blocks per ranker forward pass inside bc7_encode.
+constexpr uint32_t kBc7MlpChunk = 1u << 18;
 
 // Largest block count one bc7_encode call takes; callers split bigger jobs.
 constexpr uint32_t kBc7MaxBlocksPerCall = 1u << 20;
@@ -60,6 +72,57 @@ hipError_t etc1_encode(const BlockRange &r, hipStream_t stream);
 hipError_t bc7_decode(const uint8_t *in, uint32_t *out, uint32_t width, uint32_t height,
                       hipStream_t stream);
 
+// BC7 candidate ranker (mlp.hip): 64 -> 256 -> 256 -> 512 MLP, fp32 master
+// weights with Adam moments, bf16 copies for the matrix cores.
+size_t ranker_num_params();
+
+struct Ranker {
+  int device = 0;
+  float *master = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+  uint16_t *wbf = nullptr;
+  uint16_t *w1_io = nullptr, *w2_io = nullptr, *w3_io = nullptr;  // [in][out]
+  uint16_t *w1_oi = nullptr, *w2_oi = nullptr, *w3_oi = nullptr;  // [out][in]
+  void *work = nullptr;
+  size_t work_size = 0;
+  bool work_training = false;
+  int step = 0;
+  // views into work, valid after forward()
+  uint16_t *X = nullptr, *XT = nullptr, *h1 = nullptr, *h1T = nullptr, *h2 = nullptr,
+           *h2T = nullptr, *dl = nullptr, *dlT = nullptr, *d2 = nullptr, *d2T = nullptr,
+           *d1T = nullptr;
+  float *logits = nullptr, *loss = nullptr;
+
+  hipError_t init(int dev);
+  void release();
+  hipError_t refresh_bf16(hipStream_t s);
+  // workspace for `rows` rows (a multiple of 32)
+  hipError_t reserve(uint32_t rows, bool training);
+  // logits of blocks first_block.. of r (n real rows, `rows` padded);
+  // idx, if given, maps row -> block index relative to r.first_block
+  hipError_t forward(const BlockRange &r, const uint32_t *idx, uint32_t n, uint32_t rows,
+                     bool training, hipStream_t s);
+  // gradients into grad (labels indexed like the rows, through idx if given)
+  hipError_t backward(const int32_t *labels, const uint32_t *idx, uint32_t n, uint32_t rows,
+                      hipStream_t s);
+  hipError_t adam(float lr, hipStream_t s);
+};
+
+}  // namespace fh
+
+// The C-ABI handle of a ranker (include/fasthip.h).
+struct fh_ranker {
+  fh::Ranker net;
+  std::mutex mu;
+};
+
+namespace fh {
+
+// Inference with caller buffers (rows = n rounded up to 128): X rows x 64,
+// h1 and h2 rows x 256 bf16, logits rows x 512 fp32.  Reads only the
+// ranker's weights.
+hipError_t ranker_logits(const Ranker &rk, const BlockRange &r, uint32_t n, uint16_t *X,
+                         uint16_t *h1, uint16_t *h2, float *logits, hipStream_t s);
+
 // Host-side table builders (tables.cpp); both mirror generator programs the
 // reference documents.
 void build_bc7_mode5_table(uint8_t out[256][2]);
@@ -67,6 +130,9 @@ void build_bc7_dxt6_table(uint8_t out[256][2][3]);
 void build_bc7_single_color_table(uint32_t *out);  // kSingleColorEntries entries
 
 constexpr int kSingleColorEntries = 5 * 5 * 4 * 256;
+
+// Sets the message fh_last_error returns on this thread (capi.cpp).
+void set_error(const char *msg);
 
 // Kernel timing (timing.cpp).  timing_begin records a start event on the
 // stream when timing is enabled and returns its slot (-1 otherwise);
@@ -82,6 +148,7 @@ enum KernelTag {
   kTagEtc1,
   kTagBc7Decode,
   kTagBc7Labels,
+  kTagBc7Rank,
   kNumKernelTags
 };
 int timing_begin(hipStream_t stream);

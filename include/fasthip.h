@@ -64,6 +64,9 @@ enum {
   FH_FORMAT_BPTC = 3
 };
 
+/* The BC7 candidate ranker (see the ranker section below). */
+typedef struct fh_ranker fh_ranker;
+
 /* fh_job.flags */
 enum {
   /* Keep the solid-block watermark counter in *wm_counter (caller-owned)
@@ -73,7 +76,15 @@ enum {
    * annealed encoding decodes to a larger (premultiplied-alpha RGB) error
    * than its quality-0 encoding is written as the quality-0 encoding, so no
    * block is worse than at quality 0; the reference has no such guard. */
-  FH_FLAG_NO_Q0_GUARD = 1u << 1
+  FH_FLAG_NO_Q0_GUARD = 1u << 1,
+  /* BPTC: learned candidate pruning (no reference counterpart).  The ranker
+   * scores every block's (mode, slot) candidates; blocks whose top softmax
+   * probability reaches ranker_threshold search only their ranker_topk best
+   * candidates, the others fall back to BoxSelection's candidates.  Among a
+   * block's candidates the one that decodes (reference decoder) to the
+   * smallest premultiplied RGB error is written, quality-0 fits included, so
+   * no block is worse than its quality-0 encoding. */
+  FH_FLAG_LEARNED = 1u << 2
 };
 
 /* One compression job: FasTC::CompressionJob plus BPTC settings. */
@@ -93,6 +104,10 @@ typedef struct fh_job {
   uint64_t seed;          /* annealing random stream key */
   uint32_t *wm_counter;   /* with FH_FLAG_CALLER_WATERMARK: in/out watermark counter */
   uint32_t flags;
+  /* with FH_FLAG_LEARNED */
+  const fh_ranker *ranker;
+  uint32_t ranker_topk;     /* candidates per confident block, 1..8 (0 means 4) */
+  float ranker_threshold;   /* top softmax probability needed to prune */
 } fh_job;
 
 fh_status fh_compress(const fh_job *job);
@@ -118,6 +133,49 @@ int fh_device_available(void);
  * Asynchronous on hip_stream. */
 fh_status fh_bc7_exhaustive_labels(const uint32_t *blocks, uint32_t n, int32_t *labels,
                                    unsigned long long *errors, void *hip_stream);
+
+/* ---- BC7 candidate ranker (no reference counterpart) ----------------------
+ * A 64 -> 256 -> 256 -> 512 MLP (ReLU, bf16 weights on the matrix cores,
+ * fp32 accumulation, fp32 master weights) that scores the 512 (mode, slot)
+ * logits of a block (logit = mode * 64 + slot, the numbering of
+ * fh_bc7_exhaustive_labels; 285 are real candidates).  Inputs are the 16
+ * pixels' RGBA / 255.  Parameters are one float array of
+ * fh_ranker_num_params() = 214016 values:
+ *   W1[64][256] b1[256] W2[256][256] b2[256] W3[256][512] b3[512]
+ * (W[in][out]).  A ranker belongs to the device current at creation. */
+
+typedef struct fh_train_config {
+  uint32_t epochs;        /* passes over the training set */
+  uint32_t batch;         /* blocks per Adam step */
+  float lr;               /* Adam step size (beta1 0.9, beta2 0.999, eps 1e-8) */
+  uint64_t shuffle_seed;  /* seed of the per-epoch permutations */
+} fh_train_config;
+
+typedef struct fh_train_report {
+  double first_loss;      /* mean cross-entropy of the first batch */
+  double epoch_loss[16];  /* mean cross-entropy of the last batch of each epoch */
+  uint32_t steps;         /* Adam steps taken */
+  float ms;               /* device time of the whole training */
+} fh_train_report;
+
+fh_status fh_ranker_create(fh_ranker **ranker);
+fh_status fh_ranker_destroy(fh_ranker *ranker);
+size_t fh_ranker_num_params(void);
+/* He-uniform weights from a seed, zero biases, Adam state cleared. */
+fh_status fh_ranker_init(fh_ranker *ranker, uint64_t seed);
+fh_status fh_ranker_set_params(fh_ranker *ranker, const float *params, size_t count);
+fh_status fh_ranker_get_params(fh_ranker *ranker, float *params, size_t count);
+/* Device buffers: blocks n x 16 pixels, logits n x 512 floats. */
+fh_status fh_ranker_forward(fh_ranker *ranker, const uint32_t *blocks, uint32_t n, float *logits,
+                            void *hip_stream);
+/* Mean softmax cross-entropy over the 285 real logits of one batch and its
+ * gradient (host array of fh_ranker_num_params() floats); device inputs. */
+fh_status fh_ranker_gradients(fh_ranker *ranker, const uint32_t *blocks, const int32_t *labels,
+                              uint32_t n, float *grad, double *loss);
+/* Trains on device-resident blocks and labels: cfg->epochs passes over a
+ * fresh permutation each, one Adam step per cfg->batch blocks. */
+fh_status fh_ranker_train(fh_ranker *ranker, const uint32_t *blocks, const int32_t *labels,
+                          uint32_t n, const fh_train_config *cfg, fh_train_report *report);
 
 /* Kernel timing for benchmarks; no reference counterpart (the reference's
  * CompressImageData reports only a whole-job wall time, TexComp.cpp:517).

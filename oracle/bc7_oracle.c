@@ -1472,16 +1472,23 @@ static void compress_clusters(const shape_sel *sel, const uint32_t px[16],
 
 static void decode_block(const uint8_t blk[16], uint32_t px[16], int spec);
 
-/* One candidate at quality 0, packed: CompressionMode::Compress and Pack for
- * a fixed (mode, slot) as in ora_bc7_exhaustive_label.  Returns the internal
- * error. */
-double ora_bc7_encode_candidate(const uint32_t px[16], int mode, int slot, uint8_t out[16]) {
+/* One candidate, packed: CompressionMode::Compress and Pack for a fixed
+ * (mode, slot) as in ora_bc7_exhaustive_label, with `quality` annealing
+ * steps on the counter stream keyed by (seed, block_id, mode, shape entry,
+ * subset, rotation, index mode).  Returns the internal error. */
+static double encode_candidate(const uint32_t px[16], int mode, int slot, int entry,
+                               uint32_t quality, uint64_t seed, uint64_t block_id,
+                               uint8_t out[16]) {
   clus c;
   clus_init(&c, px);
   cmode m;
   memset(&m, 0, sizeof(m));
   m.a = &kModes[mode];
   m.metric = v4_splat(1.0f);
+  m.sa_steps = quality;
+  m.rng_kind = ORA_RNG_COUNTER;
+  m.seed = seed;
+  m.block_id = block_id;
   params pr;
   double e;
   if (m.a->has_rot) {
@@ -1491,6 +1498,7 @@ double ora_bc7_encode_candidate(const uint32_t px[16], int mode, int slot, uint8
     clus_set_partition(&c, 0);
     m.rot = mode == 4 ? slot >> 1 : slot;
     m.idx_mode = mode == 4 ? slot & 1 : 0;
+    m_rekey(&m, entry, 0);
     e = compress_cluster_alpha(&m, &c, &pr.p1[0], &pr.p2[0], pr.idx[0], pr.aidx);
     pr.rot = m.rot;
     pr.idx_mode = m.idx_mode;
@@ -1498,7 +1506,7 @@ double ora_bc7_encode_candidate(const uint32_t px[16], int mode, int slot, uint8
   } else {
     c.shape = (uint32_t)(m.a->subsets > 1 ? slot : 0);
     c.nparts = (uint32_t)m.a->subsets;
-    e = mode_compress(&m, &pr, (int)c.shape, 0, &c);
+    e = mode_compress(&m, &pr, (int)c.shape, entry, &c);
   }
   memset(out, 0, 16);
   cmode pm;
@@ -1507,6 +1515,44 @@ double ora_bc7_encode_candidate(const uint32_t px[16], int mode, int slot, uint8
   pm.metric = v4_splat(1.0f);
   mode_pack(&pm, &pr, out);
   return e;
+}
+
+double ora_bc7_encode_candidate(const uint32_t px[16], int mode, int slot, uint8_t out[16]) {
+  return encode_candidate(px, mode, slot, 0, 0, 0, 0, out);
+}
+
+/* The learned search's block (no reference counterpart): candidates given as
+ * mode | slot << 3 | shape entry << 9.  Among the quality-0 fits the block
+ * that decodes (reference decoder) to the smallest ora_block_premul_err
+ * wins, ties to the earlier candidate; with quality > 0 the same over the
+ * annealed fits, and the quality-0 winner is kept if it decodes strictly
+ * better. */
+void ora_bc7_encode_learned(const uint32_t px[16], const int32_t *cands, int n, uint32_t quality,
+                            uint64_t seed, uint64_t block_id, uint8_t out[16]) {
+  uint64_t best0 = UINT64_MAX, bestq = UINT64_MAX;
+  uint8_t blk0[16], blkq[16];
+  memset(blk0, 0, 16);
+  memset(blkq, 0, 16);
+  for (int pass = 0; pass < (quality > 0 ? 2 : 1); pass++) {
+    for (int i = 0; i < n; i++) {
+      const int mode = cands[i] & 7, slot = (cands[i] >> 3) & 63, entry = (cands[i] >> 9) & 1;
+      uint8_t b[16];
+      uint32_t dec[16];
+      encode_candidate(px, mode, slot, entry, pass ? quality : 0, seed, block_id, b);
+      decode_block(b, dec, 0);
+      const uint64_t e = ora_block_premul_err(px, dec);
+      if (pass == 0 && e < best0) {
+        best0 = e;
+        memcpy(blk0, b, 16);
+      }
+      if (pass == 1 && e < bestq) {
+        bestq = e;
+        memcpy(blkq, b, 16);
+      }
+    }
+  }
+  if (quality > 0 && !(best0 < bestq)) memcpy(out, blkq, 16);
+  else memcpy(out, blk0, 16);
 }
 
 /* Exhaustive quality-0 search over every (mode, slot) the candidate ranker

@@ -64,6 +64,12 @@ int ora_bc7_exhaustive_label(const uint32_t px[16], uint64_t *err);
  * internal error. */
 double ora_bc7_encode_candidate(const uint32_t px[16], int mode, int slot, uint8_t out[16]);
 
+/* The learned search's choice among candidates (mode | slot << 3 |
+ * entry << 9): smallest decoded premultiplied error, quality-0 fits kept when
+ * annealing decodes worse (no reference counterpart). */
+void ora_bc7_encode_learned(const uint32_t px[16], const int32_t *cands, int n, uint32_t quality,
+                            uint64_t seed, uint64_t block_id, uint8_t out[16]);
+
 /* Compress one 4x4 block (16 RGBA8 pixels, R in the low byte).
  * Follows CompressBC7Block (Compressor.cpp:1819-1860).
  * wm_index: the watermark table entry for a solid block (the reference's

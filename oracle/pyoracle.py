@@ -80,6 +80,8 @@ def lib():
         L.ora_bc7_exhaustive_label.restype = ctypes.c_int
         L.ora_bc7_encode_candidate.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
         L.ora_bc7_encode_candidate.restype = ctypes.c_double
+        L.ora_bc7_encode_learned.argtypes = [vp, vp, ctypes.c_int, ctypes.c_uint32,
+                                             ctypes.c_uint64, ctypes.c_uint64, vp]
         L.ora_counter_rng.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
         L.ora_counter_rng.restype = ctypes.c_uint32
         L.ora_fit_key.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32]
@@ -155,6 +157,15 @@ def bc7_encode_candidate(px16, mode, slot):
     out = np.zeros(16, dtype=np.uint8)
     e = lib().ora_bc7_encode_candidate(_p(px), mode, slot, _p(out))
     return out, e
+
+
+def bc7_encode_learned(px16, cands, quality=0, seed=0, block_id=0):
+    """The learned search's block for candidates [(mode, slot, entry), ...]."""
+    px = np.ascontiguousarray(px16, dtype=np.uint32)
+    c = np.array([m | (s << 3) | (e << 9) for m, s, e in cands], dtype=np.int32)
+    out = np.zeros(16, dtype=np.uint8)
+    lib().ora_bc7_encode_learned(_p(px), _p(c), len(c), quality, seed, block_id, _p(out))
+    return out
 
 
 def block_premul_err(src16, dec16):

@@ -3,6 +3,15 @@ import sys
 
 import pytest
 
+# torch ships its own HIP runtime under the same soname as ROCm's; whichever
+# loads first serves the whole process.  Load torch's before libfasthip so
+# the tests that hand torch buffers to the library share one runtime, as the
+# benchmark does.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    torch = None
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
