@@ -1022,6 +1022,11 @@ struct AState {
   uint32_t pad;
 };
 
+// Learned search: candidates whose quality-0 block decodes to more than
+// 1 + 1/kAnnealMarginDiv times the block's best quality-0 error are not
+// annealed (annealing lowers a fit's error by a few percent at most).
+constexpr unsigned long long kAnnealMarginDiv = 20;
+
 struct Work {
   uint32_t *sel;          // per block
   uint16_t *cdesc;        // per block x 16 candidate descriptors
@@ -1031,6 +1036,10 @@ struct Work {
   AState *astate;
   uint32_t qcap;          // entries per type queue
   uint32_t astate_cap;
+  // learned search: per candidate slot (rel * 16 + slot) the decoded
+  // quality-0 error, per block the smallest; null in the reference search
+  const unsigned long long *cerr;
+  const unsigned long long *q0best;
 };
 
 __device__ __forceinline__ void load_block(const BlockRange &r, uint32_t rel, uint32_t px[16]) {
@@ -1052,138 +1061,151 @@ __device__ __forceinline__ uint32_t cand_tag(int mode, int entry, int subset, in
          ((uint32_t)im << 9);
 }
 
-// Appends the wave's valid candidates to their type queues (one atomic per
-// type per wave).
-__device__ __forceinline__ void enqueue_candidates(Work &w, bool valid, int type, uint32_t id) {
-  const int lane = threadIdx.x & 63;
-  const uint64_t lt = (1ull << lane) - 1ull;
-  for (int t = 0; t < kNumTypes; t++) {
-    const uint64_t m = __ballot(valid && type == t);
-    if (!m) continue;
-    const int leader = (int)(__ffsll((unsigned long long)m) - 1);
-    uint32_t b0 = 0;
-    if (lane == leader) b0 = atomicAdd(&w.counters[t], (uint32_t)__popcll(m));
-    b0 = __shfl(b0, leader);
-    if (valid && type == t) w.queue[(size_t)t * w.qcap + b0 + __popcll(m & lt)] = id;
-  }
+// Appends the workgroup's valid candidates to their type queues: slots are
+// taken with LDS atomics, then one global atomic per type per workgroup.
+// Every thread of the workgroup must call it.
+__device__ __forceinline__ void wg_enqueue(Work &w, bool valid, int type, uint32_t id,
+                                           uint32_t *s_cnt, uint32_t *s_base) {
+  const int t = threadIdx.x;
+  if (t < kNumTypes) s_cnt[t] = 0;
+  __syncthreads();
+  uint32_t local = 0;
+  if (valid) local = atomicAdd(&s_cnt[type], 1u);
+  __syncthreads();
+  if (t < kNumTypes && s_cnt[t]) s_base[t] = atomicAdd(&w.counters[t], s_cnt[t]);
+  __syncthreads();
+  if (valid) w.queue[(size_t)type * w.qcap + s_base[type] + local] = id;
 }
 
 // ---------------------------------------------------------------------------
 // Kernel 1: classification and BoxSelection (Compressor.cpp:1670-1750,
-// 1819-1860); one wave per four blocks, lane i scores shape i.  Emits the
-// candidates into per-type queues (wave-aggregated appends).
+// 1819-1860), lane i scoring shape i; the candidates go to per-type queues.
 // ---------------------------------------------------------------------------
-// list, if given, holds the blocks to select (list_count of them, a device
-// word); otherwise all blocks of the range.
-__global__ void __launch_bounds__(64) select_kernel(BlockRange r, Bc7Params prm, Work w,
-                                                    const uint32_t *list,
-                                                    const uint32_t *list_count) {
-  __shared__ uint32_t s_px[4][16];
-  __shared__ uint32_t s_sel[4];
-  __shared__ uint16_t s_cd[4][kMaxCands];
-  const int lane = threadIdx.x;
-  const uint32_t base = blockIdx.x * 4;  // position in the list or the range
+// Classification (solid / transparent / searched) and BoxSelection of one
+// block by a whole wave (lane i scores two- and three-subset shape i); the
+// block's candidates in CompressClusters' order go to cd.  Returns the
+// packed class and count.
+__device__ uint32_t box_select_block(const uint32_t px[16], int lane, const Bc7Params &prm,
+                                     uint16_t cd[kMaxCands]) {
+  bool solid = true, transparent = true, opaque = true;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    solid = solid && px[i] == px[0];
+    transparent = transparent && (px[i] >> 24) == 0;
+    opaque = opaque && (px[i] >> 24) >= 250;
+  }
+  uint32_t sel;
+  int count = 0;
+  if (solid) {
+    sel = sel_pack(0, 0);
+  } else if (transparent) {
+    sel = sel_pack(1, 0);
+  } else {
+    double err2 = 0.0, err3 = 0.0;
+    for (int ns = 2; ns <= 3; ns++) {
+      if (ns == 3 && !opaque) break;
+      double err = 0.0;
+      for (int s = 0; s < ns; s++) {
+        const uint32_t m = subset_mask(lane, ns, s);
+        uint32_t mnc[4] = {255, 255, 255, 255}, mxc[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+          if (!((m >> i) & 1)) continue;
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            const uint32_t c = chan(px[i], k);
+            mnc[k] = c < mnc[k] ? c : mnc[k];
+            mxc[k] = c > mxc[k] ? c : mxc[k];
+          }
+        }
+        const uint32_t mn = mnc[0] | (mnc[1] << 8) | (mnc[2] << 16) | (mnc[3] << 24);
+        const uint32_t mx = mxc[0] | (mxc[1] << 8) | (mxc[2] << 16) | (mxc[3] << 24);
+        // EstimateTwo/ThreeClusterError (Compressor.cpp:1626-1655)
+        if (mn != mx) {
+          uint64_t dummy;
+          err += 0.0001 + (double)qerr(px, m, 0, mn, mx, ns == 2 ? 8 : 4, dummy);
+        } else {
+          err += 0.0;
+        }
+      }
+      if (ns == 2) err2 = err;
+      else err3 = err;
+    }
+    double v2 = err2, v3 = err3;
+    int i2 = lane, i3 = lane;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      const double o2 = __shfl_xor(v2, off);
+      const int oi2 = __shfl_xor(i2, off);
+      if (o2 < v2 || (o2 == v2 && oi2 < i2)) {
+        v2 = o2;
+        i2 = oi2;
+      }
+      const double o3 = __shfl_xor(v3, off);
+      const int oi3 = __shfl_xor(i3, off);
+      if (o3 < v3 || (o3 == v3 && oi3 < i3)) {
+        v3 = o3;
+        i3 = oi3;
+      }
+    }
+    uint32_t modes;
+    bool opaque_layout = true;
+    int nshapes = 1, s3 = 0;
+    if (v2 < 1e-9) {
+      modes = (1u << 1) | (1u << 3) | (1u << 7);
+    } else if (!opaque) {
+      modes = (1u << 4) | (1u << 5) | (1u << 6) | (1u << 7);
+      opaque_layout = false;
+    } else {
+      modes = 0xFFu & ~((1u << 4) | (1u << 5));
+      nshapes = 2;
+      s3 = i3;
+      if (v3 < 1e-9) modes = (1u << 0) | (1u << 2);
+    }
+    count = box_candidates(opaque_layout, nshapes, i2, s3, modes & prm.block_modes,
+                           prm.quality, cd);
+    sel = sel_pack(2, count);
+  }
+  return sel;
+}
+
+// Kernel 1: one workgroup of four waves selects 16 blocks, each wave four
+// in turn; the workgroup's candidates go to the type queues with one global
+// atomic per type.  list, if given, holds the blocks to select (list_count
+// of them, a device word); otherwise all blocks of the range.
+__global__ void __launch_bounds__(256) select_kernel(BlockRange r, Bc7Params prm, Work w,
+                                                     const uint32_t *list,
+                                                     const uint32_t *list_count) {
+  __shared__ uint32_t s_px[16][16];
+  __shared__ uint32_t s_sel[16];
+  __shared__ uint16_t s_cd[16][kMaxCands];
+  __shared__ uint32_t s_cnt[kNumTypes], s_base[kNumTypes];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t base = blockIdx.x * 16;  // position in the list or the range
   const uint32_t total = list ? *list_count : r.num_blocks;
   if (base >= total) return;  // whole workgroup
   auto rel_of = [&](uint32_t i) { return list ? list[i] : i; };
   {
-    const int row = lane >> 4, x = lane & 15;
-    const int blk = x >> 2;
+    const int blk = tid >> 4, p = tid & 15;
     const uint32_t i = base + blk;
     if (i < total) {
       const uint32_t rel = rel_of(i);
       const uint32_t b = r.first_block + rel;
       const uint32_t bx = b % r.blocks_wide, by = b / r.blocks_wide;
-      s_px[blk][row * 4 + (x & 3)] = r.img[(size_t)(by * 4 + row) * r.width + bx * 4 + (x & 3)];
+      s_px[blk][p] = r.img[(size_t)(by * 4 + (p >> 2)) * r.width + bx * 4 + (p & 3)];
     }
   }
   __syncthreads();
-  for (int blk = 0; blk < 4; blk++) {
-    if (base + blk >= total) break;  // uniform
+  for (int k = 0; k < 4; k++) {
+    const int blk = wave * 4 + k;
+    if (base + blk >= total) break;  // uniform in the wave
     const uint32_t rel = rel_of(base + blk);
     uint32_t px[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) px[i] = s_px[blk][i];
-    bool solid = true, transparent = true, opaque = true;
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-      solid = solid && px[i] == px[0];
-      transparent = transparent && (px[i] >> 24) == 0;
-      opaque = opaque && (px[i] >> 24) >= 250;
-    }
-    uint32_t sel;
     uint16_t cd[kMaxCands];
-    int count = 0;
-    if (solid) {
-      sel = sel_pack(0, 0);
-    } else if (transparent) {
-      sel = sel_pack(1, 0);
-    } else {
-      double err2 = 0.0, err3 = 0.0;
-      for (int ns = 2; ns <= 3; ns++) {
-        if (ns == 3 && !opaque) break;
-        double err = 0.0;
-        for (int s = 0; s < ns; s++) {
-          const uint32_t m = subset_mask(lane, ns, s);
-          uint32_t mnc[4] = {255, 255, 255, 255}, mxc[4] = {0, 0, 0, 0};
-#pragma unroll
-          for (int i = 0; i < 16; i++) {
-            if (!((m >> i) & 1)) continue;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-              const uint32_t c = chan(px[i], k);
-              mnc[k] = c < mnc[k] ? c : mnc[k];
-              mxc[k] = c > mxc[k] ? c : mxc[k];
-            }
-          }
-          const uint32_t mn = mnc[0] | (mnc[1] << 8) | (mnc[2] << 16) | (mnc[3] << 24);
-          const uint32_t mx = mxc[0] | (mxc[1] << 8) | (mxc[2] << 16) | (mxc[3] << 24);
-          // EstimateTwo/ThreeClusterError (Compressor.cpp:1626-1655)
-          if (mn != mx) {
-            uint64_t dummy;
-            err += 0.0001 + (double)qerr(px, m, 0, mn, mx, ns == 2 ? 8 : 4, dummy);
-          } else {
-            err += 0.0;
-          }
-        }
-        if (ns == 2) err2 = err;
-        else err3 = err;
-      }
-      double v2 = err2, v3 = err3;
-      int i2 = lane, i3 = lane;
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        const double o2 = __shfl_xor(v2, off);
-        const int oi2 = __shfl_xor(i2, off);
-        if (o2 < v2 || (o2 == v2 && oi2 < i2)) {
-          v2 = o2;
-          i2 = oi2;
-        }
-        const double o3 = __shfl_xor(v3, off);
-        const int oi3 = __shfl_xor(i3, off);
-        if (o3 < v3 || (o3 == v3 && oi3 < i3)) {
-          v3 = o3;
-          i3 = oi3;
-        }
-      }
-      uint32_t modes;
-      bool opaque_layout = true;
-      int nshapes = 1, s3 = 0;
-      if (v2 < 1e-9) {
-        modes = (1u << 1) | (1u << 3) | (1u << 7);
-      } else if (!opaque) {
-        modes = (1u << 4) | (1u << 5) | (1u << 6) | (1u << 7);
-        opaque_layout = false;
-      } else {
-        modes = 0xFFu & ~((1u << 4) | (1u << 5));
-        nshapes = 2;
-        s3 = i3;
-        if (v3 < 1e-9) modes = (1u << 0) | (1u << 2);
-      }
-      count = box_candidates(opaque_layout, nshapes, i2, s3, modes & prm.block_modes,
-                             prm.quality, cd);
-      sel = sel_pack(2, count);
-    }
+    const uint32_t sel = box_select_block(px, lane, prm, cd);
+    const int count = sel_count(sel);
     if (lane == 0) {
       s_sel[blk] = sel;
       w.sel[rel] = sel;
@@ -1193,16 +1215,17 @@ __global__ void __launch_bounds__(64) select_kernel(BlockRange r, Bc7Params prm,
     if (lane < kMaxCands) s_cd[blk][lane] = lane < count ? cd[lane] : (uint16_t)0;
   }
   __syncthreads();
-  // enqueue: lane = (block, candidate slot)
-  const int blk = lane >> 4, c = lane & 15;
-  const uint32_t rel = base + blk < total ? rel_of(base + blk) : 0;
+  // enqueue: thread = (block, candidate slot)
+  const int blk = tid >> 4, c = tid & 15;
   bool valid = false;
   int type = 0;
+  uint32_t rel = 0;
   if (base + blk < total && c < sel_count(s_sel[blk])) {
     valid = true;
+    rel = rel_of(base + blk);
     type = kTypeOfMode[s_cd[blk][c] & 7];
   }
-  enqueue_candidates(w, valid, type, (rel << 4) | (uint32_t)c);
+  wg_enqueue(w, valid, type, (rel << 4) | (uint32_t)c, s_cnt, s_base);
 }
 
 // ---------------------------------------------------------------------------
@@ -1233,83 +1256,100 @@ __device__ __forceinline__ uint32_t cdesc_of_logit(int j) {
   return cdesc_pack(mode, shape, rot, im, 0);
 }
 
-__global__ void __launch_bounds__(256) learned_select_kernel(BlockRange r, Bc7Params prm, Work w,
-                                                             uint32_t rel0, uint32_t nchunk,
-                                                             const float *logits,
-                                                             uint32_t *fallback,
-                                                             uint32_t *fallback_count) {
+__global__ void __launch_bounds__(1024) learned_select_kernel(BlockRange r, Bc7Params prm,
+                                                              Work w, uint32_t rel0,
+                                                              uint32_t nchunk,
+                                                              const float *logits,
+                                                              uint32_t *fallback,
+                                                              uint32_t *fallback_count) {
+  __shared__ uint32_t s_cnt[kNumTypes], s_base[kNumTypes];
+  __shared__ uint32_t s_fb, s_fbbase;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t k = blockIdx.x * 4 + wave;
-  if (k >= nchunk) return;  // whole wave
-  const uint32_t rel = rel0 + k;
-  const uint32_t b = r.first_block + rel;
-  const uint32_t bx = b % r.blocks_wide, by = b / r.blocks_wide;
-  const uint32_t mine =
-      r.img[(size_t)(by * 4 + ((lane & 15) >> 2)) * r.width + bx * 4 + (lane & 3)];
-  const uint32_t p0 = __shfl(mine, 0);
-  const bool solid = !__ballot((lane < 16) && mine != p0);
-  const bool transparent = !__ballot((lane < 16) && (mine >> 24) != 0);
-  if (solid || transparent) {
-    if (lane == 0) w.sel[rel] = sel_pack(solid ? 0 : 1, 0);
-    return;
-  }
-  float z[8];
-  bool ok[8];
-  float mx = -INFINITY;
-#pragma unroll
-  for (int i = 0; i < 8; i++) {
-    const int j = lane + 64 * i;
-    ok[i] = logit_real(j) && ((prm.block_modes >> (j >> 6)) & 1);
-    z[i] = logits[(size_t)k * 512 + j];
-    if (ok[i]) mx = fmaxf(mx, z[i]);
-  }
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-  float sum = 0.0f;
-#pragma unroll
-  for (int i = 0; i < 8; i++)
-    if (ok[i]) sum += expf(z[i] - mx);
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) sum += __shfl_xor(sum, off);
-  // top probability = exp(0) / sum
-  if (!(mx > -INFINITY) || 1.0f / sum < prm.threshold) {
-    if (lane == 0) {
-      const uint32_t at = atomicAdd(fallback_count, 1u);
-      fallback[at] = rel;
-    }
-    return;
-  }
-  const int kk = (int)(prm.topk < 1 ? 1 : (prm.topk > 8 ? 8 : prm.topk));
-  uint32_t my_desc = 0;
+  const uint32_t k = blockIdx.x * 16 + wave;  // one block per wave
+  if (threadIdx.x == 0) s_fb = 0;
+  __syncthreads();
+  bool fell_back = false;
+  uint32_t fb_slot = 0;
+  const bool real = k < nchunk;
+  const uint32_t rel = rel0 + (real ? k : 0);
+  bool searched = false;
   int count = 0;
-  for (int t = 0; t < kk; t++) {
-    float bv = -INFINITY;
-    int bj = 1 << 30;
-#pragma unroll
-    for (int i = 0; i < 8; i++)
-      if (ok[i] && z[i] > bv) {  // i ascending: the lower logit wins ties
-        bv = z[i];
-        bj = lane + 64 * i;
-      }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      const float ov = __shfl_xor(bv, off);
-      const int oj = __shfl_xor(bj, off);
-      if (ov > bv || (ov == bv && oj < bj)) {
-        bv = ov;
-        bj = oj;
-      }
+  uint32_t my_desc = 0;
+  if (real) {
+    const uint32_t b = r.first_block + rel;
+    const uint32_t bx = b % r.blocks_wide, by = b / r.blocks_wide;
+    const uint32_t mine =
+        r.img[(size_t)(by * 4 + ((lane & 15) >> 2)) * r.width + bx * 4 + (lane & 3)];
+    const uint32_t p0 = __shfl(mine, 0);
+    const bool solid = !__ballot((lane < 16) && mine != p0);
+    const bool transparent = !__ballot((lane < 16) && (mine >> 24) != 0);
+    if (solid || transparent) {
+      if (lane == 0) w.sel[rel] = sel_pack(solid ? 0 : 1, 0);
+    } else {
+      searched = true;
     }
-    if (!(bv > -INFINITY)) break;  // uniform
+  }
+  if (searched) {
+    float z[8];
+    bool ok[8];
+    float mx = -INFINITY;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int j = lane + 64 * i;
+      ok[i] = logit_real(j) && ((prm.block_modes >> (j >> 6)) & 1);
+      z[i] = logits[(size_t)k * 512 + j];
+      if (ok[i]) mx = fmaxf(mx, z[i]);
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+    float sum = 0.0f;
 #pragma unroll
     for (int i = 0; i < 8; i++)
-      if (lane + 64 * i == bj) ok[i] = false;
-    if (lane == t) my_desc = cdesc_of_logit(bj);
-    count++;
+      if (ok[i]) sum += expf(z[i] - mx);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) sum += __shfl_xor(sum, off);
+    // top probability = exp(0) / sum
+    if (!(mx > -INFINITY) || 1.0f / sum < prm.threshold) {
+      fell_back = true;
+      if (lane == 0) fb_slot = atomicAdd(&s_fb, 1u);
+    } else {
+      const int kk = (int)(prm.topk < 1 ? 1 : (prm.topk > 8 ? 8 : prm.topk));
+      for (int t = 0; t < kk; t++) {
+        float bv = -INFINITY;
+        int bj = 1 << 30;
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+          if (ok[i] && z[i] > bv) {  // i ascending: the lower logit wins ties
+            bv = z[i];
+            bj = lane + 64 * i;
+          }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+          const float ov = __shfl_xor(bv, off);
+          const int oj = __shfl_xor(bj, off);
+          if (ov > bv || (ov == bv && oj < bj)) {
+            bv = ov;
+            bj = oj;
+          }
+        }
+        if (!(bv > -INFINITY)) break;  // uniform
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+          if (lane + 64 * i == bj) ok[i] = false;
+        if (lane == t) my_desc = cdesc_of_logit(bj);
+        count++;
+      }
+      if (lane < count) w.cdesc[(size_t)rel * kMaxCands + lane] = (uint16_t)my_desc;
+      if (lane == 0) w.sel[rel] = sel_pack(2, count);
+    }
   }
-  if (lane < count) w.cdesc[(size_t)rel * kMaxCands + lane] = (uint16_t)my_desc;
-  if (lane == 0) w.sel[rel] = sel_pack(2, count);
-  enqueue_candidates(w, lane < count, kTypeOfMode[my_desc & 7], (rel << 4) | (uint32_t)lane);
+  // fallback list: one global atomic per workgroup
+  __syncthreads();
+  if (threadIdx.x == 0 && s_fb) s_fbbase = atomicAdd(fallback_count, s_fb);
+  __syncthreads();
+  if (fell_back && lane == 0) fallback[s_fbbase + fb_slot] = rel;
+  wg_enqueue(w, lane < count, kTypeOfMode[my_desc & 7], (rel << 4) | (uint32_t)lane, s_cnt,
+             s_base);
 }
 
 // ---------------------------------------------------------------------------
@@ -1707,7 +1747,18 @@ __global__ void __launch_bounds__(64) anneal_kernel(BlockRange r, Bc7Params prm,
       }
       const uint32_t avail = wend - wbase;
       const uint32_t rank = (uint32_t)__popcll(need & lt);
-      if (!L.has && rank < avail) lane_load(L, r, prm, w.astate[wbase + rank]);
+      if (!L.has && rank < avail) {
+        const AState &st = w.astate[wbase + rank];
+        // learned search: anneal only candidates whose quality-0 block
+        // decodes within kAnnealMargin of the block's best
+        bool take = true;
+        if (w.cerr) {
+          const uint32_t rel = st.id >> 4;
+          const unsigned long long best = w.q0best[rel];
+          take = w.cerr[st.id] <= best + best / kAnnealMarginDiv;
+        }
+        if (take) lane_load(L, r, prm, st);
+      }
       wbase += min(avail, (uint32_t)__popcll(need));
     }
     if (!__ballot(L.has)) break;
@@ -1931,6 +1982,7 @@ __global__ void __launch_bounds__(256) guard_kernel(BlockRange r, const uint4 *q
 __global__ void __launch_bounds__(256) pack_decoded_kernel(BlockRange r, Bc7Scratch scr, Work w,
                                                            uint4 *q0blk,
                                                            unsigned long long *q0err,
+                                                           unsigned long long *cerr,
                                                            int final_pass) {
   const uint32_t rel = blockIdx.x * blockDim.x + threadIdx.x;
   if (rel >= r.num_blocks) return;
@@ -1965,6 +2017,7 @@ __global__ void __launch_bounds__(256) pack_decoded_kernel(BlockRange r, Bc7Scra
       uint32_t dec[16];
       bc7dec::decode_block(blk, dec);
       const unsigned long long e = premul_err(px, dec);
+      if (cerr) cerr[(size_t)rel * kMaxCands + i] = e;
       if (e < best_err) {
         best_err = e;
         best = blk;
@@ -2153,7 +2206,8 @@ size_t bc7_work_bytes(uint32_t num_blocks, bool learned) {
     // q0 errors, fallback list, one chunk of network activations and logits
     const size_t c = n < kBc7MlpChunk ? n : kBc7MlpChunk;
     const size_t rows = (c + 127) & ~(size_t)127;
-    bytes += n * 8 + n * 4 + rows * (64 * 2 + 256 * 2 * 2 + 512 * 4) + 8 * 256;
+    bytes += n * 8 + n * bc7::kMaxCands * 8 + n * 4 + rows * (64 * 2 + 256 * 2 * 2 + 512 * 4) +
+             8 * 256;
   }
   return bytes;
 }
@@ -2193,9 +2247,13 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   uint16_t *mX = nullptr, *mh1 = nullptr, *mh2 = nullptr;
   float *mlogits = nullptr;
   const uint32_t mchunk = n < kBc7MlpChunk ? n : kBc7MlpChunk;
+  unsigned long long *cerr = nullptr;
+  w.cerr = nullptr;
+  w.q0best = nullptr;
   if (learned) {
     const size_t rows = ((size_t)mchunk + 127) & ~(size_t)127;
     q0err = reinterpret_cast<unsigned long long *>(carve((size_t)n * 8));
+    cerr = reinterpret_cast<unsigned long long *>(carve(nc * 8));
     fallback = reinterpret_cast<uint32_t *>(carve((size_t)n * 4));
     mX = reinterpret_cast<uint16_t *>(carve(rows * 64 * 2));
     mh1 = reinterpret_cast<uint16_t *>(carve(rows * 256 * 2));
@@ -2204,7 +2262,7 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   }
   uint32_t *fallback_count = w.counters + 20;
   const uint32_t nchunks = (n + 255) / 256;
-  const uint32_t ngroups = (n + 3) / 4;
+  const uint32_t ngroups = (n + 15) / 16;
   const uint32_t persistent = (uint32_t)g_num_cus[dev] * 16;
   const dim3 pack_grid((n + 255) / 256);
   const bool guard = p.quality > 0 && p.q0_guard;
@@ -2226,17 +2284,17 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
       timing_end(t, kTagBc7Rank, stream);
       if (e != hipSuccess) return e;
       t = timing_begin(stream);
-      hipLaunchKernelGGL(bc7::learned_select_kernel, dim3((cn + 3) / 4), dim3(256), 0, stream, r,
-                         p, w, c0, cn, (const float *)mlogits, fallback, fallback_count);
+      hipLaunchKernelGGL(bc7::learned_select_kernel, dim3((cn + 15) / 16), dim3(1024), 0, stream,
+                         r, p, w, c0, cn, (const float *)mlogits, fallback, fallback_count);
       timing_end(t, kTagBc7Select, stream);
     }
     t = timing_begin(stream);
-    hipLaunchKernelGGL(bc7::select_kernel, dim3(ngroups), dim3(64), 0, stream, r, p, w,
+    hipLaunchKernelGGL(bc7::select_kernel, dim3(ngroups), dim3(256), 0, stream, r, p, w,
                        (const uint32_t *)fallback, (const uint32_t *)fallback_count);
     timing_end(t, kTagBc7Select, stream);
   } else {
     t = timing_begin(stream);
-    hipLaunchKernelGGL(bc7::select_kernel, dim3(ngroups), dim3(64), 0, stream, r, p, w,
+    hipLaunchKernelGGL(bc7::select_kernel, dim3(ngroups), dim3(256), 0, stream, r, p, w,
                        (const uint32_t *)nullptr, (const uint32_t *)nullptr);
     timing_end(t, kTagBc7Select, stream);
   }
@@ -2247,15 +2305,19 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
     if (p.quality > 0) {
       t = timing_begin(stream);
       hipLaunchKernelGGL(bc7::pack_decoded_kernel, pack_grid, dim3(256), 0, stream, r, s, w, q0,
-                         q0err, 0);
+                         q0err, cerr, 0);
       timing_end(t, kTagBc7Pack, stream);
+      bc7::Work wa = w;
+      wa.cerr = cerr;
+      wa.q0best = q0err;
       t = timing_begin(stream);
-      hipLaunchKernelGGL(bc7::anneal_kernel, dim3(persistent), dim3(64), 0, stream, r, p, w);
+      hipLaunchKernelGGL(bc7::anneal_kernel, dim3(persistent), dim3(64), 0, stream, r, p, wa);
       timing_end(t, kTagBc7Anneal, stream);
     }
     t = timing_begin(stream);
     hipLaunchKernelGGL(bc7::pack_decoded_kernel, pack_grid, dim3(256), 0, stream, r, s, w, q0,
-                       p.quality > 0 ? q0err : (unsigned long long *)nullptr, 1);
+                       p.quality > 0 ? q0err : (unsigned long long *)nullptr,
+                       (unsigned long long *)nullptr, 1);
     timing_end(t, kTagBc7Pack, stream);
     return hipGetLastError();
   }

@@ -1522,25 +1522,38 @@ double ora_bc7_encode_candidate(const uint32_t px[16], int mode, int slot, uint8
 }
 
 /* The learned search's block (no reference counterpart): candidates given as
- * mode | slot << 3 | shape entry << 9.  Among the quality-0 fits the block
- * that decodes (reference decoder) to the smallest ora_block_premul_err
- * wins, ties to the earlier candidate; with quality > 0 the same over the
- * annealed fits, and the quality-0 winner is kept if it decodes strictly
- * better. */
+ * mode | slot << 3 | shape entry << 9 (at most 16).  Among the quality-0 fits
+ * the block that decodes (reference decoder) to the smallest
+ * ora_block_premul_err wins, ties to the earlier candidate; with quality > 0
+ * the candidates within 1/20 of that error are annealed, the same choice is
+ * made over annealed and remaining quality-0 fits, and the quality-0 winner
+ * is kept if it decodes strictly better. */
 void ora_bc7_encode_learned(const uint32_t px[16], const int32_t *cands, int n, uint32_t quality,
                             uint64_t seed, uint64_t block_id, uint8_t out[16]) {
   uint64_t best0 = UINT64_MAX, bestq = UINT64_MAX;
   uint8_t blk0[16], blkq[16];
+  uint64_t err0[16];
+  uint8_t cand0[16][16];
   memset(blk0, 0, 16);
   memset(blkq, 0, 16);
   for (int pass = 0; pass < (quality > 0 ? 2 : 1); pass++) {
-    for (int i = 0; i < n; i++) {
+    for (int i = 0; i < n && i < 16; i++) {
       const int mode = cands[i] & 7, slot = (cands[i] >> 3) & 63, entry = (cands[i] >> 9) & 1;
       uint8_t b[16];
       uint32_t dec[16];
-      encode_candidate(px, mode, slot, entry, pass ? quality : 0, seed, block_id, b);
+      /* only candidates whose quality-0 block decodes within 1/20 of the
+       * best are annealed; the others keep their quality-0 fit */
+      if (pass == 1 && !(err0[i] <= best0 + best0 / 20)) {
+        memcpy(b, cand0[i], 16);
+      } else {
+        encode_candidate(px, mode, slot, entry, pass ? quality : 0, seed, block_id, b);
+      }
       decode_block(b, dec, 0);
       const uint64_t e = ora_block_premul_err(px, dec);
+      if (pass == 0) {
+        err0[i] = e;
+        memcpy(cand0[i], b, 16);
+      }
       if (pass == 0 && e < best0) {
         best0 = e;
         memcpy(blk0, b, 16);
