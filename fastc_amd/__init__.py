@@ -63,6 +63,25 @@ class FhTrainReport(ctypes.Structure):
                 ("steps", ctypes.c_uint32), ("ms", ctypes.c_float)]
 
 
+class FhSurface(ctypes.Structure):
+    _fields_ = [("format", ctypes.c_uint32), ("in_", ctypes.c_void_p), ("out", ctypes.c_void_p),
+                ("width", ctypes.c_uint32), ("height", ctypes.c_uint32)]
+
+
+class FhBatchOptions(ctypes.Structure):
+    _fields_ = [("quality", ctypes.c_uint32), ("block_modes", ctypes.c_uint32),
+                ("error_metric", ctypes.c_uint32), ("seed", ctypes.c_uint64),
+                ("flags", ctypes.c_uint32), ("ranker", ctypes.c_void_p),
+                ("ranker_topk", ctypes.c_uint32), ("ranker_threshold", ctypes.c_float),
+                ("wm_counter", ctypes.c_void_p)]
+
+
+class FhBatchTune(ctypes.Structure):
+    _fields_ = [("wg_per_cu", ctypes.c_uint32), ("anneal_unroll", ctypes.c_uint32),
+                ("simple_wg", ctypes.c_uint32), ("best_ms", ctypes.c_float),
+                ("trials", ctypes.c_uint32)]
+
+
 class FastHipError(RuntimeError):
     def __init__(self, status, message):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
@@ -112,6 +131,15 @@ def lib():
                                           ctypes.POINTER(ctypes.c_double)]
         L.fh_ranker_train.argtypes = [vp, vp, vp, ctypes.c_uint32, ctypes.POINTER(FhTrainConfig),
                                       ctypes.POINTER(FhTrainReport)]
+        L.fh_batch_create.argtypes = [ctypes.POINTER(FhSurface), ctypes.c_uint32,
+                                      ctypes.POINTER(FhBatchOptions), ctypes.POINTER(vp)]
+        L.fh_batch_destroy.argtypes = [vp]
+        L.fh_batch_run.argtypes = [vp, ctypes.c_int, vp]
+        L.fh_batch_autotune.argtypes = [vp, vp, ctypes.POINTER(FhBatchTune)]
+        L.fh_batch_blocks.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        for name in ("fh_batch_create", "fh_batch_destroy", "fh_batch_run", "fh_batch_autotune",
+                     "fh_batch_blocks"):
+            getattr(L, name).restype = ctypes.c_int
         for name in ("fh_ranker_create", "fh_ranker_destroy", "fh_ranker_init",
                      "fh_ranker_set_params", "fh_ranker_get_params", "fh_ranker_forward",
                      "fh_ranker_gradients", "fh_ranker_train"):
@@ -125,7 +153,9 @@ EXPORTED_SYMBOLS = ["fh_compress", "fh_compress_device", "fh_decompress",
                     "fh_set_kernel_timing", "fh_kernel_time", "fh_bc7_exhaustive_labels",
                     "fh_ranker_create", "fh_ranker_destroy", "fh_ranker_num_params",
                     "fh_ranker_init", "fh_ranker_set_params", "fh_ranker_get_params",
-                    "fh_ranker_forward", "fh_ranker_gradients", "fh_ranker_train"]
+                    "fh_ranker_forward", "fh_ranker_gradients", "fh_ranker_train",
+                    "fh_batch_create", "fh_batch_destroy", "fh_batch_run", "fh_batch_autotune",
+                    "fh_batch_blocks"]
 
 
 def _check(status):
@@ -281,6 +311,52 @@ class Ranker:
         return {"first_loss": rep.first_loss,
                 "epoch_loss": list(rep.epoch_loss)[:min(epochs, 16)],
                 "steps": rep.steps, "ms": rep.ms}
+
+
+class Batch:
+    """A planned batch of surfaces (fh_batch_*).  `surfaces` is a list of
+    (format, in_ptr, out_ptr, width, height) with device pointers; BPTC
+    settings as in compress()."""
+
+    def __init__(self, surfaces, quality=0, seed=0, block_modes=0xFF, q0_guard=True,
+                 ranker=None, topk=4, threshold=0.6, wm_counter_ptr=0):
+        arr = (FhSurface * max(1, len(surfaces)))()
+        for i, (fmt, inp, out, w, h) in enumerate(surfaces):
+            arr[i] = FhSurface(fmt, inp, out, w, h)
+        opt = FhBatchOptions()
+        opt.quality = quality
+        opt.block_modes = block_modes
+        opt.seed = seed
+        opt.flags = 0 if q0_guard else FH_FLAG_NO_Q0_GUARD
+        if ranker is not None:
+            opt.flags |= FH_FLAG_LEARNED
+            opt.ranker = ranker.handle
+            opt.ranker_topk = topk
+            opt.ranker_threshold = threshold
+        opt.wm_counter = wm_counter_ptr or None
+        self._ranker = ranker
+        self._h = ctypes.c_void_p()
+        _check(lib().fh_batch_create(arr, len(surfaces), ctypes.byref(opt), ctypes.byref(self._h)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.fh_batch_destroy(h)
+            self._h = None
+
+    def run(self, use_graph=False, stream=0):
+        _check(lib().fh_batch_run(self._h, 1 if use_graph else 0, ctypes.c_void_p(stream)))
+
+    def autotune(self, stream=0):
+        t = FhBatchTune()
+        _check(lib().fh_batch_autotune(self._h, ctypes.c_void_p(stream), ctypes.byref(t)))
+        return {"wg_per_cu": t.wg_per_cu, "anneal_unroll": t.anneal_unroll,
+                "simple_wg": t.simple_wg, "best_ms": t.best_ms, "trials": t.trials}
+
+    def blocks(self, fmt):
+        n = ctypes.c_uint32(0)
+        _check(lib().fh_batch_blocks(self._h, fmt, ctypes.byref(n)))
+        return n.value
 
 
 def decompress(data, width, height, fmt=eCompressionFormat_BPTC):

@@ -52,6 +52,13 @@ __constant__ ModeAttr kModes[8] = {
     {0, 1, 2, 3, 5, 6, 1, 1, PB_NONE},      {0, 1, 2, 2, 7, 8, 1, 0, PB_NONE},
     {0, 1, 4, 0, 7, 7, 0, 0, PB_NOTSHARED}, {6, 2, 2, 0, 5, 5, 0, 0, PB_NOTSHARED}};
 
+// The same attributes for compile-time specialisation of per-mode kernels.
+constexpr ModeAttr kModeTab[8] = {
+    {4, 3, 3, 0, 4, 0, 0, 0, PB_NOTSHARED}, {6, 2, 3, 0, 6, 0, 0, 0, PB_SHARED},
+    {6, 3, 2, 0, 5, 0, 0, 0, PB_NONE},      {6, 2, 2, 0, 7, 0, 0, 0, PB_NOTSHARED},
+    {0, 1, 2, 3, 5, 6, 1, 1, PB_NONE},      {0, 1, 2, 2, 7, 8, 1, 0, PB_NONE},
+    {0, 1, 4, 0, 7, 7, 0, 0, PB_NOTSHARED}, {6, 2, 2, 0, 5, 5, 0, 0, PB_NOTSHARED}};
+
 // BC7 partition tables (same values as BPTCEncoder/include/FasTC/Shapes.h:24-53).
 __constant__ uint16_t kShape2[64] = {
     0xcccc, 0x8888, 0xeeee, 0xecc8, 0xc880, 0xfeec, 0xfec8, 0xec80, 0xc800, 0xffec, 0xfe80,
@@ -542,8 +549,8 @@ __device__ __forceinline__ void pick_neighbor(const Fit &f, Rng &rng, uint32_t p
 // Per-subset fit set-up: CompressCluster up to the annealing loop.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ void set_fit(Fit &f, int mode, int rot, int im) {
-  const ModeAttr a = kModes[mode];
+__device__ __forceinline__ void set_fit_attr(Fit &f, const ModeAttr a, int mode, int rot,
+                                             int im) {
   f.mode = mode;
   f.rot = rot;
   f.idx_mode = im;
@@ -553,6 +560,10 @@ __device__ __forceinline__ void set_fit(Fit &f, int mode, int rot, int im) {
   f.qmask = qmask_of(a.color_prec, a.alpha_prec);
   f.pbit_type = a.pbit_type;
   f.npbc = a.pbit_type == PB_SHARED ? 2 : (a.pbit_type == PB_NOTSHARED ? 4 : 1);
+}
+
+__device__ __forceinline__ void set_fit(Fit &f, int mode, int rot, int im) {
+  set_fit_attr(f, kModes[mode], mode, rot, im);
 }
 
 // Where a subset fit stands once CompressCluster reaches
@@ -755,7 +766,7 @@ __device__ void kmeans_setup(const Fit &f, uint32_t q, const uint32_t *sctab,
 }
 
 // CompressCluster, RGB variant (Compressor.cpp:921-1094) up to the annealing.
-__device__ void subset_setup(const Fit &f, uint32_t q, const uint32_t *sctab,
+__device__ __forceinline__ void subset_setup(const Fit &f, uint32_t q, const uint32_t *sctab,
                              const uint32_t px[16], uint32_t smask, int ptmode, int n,
                              const float avg[4], bool all_same, SubStart &out) {
   if (all_same) {
@@ -1359,11 +1370,16 @@ __global__ void __launch_bounds__(1024) learned_select_kernel(BlockRange r, Bc7P
 // error (CompressionMode::Compress, Compressor.cpp:1300-1371, without the
 // annealing), the record the pack kernel reads, and the annealing start
 // state.  Returns the mask of subsets that annealing continues (quality > 0).
-__device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t px[16], uint32_t quality,
-                                  const uint32_t *sctab, CandRec &rec, AState &st,
-                                  uint32_t &total) {
+// MODE >= 0 specialises the fit for that mode at compile time (k.mode must
+// equal it); -1 reads the mode from k.
+template <int MODE>
+__device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t px[16],
+                                                  uint32_t quality, const uint32_t *sctab,
+                                                  CandRec &rec, AState &st, uint32_t &total) {
   Fit f;
-  set_fit(f, k.mode, k.rot, k.im);
+  const ModeAttr attr = MODE >= 0 ? kModeTab[MODE < 0 ? 0 : MODE] : kModes[k.mode];
+  const int nsub = attr.subsets;
+  set_fit_attr(f, attr, MODE >= 0 ? MODE : k.mode, k.rot, k.im);
   rec.err = 0;
   rec.px1[0] = rec.px1[1] = rec.px1[2] = 0;
   rec.px2[0] = rec.px2[1] = rec.px2[2] = 0;
@@ -1373,7 +1389,7 @@ __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t 
   uint32_t active = 0, pbcs = 0;
   total = 0;
   uint64_t cidx = 0;
-  if (kModes[k.mode].has_rot) {
+  if (attr.has_rot) {
     uint64_t aidx;
     uint32_t a1q, a2q;
     const uint32_t aerr = alpha_fit(f, px, aidx, a1q, a2q);
@@ -1400,8 +1416,8 @@ __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t 
     total = s0.err + aerr;
     active = s0.anneal ? 1u : 0u;
   } else {
-    for (int s = 0; s < k.nsub; s++) {
-      const uint32_t smask = subset_mask(k.shape, k.nsub, s);
+    for (int s = 0; s < nsub; s++) {
+      const uint32_t smask = subset_mask(k.shape, nsub, s);
       const int n = __popc(smask);
       float avg[4] = {0.0f, 0.0f, 0.0f, 0.0f};
       bool same = true;
@@ -1452,10 +1468,15 @@ __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t 
   return active;
 }
 
+// One launch per mode (its type queue), specialised at compile time so that
+// each mode gets its own register allocation.
+template <int MODE>
 __global__ void __launch_bounds__(64) setup_kernel(BlockRange r, Bc7Params prm, Work w,
                                                    const uint32_t *sctab) {
   const int lane = threadIdx.x;
-  for (int t = 0; t < kNumTypes; t++) {
+  {
+    constexpr int kTypeOf[8] = {0, 2, 1, 3, 6, 7, 5, 4};
+    const int t = kTypeOf[MODE];
     const uint32_t count = *(volatile uint32_t *)&w.counters[t];
     for (;;) {
       uint32_t b0 = 0;
@@ -1474,7 +1495,7 @@ __global__ void __launch_bounds__(64) setup_kernel(BlockRange r, Bc7Params prm, 
       CandRec rec;
       AState st;
       uint32_t total = 0;
-      const uint32_t active = fit_candidate(k, px, prm.quality, sctab, rec, st, total);
+      const uint32_t active = fit_candidate<MODE>(k, px, prm.quality, sctab, rec, st, total);
       st.id = id;
       w.rec[(size_t)rel * kMaxCands + c] = rec;
       if (active) {
@@ -1545,12 +1566,154 @@ __device__ void lane_load(Lane &L, const BlockRange &r, const Bc7Params &prm, co
   L.extra = st.extra_err;
 }
 
+// Packed 16-bit lanes: channels 0 and 2 of a pixel are (pix & 0x00FF00FF),
+// channels 1 and 3 are ((pix >> 8) & 0x00FF00FF); every quantity of the
+// index search fits 16 bits (interpolation sums <= 255 * 64 + 32, channel
+// differences within +-255), so v_pk_* and v_dot2 give the exact integers.
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+__device__ __forceinline__ s16x2 as_s16x2(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
+__device__ __forceinline__ s16x2 as_s16x2(u16x2 v) { return __builtin_bit_cast(s16x2, v); }
+
+// Weight of the second endpoint at index j for nb = 4, 8, 16 entries:
+// (64 j + (nb - 1) / 2) / (nb - 1) by a multiply-shift, equal to
+// kInterpolationValues (Compressor.cpp:159-168); bc7_upload_tables checks
+// every entry on the host before the kernels run.
+struct WeightFn {
+  uint32_t h, m;
+};
+__host__ __device__ __forceinline__ WeightFn weight_fn(int nb) {
+  if (nb == 4) return {1u, 21846u};
+  if (nb == 8) return {3u, 9363u};
+  return {7u, 4370u};
+}
+__host__ __device__ __forceinline__ uint32_t weight_of(WeightFn fn, uint32_t j) {
+  return ((64u * j + fn.h) * fn.m) >> 16;
+}
+
+// Per-subset constants of one annealing step.
+struct SubConst {
+  uint32_t a02, a13;  // endpoint 1, channel pairs
+  uint32_t b02, b13;  // endpoint 2, channel pairs
+  uint32_t d02, d13;  // endpoint 2 - endpoint 1 (signed pairs)
+  float fl;           // squared endpoint distance
+  bool zero;          // endpoints equal
+};
+
+__device__ __forceinline__ SubConst sub_const(uint32_t e1, uint32_t e2) {
+  SubConst c;
+  c.a02 = e1 & 0x00FF00FFu;
+  c.a13 = (e1 >> 8) & 0x00FF00FFu;
+  c.b02 = e2 & 0x00FF00FFu;
+  c.b13 = (e2 >> 8) & 0x00FF00FFu;
+  c.d02 = __builtin_bit_cast(uint32_t, as_s16x2(c.b02) - as_s16x2(c.a02));
+  c.d13 = __builtin_bit_cast(uint32_t, as_s16x2(c.b13) - as_s16x2(c.a13));
+  const int l = __builtin_amdgcn_sdot2(as_s16x2(c.d02), as_s16x2(c.d02),
+                                       __builtin_amdgcn_sdot2(as_s16x2(c.d13),
+                                                              as_s16x2(c.d13), 0, false),
+                                       false);
+  c.fl = (float)l;
+  c.zero = l == 0;
+  return c;
+}
+
+// Squared error of the pixel pairs against index weight w1 (w0 = 64 - w1).
+__device__ __forceinline__ uint32_t interp_err(uint32_t p02, uint32_t p13, const SubConst &c,
+                                               uint32_t w1) {
+  const u16x2 W1 = {(unsigned short)w1, (unsigned short)w1};
+  const u16x2 W0 = {(unsigned short)(64u - w1), (unsigned short)(64u - w1)};
+  const u16x2 K = {32, 32}, S = {6, 6};
+  const u16x2 i02 = (as_u16x2(c.a02) * W0 + as_u16x2(c.b02) * W1 + K) >> S;
+  const u16x2 i13 = (as_u16x2(c.a13) * W0 + as_u16x2(c.b13) * W1 + K) >> S;
+  const s16x2 e02 = as_s16x2(p02) - as_s16x2(i02);
+  const s16x2 e13 = as_s16x2(p13) - as_s16x2(i13);
+  return (uint32_t)__builtin_amdgcn_sdot2(e02, e02, __builtin_amdgcn_sdot2(e13, e13, 0, false),
+                                          false);
+}
+
+// QuantizedError's per-pixel index search (RGBAEndpoints.cpp:240-300) for
+// pixel pix (projected as pt): returns the error, index in *best.
+__device__ __forceinline__ uint32_t pixel_err(uint32_t pix, uint32_t pt, const SubConst &c,
+                                              int nb, float kf, WeightFn wf, int *best) {
+  const uint32_t p02 = pix & 0x00FF00FFu, p13 = (pix >> 8) & 0x00FF00FFu;
+  if (c.zero) {
+    *best = 0;
+    return interp_err(p02, p13, c, 0);
+  }
+  const uint32_t t02 = pt & 0x00FF00FFu, t13 = (pt >> 8) & 0x00FF00FFu;
+  const int d = __builtin_amdgcn_sdot2(
+      as_s16x2(t02) - as_s16x2(c.a02), as_s16x2(c.d02),
+      __builtin_amdgcn_sdot2(as_s16x2(t13) - as_s16x2(c.a13), as_s16x2(c.d13), 0, false),
+      false);
+  const float pct = (float)d / c.fl;
+  const float t = pct * kf;
+  int j1 = (int)floorf(t);
+  int j2 = (int)ceilf(t);
+  j1 = j1 < 0 ? 0 : j1;
+  j1 = j1 > nb - 1 ? nb - 1 : j1;
+  j2 = j2 > nb - 1 ? nb - 1 : j2;
+  const uint32_t e1 = interp_err(p02, p13, c, weight_of(wf, (uint32_t)j1));
+  *best = j1;
+  if (j2 > j1) {
+    const uint32_t e2 = interp_err(p02, p13, c, weight_of(wf, (uint32_t)j2));
+    if (e2 < e1) {
+      *best = j2;
+      return e2;
+    }
+  }
+  return e1;
+}
+
+// The 16-pixel pass for NSUB subsets: errors per subset and the indices.
+__device__ __forceinline__ SubConst pick_sub(bool c, const SubConst &x, const SubConst &y) {
+  SubConst r;
+  r.a02 = c ? x.a02 : y.a02;
+  r.a13 = c ? x.a13 : y.a13;
+  r.b02 = c ? x.b02 : y.b02;
+  r.b13 = c ? x.b13 : y.b13;
+  r.d02 = c ? x.d02 : y.d02;
+  r.d13 = c ? x.d13 : y.d13;
+  r.fl = c ? x.fl : y.fl;
+  r.zero = c ? x.zero : y.zero;
+  return r;
+}
+
+template <int NSUB>
+__device__ __forceinline__ void score_pixels(const Lane &L, const SubConst &sc0,
+                                             const SubConst &sc1, const SubConst &sc2, int nb,
+                                             float kf, WeightFn wf, uint32_t nerr[3],
+                                             uint64_t &nidx) {
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int s = NSUB == 1 ? 0 : (int)((L.sub2 >> (2 * i)) & 3);
+    if (!((L.active >> s) & 1)) continue;
+    const SubConst c = NSUB == 1 ? sc0
+                                 : (NSUB == 2 ? pick_sub(s != 0, sc1, sc0)
+                                              : pick_sub(s == 0, sc0, pick_sub(s == 1, sc1, sc2)));
+    const uint32_t pix = L.px[i];
+    int best;
+    const uint32_t me = pixel_err(pix, point_of(pix, L.ptmode), c, nb, kf, wf, &best);
+    if (NSUB == 1) {
+      nerr[0] += me;
+    } else if (NSUB == 2) {
+      if (s) nerr[1] += me;
+      else nerr[0] += me;
+    } else {
+      if (s == 0) nerr[0] += me;
+      else if (s == 1) nerr[1] += me;
+      else nerr[2] += me;
+    }
+    nidx |= (uint64_t)best << (4 * i);
+  }
+}
+
 // One annealing step of every active subset.  Returns true when the
 // candidate is finished.
 __device__ bool lane_step(Lane &L, uint32_t q) {
   const Fit &f = L.f;
   const int nb = 1 << f.nbits_idx;
-  const int prec = f.nbits_idx;
   uint32_t np1[3], np2[3], e1[3], e2[3];
   int npbc[3];
 #pragma unroll
@@ -1567,106 +1730,36 @@ __device__ bool lane_step(Lane &L, uint32_t q) {
     e1[s] = to_pixel_int(np1[s], f.qmask, pbc_bit(f.pbit_type, npbc[s], 0));
     e2[s] = to_pixel_int(np2[s], f.qmask, pbc_bit(f.pbit_type, npbc[s], 1));
   }
-  // per-subset projection constants
-  int lsq[3];
-  float flsq[3];
-#pragma unroll
-  for (int s = 0; s < 3; s++) {
-    int l = 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int d = (int)chan(e2[s], k) - (int)chan(e1[s], k);
-      l += d * d;
-    }
-    lsq[s] = l;
-    flsq[s] = (float)l;
-  }
+  const SubConst sc0 = sub_const(e1[0], e2[0]);
+  const SubConst sc1 = sub_const(e1[1], e2[1]);
+  const SubConst sc2 = sub_const(e1[2], e2[2]);
   uint32_t nerr[3] = {0, 0, 0};
   uint64_t nidx = 0;
   const float kf = (float)(nb - 1);
-#pragma unroll
-  for (int i = 0; i < 16; i++) {
-    const int s = (L.sub2 >> (2 * i)) & 3;
-    if (!((L.active >> s) & 1)) continue;
-    const uint32_t qa = s == 0 ? e1[0] : (s == 1 ? e1[1] : e1[2]);
-    const uint32_t qb = s == 0 ? e2[0] : (s == 1 ? e2[1] : e2[2]);
-    const int l = s == 0 ? lsq[0] : (s == 1 ? lsq[1] : lsq[2]);
-    const float fl = s == 0 ? flsq[0] : (s == 1 ? flsq[1] : flsq[2]);
-    const uint32_t pix = L.px[i];
-    int a[4], b[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      a[k] = (int)chan(qa, k);
-      b[k] = (int)chan(qb, k);
-    }
-    uint32_t me;
-    int best = 0;
-    if (l == 0) {
-      me = 0;
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const int d = (int)chan(pix, k) - a[k];
-        me += (uint32_t)(d * d);
-      }
-    } else {
-      const uint32_t pt = point_of(pix, L.ptmode);
-      int d = 0;
-#pragma unroll
-      for (int k = 0; k < 4; k++) d += ((int)chan(pt, k) - a[k]) * (b[k] - a[k]);
-      const float pct = (float)d / fl;
-      const float t = pct * kf;
-      int j1 = (int)floorf(t);
-      int j2 = (int)ceilf(t);
-      j1 = j1 < 0 ? 0 : j1;
-      j1 = j1 > nb - 1 ? nb - 1 : j1;
-      j2 = j2 > nb - 1 ? nb - 1 : j2;
-      uint32_t e1v = 0;
-      {
-        const int w1 = kW[prec - 1][j1], w0 = 64 - w1;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          const int ip = (a[k] * w0 + b[k] * w1 + 32) >> 6;
-          const int dd = (int)chan(pix, k) - ip;
-          e1v += (uint32_t)(dd * dd);
-        }
-      }
-      best = j1;
-      me = e1v;
-      if (j2 > j1) {
-        const int w1 = kW[prec - 1][j2], w0 = 64 - w1;
-        uint32_t e2v = 0;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          const int ip = (a[k] * w0 + b[k] * w1 + 32) >> 6;
-          const int dd = (int)chan(pix, k) - ip;
-          e2v += (uint32_t)(dd * dd);
-        }
-        if (e2v < e1v) {
-          best = j2;
-          me = e2v;
-        }
-      }
-    }
-    if (s == 0) nerr[0] += me;
-    else if (s == 1) nerr[1] += me;
-    else nerr[2] += me;
-    nidx |= (uint64_t)best << (4 * i);
-  }
+  const WeightFn wf = weight_fn(nb);
+  if (L.nsub == 1) score_pixels<1>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx);
+  else if (L.nsub == 2) score_pixels<2>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx);
+  else score_pixels<3>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx);
 #pragma unroll
   for (int s = 0; s < 3; s++) {
     if (!((L.active >> s) & 1)) continue;
     const uint32_t err = nerr[s];
-    const float temp = (float)L.energy[s] / (float)((int)q - 1);
     bool accept = err < L.cur[s];
     if (!accept) {
-      // AcceptNewEndpointError (Compressor.cpp:524-536)
+      // AcceptNewEndpointError (Compressor.cpp:524-536): accept when
+      // frand() < exp(0.1 (cur - err) / temp).  Here err >= cur, so the
+      // probability is at most 1 and a draw in [1, 3) never accepts.
       Rng rng;
       rng.key = L.key[s];
       rng.ctr = L.ctr[s];
-      const double pr = exp(((double)0.1f * ((double)L.cur[s] - (double)err)) / (double)temp);
-      const double rr = (double)rng.frand();
+      const float rr = rng.frand();
       L.ctr[s] = rng.ctr;
-      accept = rr < pr;
+      if (rr < 1.0f) {
+        const float temp = (float)L.energy[s] / (float)((int)q - 1);
+        const double pr =
+            exp(((double)0.1f * ((double)L.cur[s] - (double)err)) / (double)temp);
+        accept = (double)rr < pr;
+      }
     }
     if (accept) {
       L.cur[s] = err;
@@ -1721,6 +1814,8 @@ __device__ void lane_finish(Lane &L, Work &w) {
   rc->err = total;
 }
 
+// UNROLL annealing steps run between refills of finished lanes.
+template <int UNROLL>
 __global__ void __launch_bounds__(64) anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
   const int lane = threadIdx.x;
   const uint64_t lt = (1ull << lane) - 1ull;
@@ -1762,10 +1857,13 @@ __global__ void __launch_bounds__(64) anneal_kernel(BlockRange r, Bc7Params prm,
       wbase += min(avail, (uint32_t)__popcll(need));
     }
     if (!__ballot(L.has)) break;
-    if (L.has) {
-      if (lane_step(L, prm.quality)) {
-        lane_finish(L, w);
-        L.has = false;
+#pragma unroll
+    for (int u = 0; u < UNROLL; u++) {
+      if (L.has) {
+        if (lane_step(L, prm.quality)) {
+          lane_finish(L, w);
+          L.has = false;
+        }
       }
     }
   }
@@ -2076,7 +2174,7 @@ __global__ void __launch_bounds__(64) label_kernel(BlockRange r, int mode, uint3
     CandRec rec;
     AState st;
     uint32_t total = 0;
-    fit_candidate(k, px, 0, sctab, rec, st, total);
+    fit_candidate<-1>(k, px, 0, sctab, rec, st, total);
     BitWriter bw;
     pack_mode(k.mode, k.shape, rec, bw);
     const uint4 blk = make_uint4((uint32_t)bw.lo, (uint32_t)(bw.lo >> 32), (uint32_t)bw.hi,
@@ -2099,6 +2197,37 @@ __global__ void __launch_bounds__(256) label_finish_kernel(const unsigned long l
   const int rank = (int)(key & 0xFFFFu);
   labels[i] = kOrder[rank >> 6] * 64 + (rank & 63);
   if (errors) errors[i] = key >> 16;
+}
+
+// Batches: global blocks [g0, g0 + n) of the BC7 group to a block array
+// (16 pixels each), and the encoded blocks back to their surfaces.
+__global__ void __launch_bounds__(256) gather_kernel(const SurfDesc *surfs,
+                                                     const uint32_t *block_surf, uint32_t g0,
+                                                     uint32_t n, uint32_t *blocks) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;  // block * 4 + row
+  const uint32_t k = t >> 2, row = t & 3;
+  if (k >= n) return;
+  const SurfDesc sd = surfs[block_surf[g0 + k]];
+  const uint32_t b = g0 + k - sd.first;
+  const uint32_t bx = b % sd.blocks_wide, by = b / sd.blocks_wide;
+  const uint4 v = *reinterpret_cast<const uint4 *>(sd.img + (size_t)(by * 4 + row) * sd.width +
+                                                   bx * 4);
+  reinterpret_cast<uint4 *>(blocks)[(size_t)k * 4 + row] = v;
+}
+
+__global__ void __launch_bounds__(256) scatter_kernel(const SurfDesc *surfs,
+                                                      const uint32_t *block_surf, uint32_t g0,
+                                                      uint32_t n, const uint8_t *enc) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const SurfDesc sd = surfs[block_surf[g0 + k]];
+  reinterpret_cast<uint4 *>(sd.out)[g0 + k - sd.first] =
+      reinterpret_cast<const uint4 *>(enc)[k];
+}
+
+__global__ void __launch_bounds__(256) fill_kernel(uint32_t *p, uint32_t n, uint32_t v) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 __global__ void reset_kernel(uint32_t *counters, int n) {
@@ -2191,8 +2320,61 @@ hipError_t bc7_upload_tables(int device) {
   e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
   if (e != hipSuccess) return e;
   g_num_cus[device] = cus > 0 ? cus : 1;
+  // the annealing step computes index weights by multiply-shift; it must
+  // reproduce the interpolation table entry for entry
+  static const uint8_t kWeights[3][16] = {
+      {0, 21, 43, 64},
+      {0, 9, 18, 27, 37, 46, 55, 64},
+      {0, 4, 9, 13, 17, 21, 26, 30, 34, 38, 43, 47, 51, 55, 60, 64}};
+  for (int t = 0; t < 3; t++) {
+    const int nb = 4 << t;
+    for (int j = 0; j < nb; j++)
+      if (bc7::weight_of(bc7::weight_fn(nb), (uint32_t)j) != kWeights[t][j])
+        return hipErrorInvalidValue;
+  }
   g_tables_uploaded[device] = 1;
   return hipSuccess;
+}
+
+void launch_anneal(uint32_t grid, const BlockRange &r, const Bc7Params &p, const bc7::Work &w,
+                   hipStream_t stream) {
+  if (p.anneal_unroll >= 4)
+    hipLaunchKernelGGL(bc7::anneal_kernel<4>, dim3(grid), dim3(64), 0, stream, r, p, w);
+  else if (p.anneal_unroll == 2)
+    hipLaunchKernelGGL(bc7::anneal_kernel<2>, dim3(grid), dim3(64), 0, stream, r, p, w);
+  else
+    hipLaunchKernelGGL(bc7::anneal_kernel<1>, dim3(grid), dim3(64), 0, stream, r, p, w);
+}
+
+hipError_t fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(bc7::fill_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t bc7_prepare() {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  e = bc7_upload_tables(dev);
+  if (e != hipSuccess) return e;
+  return bc7_single_color_table(dev) ? hipSuccess : hipErrorNotInitialized;
+}
+
+hipError_t bc7_gather(const SurfDesc *surfs, const uint32_t *block_surf, uint32_t g0, uint32_t n,
+                      uint32_t *blocks, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(bc7::gather_kernel, dim3((n * 4 + 255) / 256), dim3(256), 0, stream, surfs,
+                     block_surf, g0, n, blocks);
+  return hipGetLastError();
+}
+
+hipError_t bc7_scatter(const SurfDesc *surfs, const uint32_t *block_surf, uint32_t g0, uint32_t n,
+                       const uint8_t *enc, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(bc7::scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, surfs,
+                     block_surf, g0, n, enc);
+  return hipGetLastError();
 }
 
 size_t bc7_work_bytes(uint32_t num_blocks, bool learned) {
@@ -2263,7 +2445,7 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   uint32_t *fallback_count = w.counters + 20;
   const uint32_t nchunks = (n + 255) / 256;
   const uint32_t ngroups = (n + 15) / 16;
-  const uint32_t persistent = (uint32_t)g_num_cus[dev] * 16;
+  const uint32_t persistent = (uint32_t)g_num_cus[dev] * (p.wg_per_cu ? p.wg_per_cu : 16u);
   const dim3 pack_grid((n + 255) / 256);
   const bool guard = p.quality > 0 && p.q0_guard;
   int t = timing_begin(stream);
@@ -2299,7 +2481,15 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
     timing_end(t, kTagBc7Select, stream);
   }
   t = timing_begin(stream);
-  hipLaunchKernelGGL(bc7::setup_kernel, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+  // queue (type) order: modes 0, 2, 1, 3, 7, 6, 4, 5
+  hipLaunchKernelGGL(bc7::setup_kernel<0>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+  hipLaunchKernelGGL(bc7::setup_kernel<2>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+  hipLaunchKernelGGL(bc7::setup_kernel<1>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+  hipLaunchKernelGGL(bc7::setup_kernel<3>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+  hipLaunchKernelGGL(bc7::setup_kernel<7>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+  hipLaunchKernelGGL(bc7::setup_kernel<6>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+  hipLaunchKernelGGL(bc7::setup_kernel<4>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+  hipLaunchKernelGGL(bc7::setup_kernel<5>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
   timing_end(t, kTagBc7Setup, stream);
   if (learned) {
     if (p.quality > 0) {
@@ -2311,7 +2501,7 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
       wa.cerr = cerr;
       wa.q0best = q0err;
       t = timing_begin(stream);
-      hipLaunchKernelGGL(bc7::anneal_kernel, dim3(persistent), dim3(64), 0, stream, r, p, wa);
+      launch_anneal(persistent, r, p, wa, stream);
       timing_end(t, kTagBc7Anneal, stream);
     }
     t = timing_begin(stream);
@@ -2328,7 +2518,7 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   }
   if (p.quality > 0) {
     t = timing_begin(stream);
-    hipLaunchKernelGGL(bc7::anneal_kernel, dim3(persistent), dim3(64), 0, stream, r, p, w);
+    launch_anneal(persistent, r, p, w, stream);
     timing_end(t, kTagBc7Anneal, stream);
   }
   t = timing_begin(stream);

@@ -209,6 +209,8 @@ fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8
       p.ranker = learned ? &j->ranker->net : nullptr;
       p.topk = j->ranker_topk ? j->ranker_topk : 4;
       p.threshold = j->ranker_threshold;
+      p.wg_per_cu = 0;
+      p.anneal_unroll = 0;
       // jobs larger than one call's work area run as consecutive calls on the
       // stream; the watermark counter carries across them in order
       for (uint32_t done = 0; done < count; done += chunk) {
@@ -258,6 +260,13 @@ int current_device(fh_status *s) {
 }  // namespace
 
 void set_error(const char *msg) { t_last_error = msg; }
+
+uint32_t *device_wm_counter(int dev) {
+  DeviceState *st = device_state(dev);
+  std::lock_guard<std::mutex> g(st->mu);
+  if (ensure_init(st) != hipSuccess) return nullptr;
+  return st->counter;
+}
 
 const uint32_t *bc7_single_color_table(int device) {
   DeviceState *st = device_state(device);

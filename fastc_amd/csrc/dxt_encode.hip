@@ -387,11 +387,49 @@ __global__ void __launch_bounds__(256) encode_kernel(BlockRange r) {
   }
 }
 
+// Batch form: persistent workgroups take chunks of the group's blocks from
+// a device counter; each block finds its surface through block_surf.
+template <int DXT5>
+__global__ void batch_kernel(const SurfDesc *surfs, const uint32_t *block_surf, uint32_t total,
+                             uint32_t *counter) {
+  __shared__ uint32_t s_chunk;
+  for (;;) {
+    if (threadIdx.x == 0) s_chunk = atomicAdd(counter, blockDim.x);
+    __syncthreads();
+    const uint32_t chunk = s_chunk;
+    __syncthreads();
+    if (chunk >= total) break;
+    const uint32_t g = chunk + threadIdx.x;
+    if (g >= total) continue;
+    const SurfDesc sd = surfs[block_surf[g]];
+    const uint32_t b = g - sd.first;
+    const uint32_t bx = b % sd.blocks_wide, by = b / sd.blocks_wide;
+    const uint32_t *p = sd.img + (size_t)(by * 4) * sd.width + bx * 4;
+    uint32_t blk[16];
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+      const uint4 row = *reinterpret_cast<const uint4 *>(p + (size_t)y * sd.width);
+      blk[y * 4 + 0] = row.x;
+      blk[y * 4 + 1] = row.y;
+      blk[y * 4 + 2] = row.z;
+      blk[y * 4 + 3] = row.w;
+    }
+    const uint2 c = compress_color(blk);
+    if (DXT5) {
+      const uint2 a = compress_alpha(blk);
+      reinterpret_cast<uint4 *>(sd.out)[b] = make_uint4(a.x, a.y, c.x, c.y);
+    } else {
+      reinterpret_cast<uint2 *>(sd.out)[b] = c;
+    }
+  }
+}
+
 }  // namespace dxt
 
 namespace {
 
 int g_dxt_tables[64];
+int g_dxt_cus[64];
 
 // stb__InitDXT / stb__PrepareOptTable (stb_dxt.h:121-147, 603-621).
 void build_dxt_tables(dxt::Tables &t) {
@@ -429,8 +467,7 @@ void build_dxt_tables(dxt::Tables &t) {
 
 }  // namespace
 
-hipError_t dxt_encode(const BlockRange &r, int dxt5, hipStream_t stream) {
-  if (r.num_blocks == 0) return hipSuccess;
+hipError_t dxt_prepare() {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
@@ -440,8 +477,41 @@ hipError_t dxt_encode(const BlockRange &r, int dxt5, hipStream_t stream) {
     build_dxt_tables(t);
     e = hipMemcpyToSymbol(HIP_SYMBOL(dxt::kT), &t, sizeof(t));
     if (e != hipSuccess) return e;
+    int cus = 0;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    g_dxt_cus[dev] = cus > 0 ? cus : 1;
     g_dxt_tables[dev] = 1;
   }
+  return hipSuccess;
+}
+
+hipError_t dxt_encode_batch(const SurfDesc *surfs, const uint32_t *block_surf, uint32_t total,
+                            int dxt5, uint32_t *counter, const LaunchTune &tune,
+                            hipStream_t stream) {
+  if (total == 0) return hipSuccess;
+  hipError_t e = dxt_prepare();
+  if (e != hipSuccess) return e;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const uint32_t wg = tune.simple_wg;
+  uint32_t grid = (uint32_t)g_dxt_cus[dev] * (2048u / wg);
+  if (grid > (total + wg - 1) / wg) grid = (total + wg - 1) / wg;
+  const int t = timing_begin(stream);
+  if (dxt5)
+    hipLaunchKernelGGL(dxt::batch_kernel<1>, dim3(grid), dim3(wg), 0, stream, surfs, block_surf,
+                       total, counter);
+  else
+    hipLaunchKernelGGL(dxt::batch_kernel<0>, dim3(grid), dim3(wg), 0, stream, surfs, block_surf,
+                       total, counter);
+  timing_end(t, kTagDxt, stream);
+  return hipGetLastError();
+}
+
+hipError_t dxt_encode(const BlockRange &r, int dxt5, hipStream_t stream) {
+  if (r.num_blocks == 0) return hipSuccess;
+  hipError_t e = dxt_prepare();
+  if (e != hipSuccess) return e;
   const uint32_t grid = (r.num_blocks + 255) / 256;
   const int t = timing_begin(stream);
   if (dxt5)

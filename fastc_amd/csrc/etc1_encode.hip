@@ -414,6 +414,36 @@ __global__ void __launch_bounds__(256) encode_kernel(BlockRange r, const Tables 
   reinterpret_cast<uint2 *>(r.out)[b] = pack_block(t, px);
 }
 
+// Batch form: persistent workgroups take chunks of the group's blocks from
+// a device counter; each block finds its surface through block_surf.
+__global__ void batch_kernel(const SurfDesc *surfs, const uint32_t *block_surf, uint32_t total,
+                             uint32_t *counter, const Tables *t) {
+  __shared__ uint32_t s_chunk;
+  for (;;) {
+    if (threadIdx.x == 0) s_chunk = atomicAdd(counter, blockDim.x);
+    __syncthreads();
+    const uint32_t chunk = s_chunk;
+    __syncthreads();
+    if (chunk >= total) break;
+    const uint32_t g = chunk + threadIdx.x;
+    if (g >= total) continue;
+    const SurfDesc sd = surfs[block_surf[g]];
+    const uint32_t b = g - sd.first;
+    const uint32_t bx = b % sd.blocks_wide, by = b / sd.blocks_wide;
+    const uint32_t *p = sd.img + (size_t)(by * 4) * sd.width + bx * 4;
+    uint32_t px[16];
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+      const uint4 row = *reinterpret_cast<const uint4 *>(p + (size_t)y * sd.width);
+      px[y * 4 + 0] = row.x;
+      px[y * 4 + 1] = row.y;
+      px[y * 4 + 2] = row.z;
+      px[y * 4 + 3] = row.w;
+    }
+    reinterpret_cast<uint2 *>(sd.out)[b] = pack_block(t, px);
+  }
+}
+
 }  // namespace etc
 
 namespace {
@@ -494,11 +524,11 @@ void build_tables(etc::Tables &t) {
 }
 
 etc::Tables *g_tables[64];
+int g_etc_cus[64];
 
 }  // namespace
 
-hipError_t etc1_encode(const BlockRange &r, hipStream_t stream) {
-  if (r.num_blocks == 0) return hipSuccess;
+hipError_t etc1_prepare() {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
@@ -511,8 +541,38 @@ hipError_t etc1_encode(const BlockRange &r, hipStream_t stream) {
     if (e == hipSuccess) e = hipMemcpy(d, h, sizeof(etc::Tables), hipMemcpyHostToDevice);
     delete h;
     if (e != hipSuccess) return e;
+    int cus = 0;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    g_etc_cus[dev] = cus > 0 ? cus : 1;
     g_tables[dev] = d;
   }
+  return hipSuccess;
+}
+
+hipError_t etc1_encode_batch(const SurfDesc *surfs, const uint32_t *block_surf, uint32_t total,
+                             uint32_t *counter, const LaunchTune &tune, hipStream_t stream) {
+  if (total == 0) return hipSuccess;
+  hipError_t e = etc1_prepare();
+  if (e != hipSuccess) return e;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const uint32_t wg = tune.simple_wg;
+  uint32_t grid = (uint32_t)g_etc_cus[dev] * (2048u / wg);
+  if (grid > (total + wg - 1) / wg) grid = (total + wg - 1) / wg;
+  const int t = timing_begin(stream);
+  hipLaunchKernelGGL(etc::batch_kernel, dim3(grid), dim3(wg), 0, stream, surfs, block_surf,
+                     total, counter, (const etc::Tables *)g_tables[dev]);
+  timing_end(t, kTagEtc1, stream);
+  return hipGetLastError();
+}
+
+hipError_t etc1_encode(const BlockRange &r, hipStream_t stream) {
+  if (r.num_blocks == 0) return hipSuccess;
+  int dev = 0;
+  hipError_t e = etc1_prepare();
+  if (e != hipSuccess) return e;
+  (void)hipGetDevice(&dev);
   const uint32_t grid = (r.num_blocks + 255) / 256;
   const int t = timing_begin(stream);
   hipLaunchKernelGGL(etc::encode_kernel, dim3(grid), dim3(256), 0, stream, r, g_tables[dev]);

@@ -32,6 +32,8 @@ struct Bc7Params {
   const Ranker *ranker;
   uint32_t topk;         // candidates searched per confident block (1..8)
   float threshold;       // below this top softmax probability: BoxSelection
+  uint32_t wg_per_cu;    // persistent workgroups per CU (0 = 16)
+  uint32_t anneal_unroll;  // annealing steps between refills (0 = 1)
 };
 
 // Device scratch needed by the BC7 watermark prefix scan.
@@ -68,6 +70,52 @@ hipError_t bc7_exhaustive_labels(const BlockRange &r, unsigned long long *best, 
 
 hipError_t dxt_encode(const BlockRange &r, int dxt5, hipStream_t stream);
 hipError_t etc1_encode(const BlockRange &r, hipStream_t stream);
+
+// ---- Batches of surfaces (texture levels) --------------------------------
+// One surface of a batch, on the device.  Blocks of a format group are
+// numbered globally in submission order; surface s owns blocks
+// [first, first + nblocks) and block_surf maps a global block to its surface.
+struct SurfDesc {
+  const uint32_t *img;   // width * height RGBA8
+  uint8_t *out;          // the surface's compressed blocks (raster order)
+  uint32_t width;        // pixels, a multiple of 4
+  uint32_t blocks_wide;
+  uint32_t first;        // first global block of the surface
+  uint32_t nblocks;
+};
+
+// Launch shapes the batch autotuner chooses.
+struct LaunchTune {
+  uint32_t wg_per_cu = 16;     // persistent workgroups per CU (BC7 set-up / annealing)
+  uint32_t anneal_unroll = 1;  // annealing steps between refills (1, 2 or 4)
+  uint32_t simple_wg = 256;    // workgroup size of the DXT / ETC1 batch kernels
+};
+
+// Sets n words to v with a kernel (safe inside captured graphs).
+hipError_t fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t stream);
+
+// The library's process-wide BC7 watermark counter on device dev (capi.cpp),
+// initialising the device state if needed; null on failure.
+uint32_t *device_wm_counter(int dev);
+
+// Table uploads (host side, not capturable): run before capturing a batch.
+hipError_t dxt_prepare();
+hipError_t etc1_prepare();
+hipError_t bc7_prepare();
+
+// Persistent DXT1/DXT5/ETC1 kernels over all blocks of a format group,
+// fed by the device counter (one fetch-and-add per workgroup and chunk).
+hipError_t dxt_encode_batch(const SurfDesc *surfs, const uint32_t *block_surf, uint32_t total,
+                            int dxt5, uint32_t *counter, const LaunchTune &tune,
+                            hipStream_t stream);
+hipError_t etc1_encode_batch(const SurfDesc *surfs, const uint32_t *block_surf, uint32_t total,
+                             uint32_t *counter, const LaunchTune &tune, hipStream_t stream);
+// BC7 group: gather blocks [g0, g0 + n) into a block array (n x 16 pixels)
+// and scatter encoded blocks back to their surfaces.
+hipError_t bc7_gather(const SurfDesc *surfs, const uint32_t *block_surf, uint32_t g0,
+                      uint32_t n, uint32_t *blocks, hipStream_t stream);
+hipError_t bc7_scatter(const SurfDesc *surfs, const uint32_t *block_surf, uint32_t g0,
+                       uint32_t n, const uint8_t *enc, hipStream_t stream);
 
 hipError_t bc7_decode(const uint8_t *in, uint32_t *out, uint32_t width, uint32_t height,
                       hipStream_t stream);

@@ -51,6 +51,10 @@ void drain_locked() {
 int timing_begin(hipStream_t stream) {
   std::lock_guard<std::mutex> g(g_mu);
   if (!g_enabled) return -1;
+  // no event nodes inside a graph being captured
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+    return -1;
   hipEvent_t e;
   if (hipEventCreate(&e) != hipSuccess) return -1;
   if (hipEventRecord(e, stream) != hipSuccess) {

@@ -20,6 +20,9 @@ the same as the crop of that image.
 Pixels are packed as uint32 with R in the low byte (the reference's
 `Pixel::Pack`, Base/src/Pixel.cpp), i.e. bytes R, G, B, A in memory.
 """
+import os
+import threading
+
 import numpy as np
 
 
@@ -116,7 +119,32 @@ def generate(width, height, seed, edges=False, x0=0, y0=0, impl="auto"):
             raise ImportError("libsynth.so is not built")
         if L is not None:
             out = np.empty((height, width), dtype=np.uint32)
-            if L.synth_generate(out.ctypes.data, width, height, seed, int(edges), x0, y0) != 0:
+            if width % 4 or height % 4 or x0 % 4 or y0 % 4:
+                raise ValueError("sizes and offsets must be multiples of 4")
+            # row bands in parallel (the C call releases the GIL); each band
+            # is the crop of the same virtual image, so the result does not
+            # depend on the split
+            nthreads = max(1, min(16, os.cpu_count() or 1, height // 64))
+            rows = height // 4
+            bands = [(4 * (rows * i // nthreads), 4 * (rows * (i + 1) // nthreads))
+                     for i in range(nthreads)]
+            status = [0] * nthreads
+
+            def work(i):
+                r0, r1 = bands[i]
+                if r1 > r0:
+                    status[i] = L.synth_generate(out[r0:r1].ctypes.data, width, r1 - r0, seed,
+                                                 int(edges), x0, y0 + r0)
+
+            if nthreads == 1:
+                work(0)
+            else:
+                th = [threading.Thread(target=work, args=(i,)) for i in range(nthreads)]
+                for t in th:
+                    t.start()
+                for t in th:
+                    t.join()
+            if any(status):
                 raise ValueError("bad generator arguments")
             return out
     return generate_numpy(width, height, seed, edges, x0, y0)

@@ -177,6 +177,68 @@ fh_status fh_ranker_gradients(fh_ranker *ranker, const uint32_t *blocks, const i
 fh_status fh_ranker_train(fh_ranker *ranker, const uint32_t *blocks, const int32_t *labels,
                           uint32_t n, const fh_train_config *cfg, fh_train_report *report);
 
+/* ---- Batches (no single reference counterpart) ------------------------------
+ * Many surfaces (textures and their mip levels) of mixed formats encoded by
+ * one launch sequence: the GPU form of the reference's atomic job-list mode
+ * (CompressionJobList, Base/include/FasTC/CompressionJob.h:172-209, driven
+ * by CompressAtomic, BPTCEncoder/src/Compressor.cpp:1542-1574).  All blocks
+ * of a format are numbered in submission order and persistent kernels take
+ * them from device counters.  Each surface's output equals fh_compress_device
+ * of that surface alone, with BPTC solid blocks taking watermark entries in
+ * submission order from the process-wide counter, and annealing streams
+ * keyed by the block's index among the batch's BPTC blocks.
+ * All buffers are device memory; width and height are multiples of 4
+ * (callers pad small mip levels as CompressImage does,
+ * Core/src/TexComp.cpp:382-393). */
+typedef struct fh_surface {
+  uint32_t format;        /* FH_FORMAT_* */
+  const uint8_t *in;      /* width * height * 4 bytes */
+  uint8_t *out;           /* fh_compressed_size(format, width, height) bytes */
+  uint32_t width, height;
+} fh_surface;
+
+/* BPTC settings of a batch: the fields of fh_job with the same names; the
+ * watermark counter, if the caller keeps one, is a device word here. */
+typedef struct fh_batch_options {
+  uint32_t quality;
+  uint32_t block_modes;
+  uint32_t error_metric;
+  uint64_t seed;
+  uint32_t flags;           /* FH_FLAG_NO_Q0_GUARD, FH_FLAG_LEARNED */
+  const fh_ranker *ranker;
+  uint32_t ranker_topk;
+  float ranker_threshold;
+  uint32_t *wm_counter;     /* device word holding the watermark counter, or NULL for
+                               the library's process-wide one */
+} fh_batch_options;
+
+/* Launch shapes chosen by fh_batch_autotune. */
+typedef struct fh_batch_tune {
+  uint32_t wg_per_cu;      /* persistent workgroups per CU (BPTC set-up and annealing) */
+  uint32_t anneal_unroll;  /* annealing steps between refills of finished lanes */
+  uint32_t simple_wg;      /* blocks per workgroup of the DXT / ETC1 kernels */
+  float best_ms;           /* time of one run with these shapes */
+  uint32_t trials;
+} fh_batch_tune;
+
+typedef struct fh_batch fh_batch;
+
+/* Plans a batch: copies the surface table to the device and allocates every
+ * work buffer, so that runs allocate nothing.  The surfaces' buffers must
+ * stay valid while the plan is used. */
+fh_status fh_batch_create(const fh_surface *surfaces, uint32_t count,
+                          const fh_batch_options *options, fh_batch **batch);
+fh_status fh_batch_destroy(fh_batch *batch);
+/* Encodes every surface, asynchronously on hip_stream.  use_graph != 0
+ * captures the launch sequence into a HIP graph on the first call and
+ * replays it afterwards. */
+fh_status fh_batch_run(fh_batch *batch, int use_graph, void *hip_stream);
+/* Times the plan under a few launch shapes (coordinate descent) and keeps the
+ * fastest; runs the batch several times (its outputs are overwritten). */
+fh_status fh_batch_autotune(fh_batch *batch, void *hip_stream, fh_batch_tune *report);
+/* Number of blocks of one format in the batch. */
+fh_status fh_batch_blocks(const fh_batch *batch, uint32_t format, uint32_t *blocks);
+
 /* Kernel timing for benchmarks; no reference counterpart (the reference's
  * CompressImageData reports only a whole-job wall time, TexComp.cpp:517).
  * While enabled, every kernel the library launches is bracketed by HIP
