@@ -137,8 +137,9 @@ def lib():
         L.fh_batch_run.argtypes = [vp, ctypes.c_int, vp]
         L.fh_batch_autotune.argtypes = [vp, vp, ctypes.POINTER(FhBatchTune)]
         L.fh_batch_blocks.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        L.fh_batch_set_tune.argtypes = [vp, ctypes.POINTER(FhBatchTune)]
         for name in ("fh_batch_create", "fh_batch_destroy", "fh_batch_run", "fh_batch_autotune",
-                     "fh_batch_blocks"):
+                     "fh_batch_blocks", "fh_batch_set_tune"):
             getattr(L, name).restype = ctypes.c_int
         for name in ("fh_ranker_create", "fh_ranker_destroy", "fh_ranker_init",
                      "fh_ranker_set_params", "fh_ranker_get_params", "fh_ranker_forward",
@@ -155,7 +156,7 @@ EXPORTED_SYMBOLS = ["fh_compress", "fh_compress_device", "fh_decompress",
                     "fh_ranker_init", "fh_ranker_set_params", "fh_ranker_get_params",
                     "fh_ranker_forward", "fh_ranker_gradients", "fh_ranker_train",
                     "fh_batch_create", "fh_batch_destroy", "fh_batch_run", "fh_batch_autotune",
-                    "fh_batch_blocks"]
+                    "fh_batch_blocks", "fh_batch_set_tune"]
 
 
 def _check(status):
@@ -352,6 +353,10 @@ class Batch:
         _check(lib().fh_batch_autotune(self._h, ctypes.c_void_p(stream), ctypes.byref(t)))
         return {"wg_per_cu": t.wg_per_cu, "anneal_unroll": t.anneal_unroll,
                 "simple_wg": t.simple_wg, "best_ms": t.best_ms, "trials": t.trials}
+
+    def set_tune(self, tune):
+        t = FhBatchTune(tune["wg_per_cu"], tune["anneal_unroll"], tune["simple_wg"], 0.0, 0)
+        _check(lib().fh_batch_set_tune(self._h, ctypes.byref(t)))
 
     def blocks(self, fmt):
         n = ctypes.c_uint32(0)

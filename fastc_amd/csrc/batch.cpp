@@ -334,6 +334,24 @@ fh_status fh_batch_autotune(fh_batch *b, void *hip_stream, fh_batch_tune *report
   return FH_OK;
 }
 
+fh_status fh_batch_set_tune(fh_batch *b, const fh_batch_tune *tune) {
+  if (!b || !tune) return bfail(FH_ERR_INVALID_ARGUMENT, "null argument");
+  if ((tune->wg_per_cu != 8 && tune->wg_per_cu != 16 && tune->wg_per_cu != 32) ||
+      (tune->anneal_unroll != 1 && tune->anneal_unroll != 2 && tune->anneal_unroll != 4) ||
+      (tune->simple_wg != 64 && tune->simple_wg != 128 && tune->simple_wg != 256 &&
+       tune->simple_wg != 512))
+    return bfail(FH_ERR_INVALID_ARGUMENT, "launch shape outside the tuned set");
+  std::lock_guard<std::mutex> g(b->mu);
+  b->tune.wg_per_cu = tune->wg_per_cu;
+  b->tune.anneal_unroll = tune->anneal_unroll;
+  b->tune.simple_wg = tune->simple_wg;
+  if (b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return FH_OK;
+}
+
 fh_status fh_batch_blocks(const fh_batch *b, uint32_t format, uint32_t *blocks) {
   if (!b || !blocks) return bfail(FH_ERR_INVALID_ARGUMENT, "null argument");
   if (format > FH_FORMAT_BPTC) return bfail(FH_ERR_UNKNOWN_FORMAT, "Unknown compression format");

@@ -1012,13 +1012,16 @@ __device__ int box_candidates(bool opaque_layout, int nshapes, int s2, int s3, u
   return n;
 }
 
+// CandRec::meta bit set when annealing rewrote the record.
+constexpr uint32_t kMetaAnnealed = 1u << 12;
+
 // Result of one candidate, read by the pack kernel.
 struct CandRec {
   uint32_t err;       // total error; 0xFFFFFFFF = not a candidate
   uint32_t px1[3], px2[3];
   uint32_t cidx_lo, cidx_hi;
   uint32_t aidx_lo, aidx_hi;
-  uint32_t meta;      // pbc per subset (2 bits each) | rot << 8 | im << 10
+  uint32_t meta;      // pbc per subset (2 bits each) | rot << 8 | im << 10 | annealed << 12
 };
 
 // Annealing work item (start state from the set-up kernel).
@@ -1737,9 +1740,9 @@ __device__ bool lane_step(Lane &L, uint32_t q) {
   uint64_t nidx = 0;
   const float kf = (float)(nb - 1);
   const WeightFn wf = weight_fn(nb);
-  if (L.nsub == 1) score_pixels<1>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx);
-  else if (L.nsub == 2) score_pixels<2>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx);
-  else score_pixels<3>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx);
+  // one code path for every subset count: lanes of a wave hold candidates of
+  // different modes once they refill
+  score_pixels<3>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx);
 #pragma unroll
   for (int s = 0; s < 3; s++) {
     if (!((L.active >> s) & 1)) continue;
@@ -1810,7 +1813,7 @@ __device__ void lane_finish(Lane &L, Work &w) {
   }
   rc->cidx_lo = (uint32_t)L.idx;
   rc->cidx_hi = (uint32_t)(L.idx >> 32);
-  rc->meta = (rc->meta & ~0x3Fu) | pbcs;
+  rc->meta = (rc->meta & ~0x3Fu) | pbcs | kMetaAnnealed;
   rc->err = total;
 }
 
@@ -2104,22 +2107,43 @@ __global__ void __launch_bounds__(256) pack_decoded_kernel(BlockRange r, Bc7Scra
     uint32_t px[16];
     load_block(r, rel, px);
     unsigned long long best_err = ~0ull;
+    int best_i = -1;
+    bool best_packed = false;
     const int count = sel_count(sel);
     for (int i = 0; i < count; i++) {
-      Cand k;
-      cand_from_desc(w.cdesc[(size_t)rel * kMaxCands + i], k);
-      BitWriter bw;
-      pack_mode(k.mode, k.shape, w.rec[(size_t)rel * kMaxCands + i], bw);
-      const uint4 blk = make_uint4((uint32_t)bw.lo, (uint32_t)(bw.lo >> 32), (uint32_t)bw.hi,
-                                   (uint32_t)(bw.hi >> 32));
-      uint32_t dec[16];
-      bc7dec::decode_block(blk, dec);
-      const unsigned long long e = premul_err(px, dec);
-      if (cerr) cerr[(size_t)rel * kMaxCands + i] = e;
+      const CandRec &rc = w.rec[(size_t)rel * kMaxCands + i];
+      unsigned long long e;
+      uint4 blk = make_uint4(0, 0, 0, 0);
+      bool packed = false;
+      if (final_pass && cerr && !(rc.meta & kMetaAnnealed)) {
+        e = cerr[(size_t)rel * kMaxCands + i];  // record unchanged since pass 0
+      } else {
+        Cand k;
+        cand_from_desc(w.cdesc[(size_t)rel * kMaxCands + i], k);
+        BitWriter bw;
+        pack_mode(k.mode, k.shape, rc, bw);
+        blk = make_uint4((uint32_t)bw.lo, (uint32_t)(bw.lo >> 32), (uint32_t)bw.hi,
+                         (uint32_t)(bw.hi >> 32));
+        packed = true;
+        uint32_t dec[16];
+        bc7dec::decode_block(blk, dec);
+        e = premul_err(px, dec);
+        if (!final_pass && cerr) cerr[(size_t)rel * kMaxCands + i] = e;
+      }
       if (e < best_err) {
         best_err = e;
         best = blk;
+        best_i = i;
+        best_packed = packed;
       }
+    }
+    if (best_i >= 0 && !best_packed) {
+      Cand k;
+      cand_from_desc(w.cdesc[(size_t)rel * kMaxCands + best_i], k);
+      BitWriter bw;
+      pack_mode(k.mode, k.shape, w.rec[(size_t)rel * kMaxCands + best_i], bw);
+      best = make_uint4((uint32_t)bw.lo, (uint32_t)(bw.lo >> 32), (uint32_t)bw.hi,
+                        (uint32_t)(bw.hi >> 32));
     }
     if (!final_pass) {
       q0blk[rel] = best;
@@ -2358,6 +2382,9 @@ hipError_t bc7_prepare() {
   if (e != hipSuccess) return e;
   e = bc7_upload_tables(dev);
   if (e != hipSuccess) return e;
+  // the device state (single-colour table, watermark counter) is created on
+  // first use
+  if (!device_wm_counter(dev)) return hipErrorNotInitialized;
   return bc7_single_color_table(dev) ? hipSuccess : hipErrorNotInitialized;
 }
 
@@ -2445,7 +2472,10 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   uint32_t *fallback_count = w.counters + 20;
   const uint32_t nchunks = (n + 255) / 256;
   const uint32_t ngroups = (n + 15) / 16;
-  const uint32_t persistent = (uint32_t)g_num_cus[dev] * (p.wg_per_cu ? p.wg_per_cu : 16u);
+  // persistent workgroups of one wave, no more than the candidates could use
+  uint32_t persistent = (uint32_t)g_num_cus[dev] * (p.wg_per_cu ? p.wg_per_cu : 16u);
+  const uint64_t max_waves = ((uint64_t)n * bc7::kMaxCands + 63) / 64;
+  if (persistent > max_waves) persistent = (uint32_t)max_waves;
   const dim3 pack_grid((n + 255) / 256);
   const bool guard = p.quality > 0 && p.q0_guard;
   int t = timing_begin(stream);
@@ -2507,7 +2537,7 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
     t = timing_begin(stream);
     hipLaunchKernelGGL(bc7::pack_decoded_kernel, pack_grid, dim3(256), 0, stream, r, s, w, q0,
                        p.quality > 0 ? q0err : (unsigned long long *)nullptr,
-                       (unsigned long long *)nullptr, 1);
+                       p.quality > 0 ? cerr : (unsigned long long *)nullptr, 1);
     timing_end(t, kTagBc7Pack, stream);
     return hipGetLastError();
   }
