@@ -1,0 +1,71 @@
+"""What the trained ranker gains, on a held-out 2048x2048 image (seed 7,
+q=50), against the same search without it:
+
+* at the 0.6 confidence threshold (about one block in eight is confident)
+  the search is faster than with every block on BoxSelection's candidates;
+* without the threshold the search is much faster still, and the trained
+  network's top-4 keeps the PSNR an untrained network's top-4 loses;
+* every learned variant stays within 0.1 dB of the reference's full search
+  or above it (the decoded-error choice among candidates raises it)."""
+import os
+
+import numpy as np
+import pytest
+
+from fastc_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "fastc_amd", "data", "bc7_ranker.npy")
+
+
+def _psnr(a, b):
+    a = a.view(np.uint8).reshape(-1, 4).astype(np.float64)
+    b = b.view(np.uint8).reshape(-1, 4).astype(np.float64)
+    d = a[:, :3] * (a[:, 3:4] / 255.0) - b[:, :3] * (b[:, 3:4] / 255.0)
+    return 10 * np.log10(3 * 255.0 ** 2 / ((d * d).sum() / a.shape[0]))
+
+
+def test_trained_ranker_gains(fh):
+    import torch
+    n = 2048
+    img = synth.generate(n, n, 7)
+    d_in = torch.from_numpy(img.view(np.int32)).cuda()
+    out_bytes = n * n
+    d_out = torch.zeros(out_bytes, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    trained = fh.Ranker()
+    trained.set_params(np.load(WEIGHTS))
+    untrained = fh.Ranker(seed=123)
+
+    def run(ranker, threshold):
+        def once():
+            fh.compress_device(d_in.data_ptr(), d_out.data_ptr(), out_bytes, n, n, quality=50,
+                               seed=7, stream=stream.cuda_stream, ranker=ranker,
+                               threshold=threshold)
+        once()
+        times = []
+        for _ in range(3):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            once()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            times.append(e0.elapsed_time(e1))
+        dec = fh.decompress(d_out.cpu().numpy(), n, n)
+        return float(np.median(times)), _psnr(img, dec)
+
+    full = run(None, 0.6)
+    learned = run(trained, 0.6)
+    no_pruning = run(trained, 2.0)  # every block falls back to BoxSelection
+    learned_all = run(trained, 0.0)
+    untrained_all = run(untrained, 0.0)
+    print(f"(ms, dB): full {full}, learned {learned}, no pruning {no_pruning}, "
+          f"learned thr 0 {learned_all}, untrained thr 0 {untrained_all}", flush=True)
+    assert learned[1] >= full[1] - 0.1 and learned_all[1] >= full[1] - 0.1
+    assert learned[0] < 0.98 * no_pruning[0]
+    assert learned_all[0] < 0.8 * no_pruning[0]
+    assert learned_all[1] >= no_pruning[1] - 0.05
+    assert learned_all[1] > untrained_all[1] + 0.5
