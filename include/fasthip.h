@@ -236,6 +236,9 @@ fh_status fh_batch_run(fh_batch *batch, int use_graph, void *hip_stream);
 /* Times the plan under a few launch shapes (coordinate descent) and keeps the
  * fastest; runs the batch several times (its outputs are overwritten). */
 fh_status fh_batch_autotune(fh_batch *batch, void *hip_stream, fh_batch_tune *report);
+/* Applies launch shapes found for another batch (wg_per_cu 8/16/32,
+ * anneal_unroll 1/2/4, simple_wg 64..512). */
+fh_status fh_batch_set_tune(fh_batch *batch, const fh_batch_tune *tune);
 /* Number of blocks of one format in the batch. */
 fh_status fh_batch_blocks(const fh_batch *batch, uint32_t format, uint32_t *blocks);
 
