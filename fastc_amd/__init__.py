@@ -138,6 +138,16 @@ def lib():
         L.fh_batch_autotune.argtypes = [vp, vp, ctypes.POINTER(FhBatchTune)]
         L.fh_batch_blocks.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
         L.fh_batch_set_tune.argtypes = [vp, ctypes.POINTER(FhBatchTune)]
+        u32 = ctypes.c_uint32
+        L.fh_decompress_device.argtypes = [u32, vp, vp, u32, u32, vp]
+        L.fh_block_error_device.argtypes = [u32, vp, vp, u32, u32, vp, vp, vp]
+        L.fh_image_error_device.argtypes = [vp, vp, u32, u32, vp, vp, vp]
+        L.fh_psnr.argtypes = [vp, vp, u32, u32, ctypes.POINTER(ctypes.c_double)]
+        L.fh_psnr_from_sse.argtypes = [ctypes.POINTER(ctypes.c_ulonglong), u32, u32]
+        L.fh_psnr_from_sse.restype = ctypes.c_double
+        for name in ("fh_decompress_device", "fh_block_error_device", "fh_image_error_device",
+                     "fh_psnr"):
+            getattr(L, name).restype = ctypes.c_int
         for name in ("fh_batch_create", "fh_batch_destroy", "fh_batch_run", "fh_batch_autotune",
                      "fh_batch_blocks", "fh_batch_set_tune"):
             getattr(L, name).restype = ctypes.c_int
@@ -156,7 +166,9 @@ EXPORTED_SYMBOLS = ["fh_compress", "fh_compress_device", "fh_decompress",
                     "fh_ranker_init", "fh_ranker_set_params", "fh_ranker_get_params",
                     "fh_ranker_forward", "fh_ranker_gradients", "fh_ranker_train",
                     "fh_batch_create", "fh_batch_destroy", "fh_batch_run", "fh_batch_autotune",
-                    "fh_batch_blocks", "fh_batch_set_tune"]
+                    "fh_batch_blocks", "fh_batch_set_tune", "fh_decompress_device",
+                    "fh_block_error_device", "fh_image_error_device", "fh_psnr_from_sse",
+                    "fh_psnr"]
 
 
 def _check(status):
@@ -365,9 +377,51 @@ class Batch:
 
 
 def decompress(data, width, height, fmt=eCompressionFormat_BPTC):
-    """Decode on the GPU to a (h, w) uint32 image (BPTCC::Decompress)."""
+    """Decode on the GPU to a (h, w) uint32 image (BPTCC::Decompress,
+    DXTC::DecompressDXT1/DXT5, ETCC::Decompress)."""
     data = np.ascontiguousarray(data, dtype=np.uint8)
     out = np.zeros((height, width), dtype=np.uint32)
     _check(lib().fh_decompress(fmt, data.ctypes.data, data.nbytes, out.ctypes.data, width,
                                height))
     return out
+
+
+def decompress_device(in_ptr, out_ptr, width, height, fmt=eCompressionFormat_BPTC, stream=0):
+    """Decode device data to a device RGBA8 image, asynchronously on `stream`."""
+    _check(lib().fh_decompress_device(fmt, ctypes.c_void_p(in_ptr), ctypes.c_void_p(out_ptr),
+                                      width, height, ctypes.c_void_p(stream)))
+
+
+def block_error_device(orig_ptr, enc_ptr, width, height, sse_ptr, block_err_ptr=0,
+                       fmt=eCompressionFormat_BPTC, stream=0):
+    """Premultiplied error of an encoded device surface against the original
+    (fh_block_error_device): per-block uint64 errors (optional) and the
+    128-bit total as two uint64 words at sse_ptr."""
+    _check(lib().fh_block_error_device(fmt, ctypes.c_void_p(orig_ptr), ctypes.c_void_p(enc_ptr),
+                                       width, height, ctypes.c_void_p(block_err_ptr or None),
+                                       ctypes.c_void_p(sse_ptr), ctypes.c_void_p(stream)))
+
+
+def image_error_device(a_ptr, b_ptr, width, height, sse_ptr, block_err_ptr=0, stream=0):
+    """The same comparison between two RGBA8 device images."""
+    _check(lib().fh_image_error_device(ctypes.c_void_p(a_ptr), ctypes.c_void_p(b_ptr), width,
+                                       height, ctypes.c_void_p(block_err_ptr or None),
+                                       ctypes.c_void_p(sse_ptr), ctypes.c_void_p(stream)))
+
+
+def psnr_from_sse(sse, width, height):
+    """PSNR (dB) from the two-word total of the error functions."""
+    words = (ctypes.c_ulonglong * 2)(int(sse[0]), int(sse[1]))
+    return lib().fh_psnr_from_sse(words, width, height)
+
+
+def psnr(a, b):
+    """Image::ComputePSNR of two (h, w) uint32 RGBA8 images, on the GPU."""
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    if a.shape != b.shape:
+        return -1.0
+    out = ctypes.c_double(0)
+    _check(lib().fh_psnr(a.ctypes.data, b.ctypes.data, a.shape[1], a.shape[0],
+                         ctypes.byref(out)))
+    return out.value

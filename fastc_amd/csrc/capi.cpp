@@ -5,6 +5,7 @@
 // prints to stderr is available from fh_last_error().
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <math.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -45,6 +46,7 @@ struct DeviceState {
   size_t in_size = 0;
   uint8_t *out_buf = nullptr;
   size_t out_size = 0;
+  unsigned long long *sse = nullptr;  // 2 words, fh_psnr's total
   bool init = false;
 };
 
@@ -71,8 +73,19 @@ hipError_t ensure_init(DeviceState *st) {
   if (e != hipSuccess) return e;
   e = hipMemcpy(st->sctab, tab.data(), tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
   if (e != hipSuccess) return e;
+  e = hipMalloc(&st->sse, 2 * sizeof(unsigned long long));
+  if (e != hipSuccess) return e;
   st->init = true;
   return hipSuccess;
+}
+
+bool aligned(const void *p, size_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
+
+// Block counts of decoders and error reductions stay within 32-bit lane
+// indices (with room for the last workgroup).
+bool decode_dims_ok(uint32_t width, uint32_t height) {
+  const uint64_t nb = (uint64_t)((width + 3ull) / 4) * ((height + 3ull) / 4);
+  return nb <= 0xFFFFFF00ull;
 }
 
 hipError_t grow(void **p, size_t *cur, size_t need) {
@@ -333,8 +346,10 @@ fh_status fh_decompress(uint32_t format, const uint8_t *in, size_t in_size, uint
                         uint32_t width, uint32_t height) {
   if (!in || !out || width == 0 || height == 0)
     return fail(FH_ERR_INVALID_ARGUMENT, "null buffer or empty image");
-  if (format != FH_FORMAT_BPTC) return fail(FH_ERR_UNSUPPORTED, "decode implemented for BPTC only");
-  const size_t need = (size_t)((width + 3) / 4) * ((height + 3) / 4) * 16;
+  const uint32_t bb = block_bytes(format);
+  if (!bb) return fail(FH_ERR_UNKNOWN_FORMAT, "Unknown compression format");
+  if (!decode_dims_ok(width, height)) return fail(FH_ERR_INVALID_ARGUMENT, "image too large");
+  const size_t need = (size_t)((width + 3) / 4) * ((height + 3) / 4) * bb;
   if (in_size < need) return fail(FH_ERR_BUFFER_TOO_SMALL, "compressed buffer too small");
   fh_status s;
   const int dev = current_device(&s);
@@ -350,10 +365,89 @@ fh_status fh_decompress(uint32_t format, const uint8_t *in, size_t in_size, uint
   if (e != hipSuccess) return fail_hip(e, "device buffer allocation");
   e = hipMemcpy(st->in_buf, in, need, hipMemcpyHostToDevice);
   if (e != hipSuccess) return fail_hip(e, "input upload");
-  e = bc7_decode(st->in_buf, reinterpret_cast<uint32_t *>(st->out_buf), width, height, nullptr);
+  e = texture_decode(format, st->in_buf, reinterpret_cast<uint32_t *>(st->out_buf), width, height,
+                     nullptr);
   if (e != hipSuccess) return fail_hip(e, "decode launch");
   e = hipMemcpy(out, st->out_buf, out_bytes, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return fail_hip(e, "output download");
+  return FH_OK;
+}
+
+fh_status fh_decompress_device(uint32_t format, const uint8_t *in, uint8_t *out, uint32_t width,
+                               uint32_t height, void *hip_stream) {
+  if (!in || !out) return fail(FH_ERR_INVALID_ARGUMENT, "null buffer");
+  if (!block_bytes(format)) return fail(FH_ERR_UNKNOWN_FORMAT, "Unknown compression format");
+  if (!decode_dims_ok(width, height)) return fail(FH_ERR_INVALID_ARGUMENT, "image too large");
+  if (!aligned(in, block_bytes(format)) || !aligned(out, 4))
+    return fail(FH_ERR_INVALID_ARGUMENT, "misaligned buffer (blocks: block size, pixels: 4 B)");
+  hipError_t e = texture_decode(format, in, reinterpret_cast<uint32_t *>(out), width, height,
+                                reinterpret_cast<hipStream_t>(hip_stream));
+  if (e != hipSuccess) return fail_hip(e, "decode launch");
+  return FH_OK;
+}
+
+fh_status fh_block_error_device(uint32_t format, const uint32_t *original, const uint8_t *encoded,
+                                uint32_t width, uint32_t height,
+                                unsigned long long *block_errors, unsigned long long *sse,
+                                void *hip_stream) {
+  if (!original || !encoded || !sse) return fail(FH_ERR_INVALID_ARGUMENT, "null buffer");
+  if (!block_bytes(format)) return fail(FH_ERR_UNKNOWN_FORMAT, "Unknown compression format");
+  if (!decode_dims_ok(width, height)) return fail(FH_ERR_INVALID_ARGUMENT, "image too large");
+  if (!aligned(encoded, block_bytes(format)) || !aligned(original, 4) || !aligned(sse, 8) ||
+      !aligned(block_errors, 8))
+    return fail(FH_ERR_INVALID_ARGUMENT, "misaligned buffer");
+  hipError_t e = block_error((int)format, original, encoded, width, height, block_errors, sse,
+                             reinterpret_cast<hipStream_t>(hip_stream));
+  if (e != hipSuccess) return fail_hip(e, "error launch");
+  return FH_OK;
+}
+
+fh_status fh_image_error_device(const uint32_t *a, const uint32_t *b, uint32_t width,
+                                uint32_t height, unsigned long long *block_errors,
+                                unsigned long long *sse, void *hip_stream) {
+  if (!a || !b || !sse) return fail(FH_ERR_INVALID_ARGUMENT, "null buffer");
+  if (!decode_dims_ok(width, height)) return fail(FH_ERR_INVALID_ARGUMENT, "image too large");
+  if (!aligned(a, 4) || !aligned(b, 4) || !aligned(sse, 8) || !aligned(block_errors, 8))
+    return fail(FH_ERR_INVALID_ARGUMENT, "misaligned buffer");
+  hipError_t e = block_error(kFormatRgba8, a, b, width, height, block_errors, sse,
+                             reinterpret_cast<hipStream_t>(hip_stream));
+  if (e != hipSuccess) return fail_hip(e, "error launch");
+  return FH_OK;
+}
+
+double fh_psnr_from_sse(const unsigned long long sse[2], uint32_t width, uint32_t height) {
+  // sse is 255^2 times the reference's sum of squared differences
+  const double total = (double)sse[1] * 18446744073709551616.0 + (double)sse[0];
+  const double mse = total / (255.0 * 255.0) / ((double)width * (double)height);
+  return 10.0 * log10(3.0 * 255.0 * 255.0 / mse);
+}
+
+fh_status fh_psnr(const uint8_t *a, const uint8_t *b, uint32_t width, uint32_t height,
+                  double *psnr) {
+  if (!a || !b || !psnr || width == 0 || height == 0)
+    return fail(FH_ERR_INVALID_ARGUMENT, "null buffer or empty image");
+  if (!decode_dims_ok(width, height)) return fail(FH_ERR_INVALID_ARGUMENT, "image too large");
+  fh_status s;
+  const int dev = current_device(&s);
+  if (dev < 0) return s;
+  DeviceState *st = device_state(dev);
+  std::lock_guard<std::mutex> g(st->mu);
+  hipError_t e = ensure_init(st);
+  if (e != hipSuccess) return fail_hip(e, "device initialisation");
+  const size_t bytes = (size_t)width * height * 4;
+  e = grow(reinterpret_cast<void **>(&st->in_buf), &st->in_size, bytes);
+  if (e == hipSuccess) e = grow(reinterpret_cast<void **>(&st->out_buf), &st->out_size, bytes);
+  if (e != hipSuccess) return fail_hip(e, "device buffer allocation");
+  e = hipMemcpy(st->in_buf, a, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(st->out_buf, b, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail_hip(e, "image upload");
+  e = block_error(kFormatRgba8, reinterpret_cast<const uint32_t *>(st->in_buf), st->out_buf,
+                  width, height, nullptr, st->sse, nullptr);
+  if (e != hipSuccess) return fail_hip(e, "error launch");
+  unsigned long long total[2];
+  e = hipMemcpy(total, st->sse, sizeof(total), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return fail_hip(e, "error download");
+  *psnr = fh_psnr_from_sse(total, width, height);
   return FH_OK;
 }
 

@@ -117,8 +117,18 @@ hipError_t bc7_gather(const SurfDesc *surfs, const uint32_t *block_surf, uint32_
 hipError_t bc7_scatter(const SurfDesc *surfs, const uint32_t *block_surf, uint32_t g0,
                        uint32_t n, const uint8_t *enc, hipStream_t stream);
 
-hipError_t bc7_decode(const uint8_t *in, uint32_t *out, uint32_t width, uint32_t height,
-                      hipStream_t stream);
+// Decoders and error reductions (decode.hip).  texture_decode writes the
+// width x height RGBA8 image of an encoded DXT1/DXT5/ETC1/BPTC surface.
+// block_error compares `original` with the decoded `encoded` surface
+// (format >= 0) or with the RGBA8 image `encoded` (format kFormatRgba8):
+// per-block premultiplied errors (block_err, optional) and their 128-bit
+// total in sse[0] (low) and sse[1] (high), both overwritten.
+constexpr int kFormatRgba8 = -1;
+hipError_t texture_decode(uint32_t format, const uint8_t *in, uint32_t *out, uint32_t width,
+                          uint32_t height, hipStream_t stream);
+hipError_t block_error(int format, const uint32_t *original, const void *encoded, uint32_t width,
+                       uint32_t height, unsigned long long *block_err, unsigned long long *sse,
+                       hipStream_t stream);
 
 // BC7 candidate ranker (mlp.hip): 64 -> 256 -> 256 -> 512 MLP, fp32 master
 // weights with Adam moments, bf16 copies for the matrix cores.
@@ -194,9 +204,10 @@ enum KernelTag {
   kTagBc7Guard,
   kTagDxt,
   kTagEtc1,
-  kTagBc7Decode,
   kTagBc7Labels,
   kTagBc7Rank,
+  kTagDecode,
+  kTagBlockError,
   kNumKernelTags
 };
 int timing_begin(hipStream_t stream);

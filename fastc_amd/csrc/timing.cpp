@@ -18,7 +18,7 @@ namespace {
 
 const char *const kTagNames[kNumKernelTags] = {
     "bc7_scan", "bc7_select", "bc7_setup", "bc7_anneal", "bc7_pack", "bc7_guard",
-    "dxt",      "etc1",       "bc7_decode", "bc7_labels", "bc7_rank"};
+    "dxt",      "etc1",       "bc7_labels", "bc7_rank", "decode",    "block_error"};
 
 struct Pending {
   int tag;

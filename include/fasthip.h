@@ -18,7 +18,11 @@
  *   fh_decompress        BPTCC::Decompress (BPTCCompressor.h:216),
  *                        DXTC::DecompressDXT1/5, ETCC::Decompress
  *                        (via CompressedImage::DecompressImage,
- *                        Core/src/CompressedImage.cpp:86-118).
+ *                        Core/src/CompressedImage.cpp:86-118); the _device
+ *                        variant on device buffers.
+ *   fh_psnr, fh_block_error_device, fh_image_error_device
+ *                        Image::ComputePSNR (Base/src/Image.cpp:205-255) and
+ *                        its per-block terms, fused with decoding.
  *   fh_compressed_size   CompressedImage::GetCompressedSize
  *                        (Core/src/CompressedImage.cpp:136-149).
  *   fh_last_error        the stderr message CompressImageData prints
@@ -113,8 +117,46 @@ typedef struct fh_job {
 fh_status fh_compress(const fh_job *job);
 fh_status fh_compress_device(const fh_job *job, void *hip_stream);
 
+/* Decodes a width x height surface of any format to RGBA8 (host buffers:
+ * in holds ceil(w/4) * ceil(h/4) blocks, out w * h * 4 bytes), bit-exact
+ * with BPTCC::Decompress (BPTCEncoder/src/Decompressor.cpp:345-367),
+ * DXTC::DecompressDXT1/DXT5 (DXTEncoder/src/Decompressor.cpp:96-155) and
+ * ETCC::Decompress (ETCEncoder/src/Decompressor.cpp:27-48): DXT1 output
+ * alpha is always 255, ETC1 alpha 255, edge blocks clipped to the image. */
 fh_status fh_decompress(uint32_t format, const uint8_t *in, size_t in_size, uint8_t *out,
                         uint32_t width, uint32_t height);
+/* The same on device buffers (in aligned to the block size, out to 4 bytes),
+ * asynchronous on hip_stream. */
+fh_status fh_decompress_device(uint32_t format, const uint8_t *in, uint8_t *out, uint32_t width,
+                               uint32_t height, void *hip_stream);
+
+/* Image comparison of Image::ComputePSNR (Base/src/Image.cpp:205-255).  The
+ * error of a pixel pair is the sum over R, G, B of (a*c - a'*c')^2 with
+ * 8-bit values, i.e. 255^2 times the reference's squared difference of
+ * alpha-premultiplied channels; it is an exact integer, so the totals below
+ * do not depend on summation order.
+ *
+ * fh_block_error_device decodes `encoded` (a surface of `format`) block by
+ * block and compares it with `original` (RGBA8, width * height) without
+ * writing the decoded image; fh_image_error_device compares two RGBA8
+ * images.  Device buffers, blocks aligned to their size, pixels to 4 bytes,
+ * error words to 8 bytes.  block_errors (NULL or ceil(w/4) * ceil(h/4)
+ * words, raster block order) receives each block's error over its pixels
+ * inside the image; sse[0] (low) and sse[1] (high) receive the 128-bit total.
+ * Both are overwritten.  Asynchronous on hip_stream. */
+fh_status fh_block_error_device(uint32_t format, const uint32_t *original, const uint8_t *encoded,
+                                uint32_t width, uint32_t height,
+                                unsigned long long *block_errors, unsigned long long *sse,
+                                void *hip_stream);
+fh_status fh_image_error_device(const uint32_t *a, const uint32_t *b, uint32_t width,
+                                uint32_t height, unsigned long long *block_errors,
+                                unsigned long long *sse, void *hip_stream);
+/* PSNR in dB from such a total (host sse[2]): 10 log10(3 * 255^2 / mse) with
+ * mse = sse / 255^2 / (width * height), +inf for identical images. */
+double fh_psnr_from_sse(const unsigned long long sse[2], uint32_t width, uint32_t height);
+/* Image::ComputePSNR of two RGBA8 host images of the same size. */
+fh_status fh_psnr(const uint8_t *a, const uint8_t *b, uint32_t width, uint32_t height,
+                  double *psnr);
 
 fh_status fh_compressed_size(uint32_t format, uint32_t width, uint32_t height, size_t *size);
 
@@ -248,7 +290,8 @@ fh_status fh_batch_blocks(const fh_batch *batch, uint32_t format, uint32_t *bloc
  * events on its own stream.  Enabling or disabling clears the totals.
  * fh_kernel_time waits for the recorded events and returns the summed time
  * and launch count of one kernel: "bc7_scan", "bc7_select", "bc7_setup",
- * "bc7_anneal", "bc7_pack", "bc7_guard", "dxt", "etc1", "bc7_decode". */
+ * "bc7_anneal", "bc7_pack", "bc7_guard", "dxt", "etc1", "bc7_labels",
+ * "bc7_rank", "decode", "block_error". */
 fh_status fh_set_kernel_timing(int enable);
 fh_status fh_kernel_time(const char *kernel, double *total_ms, uint32_t *launches);
 

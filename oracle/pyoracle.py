@@ -86,6 +86,10 @@ def lib():
         L.ora_counter_rng.restype = ctypes.c_uint32
         L.ora_fit_key.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32]
         L.ora_fit_key.restype = ctypes.c_uint32
+        L.ora_dxt_decompress.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, vp]
+        L.ora_etc1_decompress.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, vp]
+        L.ora_block_errors.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp]
+        L.ora_block_errors.restype = ctypes.c_uint64
         _lib = L
     return _lib
 
@@ -128,6 +132,32 @@ def bc7_decompress(data, width, height):
     out = np.zeros((height, width), dtype=np.uint32)
     lib().ora_bc7_decompress(_p(data), _p(out), width, height)
     return out
+
+
+def dxt_decompress(data, width, height, dxt5):
+    """DXTC::DecompressDXT1 / DecompressDXT5 to a (h, w) uint32 image."""
+    data = np.ascontiguousarray(data, dtype=np.uint8)
+    out = np.zeros((height, width), dtype=np.uint32)
+    lib().ora_dxt_decompress(_p(data), width, height, int(dxt5), _p(out))
+    return out
+
+
+def etc1_decompress(data, width, height):
+    """ETCC::Decompress to a (h, w) uint32 image."""
+    data = np.ascontiguousarray(data, dtype=np.uint8)
+    out = np.zeros((height, width), dtype=np.uint32)
+    lib().ora_etc1_decompress(_p(data), width, height, _p(out))
+    return out
+
+
+def block_errors(a, b):
+    """Per-block premultiplied errors (raster block order) and their total."""
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    h, w = a.shape
+    out = np.zeros(((h + 3) // 4) * ((w + 3) // 4), dtype=np.uint64)
+    total = lib().ora_block_errors(_p(a), _p(b), w, h, _p(out))
+    return out, total
 
 
 def bc7_decode_block_spec(blk):

@@ -17,6 +17,11 @@ Fixtures written (data only, no reference source text):
                    64x64 generator image (seed 2, hard edges).
   dxt_etc_hashes.json  SHA-256 of the same encoders' output for the 256x256
                    and 2048x2048 seed-2 images (the DXT workload's full size).
+  decode.npz       the reference's rg_etc1::unpack_etc1_block (built from the
+                   reference tree) on 256 random 8-byte blocks (64x64,
+                   numpy seed 11) and on the 64x64 ETC1 fixture above.
+
+    python tests/golden/make_golden.py --decode-only   rewrites decode.npz only
 """
 import ctypes
 import hashlib
@@ -76,6 +81,18 @@ def encode_all(L, img):
     return d1, d5, e1
 
 
+def write_decode_fixture(L):
+    from fastc_amd import synth
+    rnd = np.random.default_rng(11).integers(0, 256, 64 * 64 // 2, dtype=np.uint8)
+    enc = encode_all(L, synth.generate(64, 64, 2, edges=True))[2]
+    out = {"etc1_random": rnd, "etc1_encoded": enc}
+    for key, name in (("etc1_random", "etc1_random_decoded"), ("etc1_encoded", "etc1_decoded")):
+        dec = np.zeros((64, 64), np.uint32)
+        L.ref_etc1_decompress(out[key].ctypes.data, 64, 64, dec.ctypes.data)
+        out[name] = dec
+    np.savez_compressed(os.path.join(HERE, "decode.npz"), **out)
+
+
 def main():
     from fastc_amd import synth
     text = open(os.path.join(REF, "BPTCEncoder", "src", "BCLookupTables.h")).read()
@@ -93,6 +110,10 @@ def main():
         return
 
     L = ref_lib()
+    write_decode_fixture(L)
+    if "--decode-only" in sys.argv:
+        print("decode fixture written")
+        return
     img = synth.generate(64, 64, 2, edges=True)
     d1, d5, e1 = encode_all(L, img)
     np.savez_compressed(os.path.join(HERE, "dxt_etc.npz"), image=img, dxt1=d1, dxt5=d5, etc1=e1)
