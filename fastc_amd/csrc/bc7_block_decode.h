@@ -53,131 +53,134 @@ static __constant__ uint8_t kA3[2][64] = {
     {15, 8,  8,  3,  15, 15, 3,  8,  15, 15, 15, 15, 15, 15, 15, 8,  15, 8,  15, 3,  15, 8,
      15, 8,  3,  15, 6,  10, 15, 15, 10, 8,  15, 3,  15, 10, 10, 8,  9,  10, 6,  15, 8,  15,
      3,  6,  6,  8,  15, 3,  15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 3,  15, 15, 8}};
-// weights for 1..4-bit indices; 1-bit entries beyond index 2 are zero, and
-// 2-bit entries past 3 are zero, which the mode-4/index-mode-1 path reads.
-static __constant__ uint8_t kW0[4][16] = {
-    {64, 33, 0},
-    {64, 43, 21, 0},
-    {64, 55, 46, 37, 27, 18, 9, 0},
-    {64, 60, 55, 51, 47, 43, 38, 34, 30, 26, 21, 17, 13, 9, 4, 0}};
-static __constant__ uint8_t kW1[4][16] = {
-    {0, 31, 64},
-    {0, 21, 43, 64},
-    {0, 9, 18, 27, 37, 46, 55, 64},
-    {0, 4, 9, 13, 17, 21, 26, 30, 34, 38, 43, 47, 51, 55, 60, 64}};
-
-struct Reader {
-  uint64_t lo, hi;
-  int pos;
-  __device__ __forceinline__ uint32_t get(int n) {
-    if (n <= 0) return 0;
-    uint64_t v;
-    if (pos >= 64) {
-      v = hi >> (pos - 64);
-    } else {
-      v = lo >> pos;
-      if (pos + n > 64) v |= hi << (64 - pos);
-    }
-    pos += n;
-    return (uint32_t)(v & ((1ull << n) - 1ull));
-  }
-};
-
-__device__ __forceinline__ int subset_of(int i, int shape, int ns) {
-  if (ns == 2) return (kS2[shape] >> i) & 1;
-  if (ns == 3) {
-    if ((kS3[shape][0] >> i) & 1) return 1 + ((kS3[shape][1] >> i) & 1);
-    return 0;
-  }
-  return 0;
+// Bits [pos, pos + n) of the 128-bit block (n <= 8; pos + n <= 128).
+__device__ __forceinline__ uint32_t bits_at(uint64_t lo, uint64_t hi, uint32_t pos, uint32_t n) {
+  // shift counts masked to 0..63: the branch not taken never shifts out of range
+  const uint64_t v = pos >= 64 ? (hi >> ((pos - 64) & 63u))
+                               : ((lo >> (pos & 63u)) | ((hi << 1) << ((63 - pos) & 63u)));
+  return (uint32_t)v & ((1u << n) - 1u);
 }
-__device__ __forceinline__ int anchor_of(int s, int shape, int ns) {
-  if (s == 1) return ns == 2 ? kA2[shape] : kA3[0][shape];
-  if (s == 2) return kA3[1][shape];
-  return 0;
+
+// Interpolation weight of index v for n-bit indices, the reference's
+// kInterpolationValues (Compressor.cpp:159-168): round(64 v / (2^n - 1)),
+// computed as (128 v + m) / (2 m) with a 16-bit reciprocal that is exact for
+// these operands; 0 for an index past the table row, which the mode-4 index-mode-1
+// quirk reads.
+__device__ __forceinline__ uint32_t weight1(uint32_t v, uint32_t n) {
+  const uint32_t m = (1u << n) - 1u;
+  const uint32_t recip = n == 2 ? 10923u : (n == 3 ? 4682u : 2185u);
+  return v <= m ? ((128u * v + m) * recip) >> 16 : 0u;
+}
+
+// One channel of the interpolated colour: (e0 * w0 + e1 * w1 + 32) >> 6 with
+// w0 = 64 - w1 for a valid index, and both weights zero past the table.
+__device__ __forceinline__ uint32_t lerp_chan(uint32_t e0, uint32_t e1, uint32_t w1, bool valid) {
+  return valid ? ((e0 * (64u - w1) + e1 * w1 + 32u) >> 6) & 0xFFu : 0u;
 }
 
 // Decodes one 16-byte block (little-endian 128-bit stream) into 16 RGBA8
-// pixels in raster order.
+// pixels in raster order.  Every field position is computed from the mode's
+// attributes, so lanes holding blocks of different modes run one path and
+// nothing is indexed dynamically in private memory.
 __device__ inline void decode_block(const uint4 raw, uint32_t px[16]) {
-  Reader r;
-  r.lo = (uint64_t)raw.x | ((uint64_t)raw.y << 32);
-  r.hi = (uint64_t)raw.z | ((uint64_t)raw.w << 32);
-  r.pos = 0;
-  int mode = 0;
-  while (mode < 8 && !r.get(1)) mode++;
-  if (mode >= 8) {
+  const uint64_t lo = (uint64_t)raw.x | ((uint64_t)raw.y << 32);
+  const uint64_t hi = (uint64_t)raw.z | ((uint64_t)raw.w << 32);
+  const uint32_t first = raw.x & 0xFFu;
+  if (first == 0) {  // reserved mode
 #pragma unroll
     for (int i = 0; i < 16; i++) px[i] = 0;
-  } else {
-    const Attr a = kAttr[mode];
-    const int ns = a.subsets;
-    int shape = 0, rot = 0, im = 0;
-    if (ns > 1) shape = (int)r.get(a.part_bits);
-    else if (a.rot_bits) {
-      rot = (int)r.get(2);
-      if (a.idxm_bits) im = (int)r.get(1);
+    return;
+  }
+  const int mode = __builtin_ctz(first);
+  const Attr a = kAttr[mode];
+  const uint32_t ns = a.subsets, cp = a.cprec, ap = a.aprec;
+  uint32_t pos = mode + 1;
+  const uint32_t shape = bits_at(lo, hi, pos, a.part_bits);
+  pos += a.part_bits;
+  const uint32_t rot = bits_at(lo, hi, pos, a.rot_bits);
+  pos += a.rot_bits;
+  const uint32_t im = bits_at(lo, hi, pos, a.idxm_bits);
+  pos += a.idxm_bits;
+  // endpoints: R, G, B of every subset and endpoint, then A, then p-bits
+  const uint32_t apos = pos + 6 * ns * cp;
+  const uint32_t pbpos = apos + 2 * ns * ap;
+  const uint32_t npb = a.pbit == 1 ? ns : (a.pbit == 2 ? 2 * ns : 0);
+  const uint32_t cpp = cp + (a.pbit ? 1 : 0), app = ap + (a.pbit ? 1 : 0);
+  uint32_t ep[3][2];  // packed RGBA8 endpoints
+#pragma unroll
+  for (uint32_t s = 0; s < 3; s++)
+#pragma unroll
+    for (uint32_t e = 0; e < 2; e++) {
+      uint32_t w = 0;
+      if (s < ns) {
+        const uint32_t pb = a.pbit == 1 ? bits_at(lo, hi, pbpos + s, 1)
+                                        : (a.pbit == 2 ? bits_at(lo, hi, pbpos + 2 * s + e, 1) : 0u);
+#pragma unroll
+        for (uint32_t ch = 0; ch < 3; ch++) {
+          uint32_t v = (bits_at(lo, hi, pos + ((ch * ns + s) * 2 + e) * cp, cp) << (8 - cp)) & 0xFFu;
+          if (a.pbit) v |= (pb << (8 - cpp)) & 0xFFu;
+          v |= v >> cpp;
+          w |= v << (8 * ch);
+        }
+        uint32_t al = 0xFFu;
+        if (ap) {
+          al = (bits_at(lo, hi, apos + (s * 2 + e) * ap, ap) << (8 - ap)) & 0xFFu;
+          if (a.pbit) al |= (pb << (8 - app)) & 0xFFu;
+          al |= al >> app;
+        }
+        w |= al << 24;
+      }
+      ep[s][e] = w;
     }
-    uint32_t ep[3][2][4];
-    int cp = a.cprec, ap = a.aprec;
+  // index streams: colour (anchors carry one bit less), then alpha (modes 4, 5)
+  const uint32_t ib = a.idx_bits, aib = a.aidx_bits;
+  const uint32_t cstart = pbpos + npb;
+  const uint32_t astart = cstart + 16 * ib - ns;
+  uint32_t m1 = 0, m2 = 0, a1 = 16, a2 = 16;
+  if (ns == 2) {
+    m1 = kS2[shape];
+    a1 = kA2[shape];
+  } else if (ns == 3) {
+    m1 = kS3[shape][0];
+    m2 = kS3[shape][1];
+    a1 = kA3[0][shape];
+    a2 = kA3[1][shape];
+  }
+#pragma unroll
+  for (uint32_t i = 0; i < 16; i++) {
+    const uint32_t skip = (i > 0 ? 1u : 0u) + (i > a1 ? 1u : 0u) + (i > a2 ? 1u : 0u);
+    const bool anchor = i == 0 || i == a1 || i == a2;
+    uint32_t ci = bits_at(lo, hi, cstart + i * ib - skip, ib - (anchor ? 1u : 0u));
+    uint32_t ai = ci;
+    if (aib) {
+      ai = bits_at(lo, hi, astart + i * aib - (i > 0 ? 1u : 0u), aib - (i == 0 ? 1u : 0u));
+      if (im) {
+        const uint32_t t = ai;
+        ai = ci;
+        ci = t;
+      }
+    }
+    // subset: partition masks (3 subsets: second mask picks subset 2)
+    const uint32_t sub = ((m1 >> i) & 1u) ? 1u + ((m2 >> i) & 1u) : 0u;
+    const uint32_t e0 = sub == 0 ? ep[0][0] : (sub == 1 ? ep[1][0] : ep[2][0]);
+    const uint32_t e1 = sub == 0 ? ep[0][1] : (sub == 1 ? ep[1][1] : ep[2][1]);
+    const uint32_t wc = weight1(ci, ib);
+    const bool cvalid = ci < (1u << ib);
+    const uint32_t na = aib ? aib : ib;
+    const uint32_t wa = weight1(ai, na);
+    const bool avalid = ai < (1u << na);
+    uint32_t pixel = 0;
+#pragma unroll
     for (int ch = 0; ch < 3; ch++)
-      for (int s = 0; s < ns; s++)
-        for (int e = 0; e < 2; e++) ep[s][e][ch] = (r.get(cp) << (8 - cp)) & 0xFFu;
-    for (int s = 0; s < ns; s++)
-      for (int e = 0; e < 2; e++) ep[s][e][3] = ap == 0 ? 0xFFu : (r.get(ap) << (8 - ap)) & 0xFFu;
-    if (a.pbit) {
-      cp++;
-      ap++;
-      for (int s = 0; s < ns; s++) {
-        uint32_t pb = a.pbit == 1 ? r.get(1) : 0;
-        for (int e = 0; e < 2; e++) {
-          if (a.pbit == 2) pb = r.get(1);
-          for (int ch = 0; ch < 4; ch++)
-            ep[s][e][ch] = (ep[s][e][ch] | (pb << (8 - (ch == 3 ? ap : cp)))) & 0xFFu;
-        }
-      }
+      pixel |= lerp_chan((e0 >> (8 * ch)) & 0xFFu, (e1 >> (8 * ch)) & 0xFFu, wc, cvalid)
+               << (8 * ch);
+    pixel |= lerp_chan(e0 >> 24, e1 >> 24, wa, avalid) << 24;
+    if (rot >= 1) {
+      const uint32_t sh = 8 * (rot - 1);
+      const uint32_t c = (pixel >> sh) & 0xFFu, al = pixel >> 24;
+      pixel = (pixel & ~(0xFFu << sh) & 0x00FFFFFFu) | (al << sh) | (c << 24);
     }
-    for (int s = 0; s < ns; s++)
-      for (int e = 0; e < 2; e++)
-        for (int ch = 0; ch < 4; ch++) ep[s][e][ch] |= ep[s][e][ch] >> (ch == 3 ? ap : cp);
-    uint32_t cidx[16], aidx[16];
-    for (int i = 0; i < 16; i++) {
-      const int s = subset_of(i, shape, ns);
-      cidx[i] = r.get(anchor_of(s, shape, ns) == i ? a.idx_bits - 1 : a.idx_bits);
-    }
-    if (a.aidx_bits == 0) {
-      for (int i = 0; i < 16; i++) aidx[i] = cidx[i];
-    } else {
-      for (int i = 0; i < 16; i++) aidx[i] = r.get(i == 0 ? a.aidx_bits - 1 : a.aidx_bits);
-      if (im)
-        for (int i = 0; i < 16; i++) {
-          const uint32_t t = aidx[i];
-          aidx[i] = cidx[i];
-          cidx[i] = t;
-        }
-    }
-    const int nba = a.aidx_bits, nbc = a.idx_bits;
-    for (int i = 0; i < 16; i++) {
-      const int s = subset_of(i, shape, ns);
-      uint32_t pixel = 0;
-      for (int ch = 0; ch < 4; ch++) {
-        uint32_t w0, w1;
-        if (ch == 3 && nba > 0) {
-          w0 = kW0[nba - 1][aidx[i] & 15];
-          w1 = kW1[nba - 1][aidx[i] & 15];
-        } else {
-          w0 = kW0[nbc - 1][cidx[i] & 15];
-          w1 = kW1[nbc - 1][cidx[i] & 15];
-        }
-        pixel |= (((ep[s][0][ch] * w0 + ep[s][1][ch] * w1) + 32) >> 6 & 0xFFu) << (8 * ch);
-      }
-      if (rot >= 1) {
-        const uint32_t sh = 8 * (rot - 1);
-        const uint32_t c = (pixel >> sh) & 0xFFu, al = pixel >> 24;
-        pixel = (pixel & ~(0xFFu << sh) & 0x00FFFFFFu) | (al << sh) | (c << 24);
-      }
-      px[i] = pixel;
-    }
+    px[i] = pixel;
   }
 }
 

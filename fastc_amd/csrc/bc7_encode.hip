@@ -213,8 +213,11 @@ __device__ __forceinline__ void normalize4(float a[4]) {
 // QuantizedError with endpoints already quantised to qp1/qp2
 // (RGBAEndpoints.cpp:190-310, uniform metric).  Indices are written at pixel
 // positions (4 bits each) for the pixels in smask.
-__device__ uint32_t qerr(const uint32_t px[16], uint32_t smask, int ptmode, uint32_t qp1,
-                         uint32_t qp2, int nb, uint64_t &idx) {
+// UNROLL: 16 when px is a register array (constant indices), 1 when it
+// points to LDS (the loop stays rolled).
+template <int UNROLL>
+__device__ __forceinline__ uint32_t qerr(const uint32_t px[16], uint32_t smask, int ptmode,
+                                         uint32_t qp1, uint32_t qp2, int nb, uint64_t &idx) {
   int a[4], b[4], dir[4];
   int lsq = 0;
 #pragma unroll
@@ -243,7 +246,7 @@ __device__ uint32_t qerr(const uint32_t px[16], uint32_t smask, int ptmode, uint
   const int prec = nb == 4 ? 2 : (nb == 8 ? 3 : 4);
   const float flsq = (float)lsq;
   const float kf = (float)(nb - 1);
-#pragma unroll
+#pragma unroll UNROLL
   for (int i = 0; i < 16; i++) {
     if (!((smask >> i) & 1)) continue;
     const uint32_t pt = point_of(px[i], ptmode);
@@ -322,7 +325,7 @@ __device__ __forceinline__ uint32_t qmask_of(int color_prec, int alpha_prec) {
 
 // CompressSingleColor (Compressor.cpp:252-353) through the precomputed
 // per-value endpoint table.  Returns the per-pixel error.
-__device__ uint32_t single_color(const Fit &f, const uint32_t *sctab, uint32_t pixel,
+__device__ __forceinline__ uint32_t single_color(const Fit &f, const uint32_t *sctab, uint32_t pixel,
                                  uint32_t &p1, uint32_t &p2, int &pbc) {
   uint32_t best = 0xFFFFFFFFu;
   const int bpi = f.nbits_idx - 1;
@@ -361,7 +364,7 @@ __device__ uint32_t single_color(const Fit &f, const uint32_t *sctab, uint32_t p
 
 // MatrixSquare<float,4>::PowerMethod without the eigenvalue
 // (Base/include/FasTC/MatrixSquare.h:44-105).
-__device__ void power_method(const float m[16], float b[4]) {
+__device__ __forceinline__ void power_method(const float m[16], float b[4]) {
   const float norm = 1.0f / sqrtf(4.0f);
 #pragma unroll
   for (int i = 0; i < 4; i++) b[i] = norm;
@@ -399,19 +402,19 @@ __device__ __forceinline__ void unpack4f(uint32_t p, float v[4]) {
 // GetPrincipalAxis with no eigenvalues (RGBAEndpoints.cpp:327-428),
 // including the collinearity test's reads of default (-1,-1,-1,-1) entries
 // past the last unique point (:364-371).
-__device__ void principal_axis(const uint32_t px[16], uint32_t smask, int ptmode, int n,
+__device__ __forceinline__ void principal_axis(const uint32_t px[16], uint32_t smask, int ptmode, int n,
                                const float avg[4], float axis[4]) {
   // unique points in first-occurrence order; test u2.. as they appear
   int nu = 0;
   float u0[4], dir[4];
   bool collinear = true;
   uint32_t u0p = 0;
-#pragma unroll
+#pragma unroll 1
   for (int i = 0; i < 16; i++) {
     if (!((smask >> i) & 1)) continue;
     const uint32_t pi = point_of(px[i], ptmode);
     bool seen = false;
-#pragma unroll
+#pragma unroll 1
     for (int j = 0; j < i; j++)
       if (((smask >> j) & 1) && point_of(px[j], ptmode) == pi) seen = true;
     if (seen) continue;
@@ -451,23 +454,34 @@ __device__ void principal_axis(const uint32_t px[16], uint32_t smask, int ptmode
     for (int k = 0; k < 4; k++) axis[k] = dir[k];
     return;
   }
+  // each entry sums its products in point order
+  float sum[10];
+#pragma unroll
+  for (int e = 0; e < 10; e++) sum[e] = 0.0f;
+#pragma unroll 1
+  for (int p = 0; p < 16; p++) {
+    if (!((smask >> p) & 1)) continue;
+    const uint32_t pt = point_of(px[p], ptmode);
+    float t[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) t[k] = (float)chan(pt, k) - avg[k];
+    int e = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j <= i; j++) sum[e++] += t[i] * t[j];
+  }
   float cov[16];
+  {
+    int e = 0;
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 4; i++)
 #pragma unroll
-    for (int j = 0; j <= i; j++) {
-      float sum = 0.0f;
-#pragma unroll
-      for (int p = 0; p < 16; p++) {
-        if (!((smask >> p) & 1)) continue;
-        const uint32_t pt = point_of(px[p], ptmode);
-        const float ti = (float)chan(pt, i) - avg[i];
-        const float tj = (float)chan(pt, j) - avg[j];
-        sum += ti * tj;
+      for (int j = 0; j <= i; j++) {
+        cov[i * 4 + j] = sum[e++] / 3.0f;
+        cov[j * 4 + i] = cov[i * 4 + j];
       }
-      cov[i * 4 + j] = sum / 3.0f;
-      cov[j * 4 + i] = cov[i * 4 + j];
-    }
+  }
   power_method(cov, axis);
 }
 
@@ -577,14 +591,33 @@ struct SubStart {
   bool anneal;
 };
 
+// Index of the centroid nearest to v (squared distance, first minimum).
 template <int NB>
-__device__ void kmeans_setup(const Fit &f, uint32_t q, const uint32_t *sctab,
+__device__ __forceinline__ int nearest_centroid(const float c[NB][4], const float v[4]) {
+  int minb = -1;
+  float mind = FLT_MAX;
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    float d[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) d[k] = v[k] - c[j][k];
+    const float ds = dot4(d, d);
+    if (ds < mind) {
+      mind = ds;
+      minb = j;
+    }
+  }
+  return minb;
+}
+
+template <int NB>
+__device__ __forceinline__ void kmeans_setup(const Fit &f, uint32_t q, const uint32_t *sctab,
                              const uint32_t px[16], uint32_t smask, int ptmode, int n,
                              const float avg[4], SubStart &out) {
   float axis[4];
   principal_axis(px, smask, ptmode, n, avg, axis);
   float mindp = FLT_MAX, maxdp = -FLT_MAX;
-#pragma unroll
+#pragma unroll 1
   for (int i = 0; i < 16; i++) {
     if (!((smask >> i) & 1)) continue;
     float v[4];
@@ -612,56 +645,80 @@ __device__ void kmeans_setup(const Fit &f, uint32_t q, const uint32_t *sctab,
 #pragma unroll
     for (int k = 0; k < 4; k++) c[i][k] = p1[k] * (1.0f - s) + p2[k] * s;
   }
-  uint32_t bidx[16];
-#pragma unroll
-  for (int i = 0; i < 16; i++) bidx[i] = 0;
   uint32_t cnt[NB];
   bool fixed = false;
   int guard = 0;
   while (!fixed && guard++ < 4096) {
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-      if (!((smask >> i) & 1)) continue;
-      float v[4];
-      unpack4f(point_of(px[i], ptmode), v);
-      int minb = -1;
-      float mind = FLT_MAX;
+    if constexpr (NB <= 8) {
+      float sum[NB][4];
 #pragma unroll
       for (int j = 0; j < NB; j++) {
-        float d[4];
+        cnt[j] = 0;
 #pragma unroll
-        for (int k = 0; k < 4; k++) d[k] = v[k] - c[j][k];
-        const float ds = dot4(d, d);
-        if (ds < mind) {
-          mind = ds;
-          minb = j;
-        }
+        for (int k = 0; k < 4; k++) sum[j][k] = 0.0f;
       }
-      bidx[i] = (uint32_t)minb;
-    }
-    fixed = true;
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-      uint32_t cn = 0;
-      float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
+      // assign every point to its nearest centroid (first minimum), then
+      // accumulate the buckets in point order
+#pragma unroll 1
       for (int i = 0; i < 16; i++) {
-        if (((smask >> i) & 1) && bidx[i] == (uint32_t)j) {
-          cn++;
+        if (!((smask >> i) & 1)) continue;
+        float v[4];
+        unpack4f(point_of(px[i], ptmode), v);
+        const int minb = nearest_centroid<NB>(c, v);
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+          if (minb == j) {
+            cnt[j]++;
+#pragma unroll
+            for (int k = 0; k < 4; k++) sum[j][k] += v[k];
+          }
+      }
+      fixed = true;
+#pragma unroll
+      for (int j = 0; j < NB; j++) {
+        float s[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) s[k] = cnt[j] != 0 ? sum[j][k] / (float)cnt[j] : 0.0f;
+        if (s[0] != c[j][0] || s[1] != c[j][1] || s[2] != c[j][2] || s[3] != c[j][3])
+          fixed = false;
+#pragma unroll
+        for (int k = 0; k < 4; k++) c[j][k] = s[k];
+      }
+    } else {
+      // 16 buckets: assignments first (4 bits per point), then one bucket
+      // at a time, so that no second set of 16 sums is live
+      uint64_t bucket = 0;
+#pragma unroll 1
+      for (int i = 0; i < 16; i++) {
+        if (!((smask >> i) & 1)) continue;
+        float v[4];
+        unpack4f(point_of(px[i], ptmode), v);
+        bucket |= (uint64_t)nearest_centroid<NB>(c, v) << (4 * i);
+      }
+      fixed = true;
+#pragma unroll
+      for (int j = 0; j < NB; j++) {
+        uint32_t cn = 0;
+        float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 1
+        for (int i = 0; i < 16; i++) {
+          if (!((smask >> i) & 1) || ((bucket >> (4 * i)) & 15) != (uint64_t)j) continue;
           float v[4];
           unpack4f(point_of(px[i], ptmode), v);
+          cn++;
 #pragma unroll
           for (int k = 0; k < 4; k++) s[k] += v[k];
         }
-      }
-      if (cn != 0) {
+        if (cn != 0) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) s[k] = s[k] / (float)cn;
-      }
-      if (s[0] != c[j][0] || s[1] != c[j][1] || s[2] != c[j][2] || s[3] != c[j][3]) fixed = false;
+          for (int k = 0; k < 4; k++) s[k] = s[k] / (float)cn;
+        }
+        if (s[0] != c[j][0] || s[1] != c[j][1] || s[2] != c[j][2] || s[3] != c[j][3])
+          fixed = false;
 #pragma unroll
-      for (int k = 0; k < 4; k++) c[j][k] = s[k];
-      cnt[j] = cn;
+        for (int k = 0; k < 4; k++) c[j][k] = s[k];
+        cnt[j] = cn;
+      }
     }
   }
   int nfilled = 0, last = 0;
@@ -753,7 +810,7 @@ __device__ void kmeans_setup(const Fit &f, uint32_t q, const uint32_t *sctab,
   const int nb = 1 << f.nbits_idx;
   const uint32_t q1 = to_pixel_int(g1, f.qmask, pbc_bit(f.pbit_type, pbc, 0));
   const uint32_t q2 = to_pixel_int(g2, f.qmask, pbc_bit(f.pbit_type, pbc, 1));
-  out.err = qerr(px, smask, ptmode, q1, q2, nb, out.idx);
+  out.err = qerr<1>(px, smask, ptmode, q1, q2, nb, out.idx);
   out.pbc = pbc;
   if (f.pbit_type != PB_NONE) {
     out.p1 = q1;
@@ -793,7 +850,7 @@ __device__ __forceinline__ void subset_setup(const Fit &f, uint32_t q, const uin
 // k-means and least-squares fit (or the table lookup for a constant alpha).
 // Returns the alpha error; a1q/a2q are the alpha bytes Pack will write
 // (rounded, then quantised to the mode's alpha precision).
-__device__ uint32_t alpha_fit(const Fit &f, const uint32_t px[16], uint64_t &aidx, uint32_t &a1q,
+__device__ __forceinline__ uint32_t alpha_fit(const Fit &f, const uint32_t px[16], uint64_t &aidx, uint32_t &a1q,
                               uint32_t &a2q) {
   float avals[16];
   float amin = FLT_MAX, amax = -FLT_MAX;
@@ -1137,7 +1194,7 @@ __device__ uint32_t box_select_block(const uint32_t px[16], int lane, const Bc7P
         // EstimateTwo/ThreeClusterError (Compressor.cpp:1626-1655)
         if (mn != mx) {
           uint64_t dummy;
-          err += 0.0001 + (double)qerr(px, m, 0, mn, mx, ns == 2 ? 8 : 4, dummy);
+          err += 0.0001 + (double)qerr<16>(px, m, 0, mn, mx, ns == 2 ? 8 : 4, dummy);
         } else {
           err += 0.0;
         }
@@ -1382,7 +1439,11 @@ __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t 
   Fit f;
   const ModeAttr attr = MODE >= 0 ? kModeTab[MODE < 0 ? 0 : MODE] : kModes[k.mode];
   const int nsub = attr.subsets;
-  set_fit_attr(f, attr, MODE >= 0 ? MODE : k.mode, k.rot, k.im);
+  // a specialised mode without rotation or index-mode bits fixes both at
+  // zero, so only its own index width's k-means is compiled
+  const int rot = (MODE >= 0 && !attr.has_rot) ? 0 : k.rot;
+  const int im = (MODE >= 0 && MODE != 4) ? 0 : k.im;
+  set_fit_attr(f, attr, MODE >= 0 ? MODE : k.mode, rot, im);
   rec.err = 0;
   rec.px1[0] = rec.px1[1] = rec.px1[2] = 0;
   rec.px2[0] = rec.px2[1] = rec.px2[2] = 0;
@@ -1405,7 +1466,7 @@ __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t 
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) avg[kk] = avg[kk] / 16.0f;
     SubStart s0;
-    subset_setup(f, quality, sctab, px, 0xFFFFu, k.rot + 1, 16, avg, false, s0);
+    subset_setup(f, quality, sctab, px, 0xFFFFu, rot + 1, 16, avg, false, s0);
     rec.aidx_lo = (uint32_t)aidx;
     rec.aidx_hi = (uint32_t)(aidx >> 32);
     // Pack writes the RGB of the fit and the fitted alpha, no p-bit
@@ -1419,6 +1480,7 @@ __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t 
     total = s0.err + aerr;
     active = s0.anneal ? 1u : 0u;
   } else {
+#pragma unroll 1
     for (int s = 0; s < nsub; s++) {
       const uint32_t smask = subset_mask(k.shape, nsub, s);
       const int n = __popc(smask);
@@ -1463,7 +1525,7 @@ __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t 
   }
   rec.cidx_lo = (uint32_t)cidx;
   rec.cidx_hi = (uint32_t)(cidx >> 32);
-  rec.meta = pbcs | ((uint32_t)k.rot << 8) | ((uint32_t)k.im << 10);
+  rec.meta = pbcs | ((uint32_t)rot << 8) | ((uint32_t)im << 10);
   rec.err = total;  // the quality-0 result; annealing overwrites active ones
   st.pbc = pbcs;
   st.idx_lo = (uint32_t)cidx;
@@ -1473,10 +1535,13 @@ __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t 
 
 // One launch per mode (its type queue), specialised at compile time so that
 // each mode gets its own register allocation.
+constexpr int kPxStride = 17;  // odd stride: lanes reading texel i hit distinct banks
+
 template <int MODE>
 __global__ void __launch_bounds__(64) setup_kernel(BlockRange r, Bc7Params prm, Work w,
                                                    const uint32_t *sctab) {
   const int lane = threadIdx.x;
+  __shared__ uint32_t s_px[64 * kPxStride];
   {
     constexpr int kTypeOf[8] = {0, 2, 1, 3, 6, 7, 5, 4};
     const int t = kTypeOf[MODE];
@@ -1493,8 +1558,16 @@ __global__ void __launch_bounds__(64) setup_kernel(BlockRange r, Bc7Params prm, 
       const int c = id & 15;
       Cand k;
       cand_from_desc(w.cdesc[(size_t)rel * kMaxCands + c], k);
-      uint32_t px[16];
-      load_block(r, rel, px);
+      // the lane's texels live in LDS: the fit reads them many times, and
+      // keeping them (and values derived from them) out of registers lets
+      // the loops below stay rolled
+      {
+        uint32_t v[16];
+        load_block(r, rel, v);
+#pragma unroll
+        for (int i = 0; i < 16; i++) s_px[lane * kPxStride + i] = v[i];
+      }
+      const uint32_t *px = s_px + lane * kPxStride;
       CandRec rec;
       AState st;
       uint32_t total = 0;
@@ -1593,120 +1666,118 @@ __host__ __device__ __forceinline__ WeightFn weight_fn(int nb) {
   return {7u, 4370u};
 }
 __host__ __device__ __forceinline__ uint32_t weight_of(WeightFn fn, uint32_t j) {
-  return ((64u * j + fn.h) * fn.m) >> 16;
+  // j < 16: with the mask the compiler sees both factors below 2^24 and
+  // uses the full-rate 24-bit multiply
+  return ((64u * (j & 15u) + fn.h) * fn.m) >> 16;
 }
 
-// Per-subset constants of one annealing step.
+// Per-subset constants of one annealing step.  The interpolated value is
+// (e1 * (64 - w) + e2 * w + 32) >> 6 = e1 + ((e2 - e1) * w + 32) >> 6 (the
+// 64 * e1 term is a multiple of 64), so endpoint 1 and the difference suffice.
 struct SubConst {
   uint32_t a02, a13;  // endpoint 1, channel pairs
-  uint32_t b02, b13;  // endpoint 2, channel pairs
   uint32_t d02, d13;  // endpoint 2 - endpoint 1 (signed pairs)
-  float fl;           // squared endpoint distance
-  bool zero;          // endpoints equal
+  float fl;           // squared endpoint distance (0: endpoints equal)
 };
 
 __device__ __forceinline__ SubConst sub_const(uint32_t e1, uint32_t e2) {
   SubConst c;
   c.a02 = e1 & 0x00FF00FFu;
   c.a13 = (e1 >> 8) & 0x00FF00FFu;
-  c.b02 = e2 & 0x00FF00FFu;
-  c.b13 = (e2 >> 8) & 0x00FF00FFu;
-  c.d02 = __builtin_bit_cast(uint32_t, as_s16x2(c.b02) - as_s16x2(c.a02));
-  c.d13 = __builtin_bit_cast(uint32_t, as_s16x2(c.b13) - as_s16x2(c.a13));
+  const uint32_t b02 = e2 & 0x00FF00FFu, b13 = (e2 >> 8) & 0x00FF00FFu;
+  c.d02 = __builtin_bit_cast(uint32_t, as_s16x2(b02) - as_s16x2(c.a02));
+  c.d13 = __builtin_bit_cast(uint32_t, as_s16x2(b13) - as_s16x2(c.a13));
   const int l = __builtin_amdgcn_sdot2(as_s16x2(c.d02), as_s16x2(c.d02),
                                        __builtin_amdgcn_sdot2(as_s16x2(c.d13),
                                                               as_s16x2(c.d13), 0, false),
                                        false);
   c.fl = (float)l;
-  c.zero = l == 0;
   return c;
 }
 
-// Squared error of the pixel pairs against index weight w1 (w0 = 64 - w1).
+// Squared error of the pixel pairs against index weight w1: the products
+// (e2 - e1) * w1 lie within +-255 * 64 and the interpolants within 0..255,
+// so signed 16-bit lanes hold every intermediate exactly.
 __device__ __forceinline__ uint32_t interp_err(uint32_t p02, uint32_t p13, const SubConst &c,
                                                uint32_t w1) {
-  const u16x2 W1 = {(unsigned short)w1, (unsigned short)w1};
-  const u16x2 W0 = {(unsigned short)(64u - w1), (unsigned short)(64u - w1)};
-  const u16x2 K = {32, 32}, S = {6, 6};
-  const u16x2 i02 = (as_u16x2(c.a02) * W0 + as_u16x2(c.b02) * W1 + K) >> S;
-  const u16x2 i13 = (as_u16x2(c.a13) * W0 + as_u16x2(c.b13) * W1 + K) >> S;
-  const s16x2 e02 = as_s16x2(p02) - as_s16x2(i02);
-  const s16x2 e13 = as_s16x2(p13) - as_s16x2(i13);
+  const s16x2 W = {(short)w1, (short)w1};
+  const s16x2 K = {32, 32};
+  const s16x2 S = {6, 6};
+  const s16x2 i02 = as_s16x2(c.a02) + ((as_s16x2(c.d02) * W + K) >> S);
+  const s16x2 i13 = as_s16x2(c.a13) + ((as_s16x2(c.d13) * W + K) >> S);
+  const s16x2 e02 = as_s16x2(p02) - i02;
+  const s16x2 e13 = as_s16x2(p13) - i13;
   return (uint32_t)__builtin_amdgcn_sdot2(e02, e02, __builtin_amdgcn_sdot2(e13, e13, 0, false),
                                           false);
 }
 
 // QuantizedError's per-pixel index search (RGBAEndpoints.cpp:240-300) for
-// pixel pix (projected as pt): returns the error, index in *best.
+// pixel pix (projected as pt), without branches: both neighbouring indices
+// are scored and the second kept only when it exists and is strictly better;
+// equal endpoints give index 0.  Returns the error, the index in *best.
 __device__ __forceinline__ uint32_t pixel_err(uint32_t pix, uint32_t pt, const SubConst &c,
                                               int nb, float kf, WeightFn wf, int *best) {
   const uint32_t p02 = pix & 0x00FF00FFu, p13 = (pix >> 8) & 0x00FF00FFu;
-  if (c.zero) {
-    *best = 0;
-    return interp_err(p02, p13, c, 0);
-  }
   const uint32_t t02 = pt & 0x00FF00FFu, t13 = (pt >> 8) & 0x00FF00FFu;
   const int d = __builtin_amdgcn_sdot2(
       as_s16x2(t02) - as_s16x2(c.a02), as_s16x2(c.d02),
       __builtin_amdgcn_sdot2(as_s16x2(t13) - as_s16x2(c.a13), as_s16x2(c.d13), 0, false),
       false);
-  const float pct = (float)d / c.fl;
+  const bool zero = c.fl == 0.0f;
+  const float pct = (float)d / (zero ? 1.0f : c.fl);
   const float t = pct * kf;
   int j1 = (int)floorf(t);
   int j2 = (int)ceilf(t);
   j1 = j1 < 0 ? 0 : j1;
   j1 = j1 > nb - 1 ? nb - 1 : j1;
   j2 = j2 > nb - 1 ? nb - 1 : j2;
+  j1 = zero ? 0 : j1;
+  j2 = zero ? 0 : j2;
   const uint32_t e1 = interp_err(p02, p13, c, weight_of(wf, (uint32_t)j1));
-  *best = j1;
-  if (j2 > j1) {
-    const uint32_t e2 = interp_err(p02, p13, c, weight_of(wf, (uint32_t)j2));
-    if (e2 < e1) {
-      *best = j2;
-      return e2;
-    }
-  }
-  return e1;
+  const uint32_t e2 = interp_err(p02, p13, c, weight_of(wf, (uint32_t)j2));
+  const bool second = j2 > j1 && e2 < e1;
+  *best = second ? j2 : j1;
+  return second ? e2 : e1;
 }
 
-// The 16-pixel pass for NSUB subsets: errors per subset and the indices.
 __device__ __forceinline__ SubConst pick_sub(bool c, const SubConst &x, const SubConst &y) {
   SubConst r;
   r.a02 = c ? x.a02 : y.a02;
   r.a13 = c ? x.a13 : y.a13;
-  r.b02 = c ? x.b02 : y.b02;
-  r.b13 = c ? x.b13 : y.b13;
   r.d02 = c ? x.d02 : y.d02;
   r.d13 = c ? x.d13 : y.d13;
   r.fl = c ? x.fl : y.fl;
-  r.zero = c ? x.zero : y.zero;
   return r;
 }
 
+// The 16-pixel pass for up to NSUB subsets: errors per subset and the
+// indices.  Every pixel is scored; errors of subsets that have stopped
+// annealing are not used.
 template <int NSUB>
 __device__ __forceinline__ void score_pixels(const Lane &L, const SubConst &sc0,
                                              const SubConst &sc1, const SubConst &sc2, int nb,
                                              float kf, WeightFn wf, uint32_t nerr[3],
                                              uint64_t &nidx) {
+  // the RGB fit of a rotation mode sees channel rot-1 swapped with A and A = 255
+  const uint32_t pt_or = L.ptmode ? 0xFF000000u : 0u;
+  const uint32_t sh = L.ptmode >= 2 ? 8u * (uint32_t)(L.ptmode - 2) : 0u;
+  const uint32_t swap = L.ptmode >= 2 ? 0xFFu << sh : 0u;
 #pragma unroll
   for (int i = 0; i < 16; i++) {
-    const int s = NSUB == 1 ? 0 : (int)((L.sub2 >> (2 * i)) & 3);
-    if (!((L.active >> s) & 1)) continue;
+    const uint32_t s = NSUB == 1 ? 0u : (L.sub2 >> (2 * i)) & 3u;
     const SubConst c = NSUB == 1 ? sc0
                                  : (NSUB == 2 ? pick_sub(s != 0, sc1, sc0)
                                               : pick_sub(s == 0, sc0, pick_sub(s == 1, sc1, sc2)));
     const uint32_t pix = L.px[i];
+    const uint32_t pt = ((pix | pt_or) & ~swap) | (((pix >> 24) << sh) & swap);
     int best;
-    const uint32_t me = pixel_err(pix, point_of(pix, L.ptmode), c, nb, kf, wf, &best);
+    const uint32_t me = pixel_err(pix, pt, c, nb, kf, wf, &best);
     if (NSUB == 1) {
       nerr[0] += me;
-    } else if (NSUB == 2) {
-      if (s) nerr[1] += me;
-      else nerr[0] += me;
     } else {
-      if (s == 0) nerr[0] += me;
-      else if (s == 1) nerr[1] += me;
-      else nerr[2] += me;
+      nerr[0] += s == 0 ? me : 0u;
+      nerr[1] += s == 1 ? me : 0u;
+      if (NSUB == 3) nerr[2] += s == 2 ? me : 0u;
     }
     nidx |= (uint64_t)best << (4 * i);
   }
@@ -1929,7 +2000,7 @@ __device__ __forceinline__ uint32_t invert_idx(uint32_t &lo, uint32_t &hi, uint3
 // CompressionMode::Pack (Compressor.cpp:1096-1298).  The p-bits are written
 // unswapped after an anchor fix-up swap, as the reference does
 // (:1149-1170 vs :1234-1241).
-__device__ void pack_mode(int mode, int shape, const CandRec &rec, BitWriter &w) {
+__device__ __forceinline__ void pack_mode(int mode, int shape, const CandRec &rec, BitWriter &w) {
   const ModeAttr a = kModes[mode];
   const int ns = a.subsets;
   const int rot = (rec.meta >> 8) & 3;
@@ -1944,7 +2015,10 @@ __device__ void pack_mode(int mode, int shape, const CandRec &rec, BitWriter &w)
   uint32_t al = rec.aidx_lo, ah = rec.aidx_hi;
   const int nib = (im == 0) ? a.idx_bits : a.alpha_idx_bits;
   const int nab = (im == 0) ? a.alpha_idx_bits : a.idx_bits;
-  for (int s = 0; s < ns; s++) {
+  // subset loops run to 3 with a guard, so px1/px2 are indexed by constants
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+    if (s >= ns) continue;
     const uint32_t smask = subset_mask(shape, ns, s);
     const int anchor = anchor_of(s, shape, ns);
     if (idx_at(cl, ch, anchor) >> (nib - 1)) {
@@ -1962,19 +2036,26 @@ __device__ void pack_mode(int mode, int shape, const CandRec &rec, BitWriter &w)
     }
   }
   const int cp = a.color_prec, ap = a.alpha_prec;
+#pragma unroll
   for (int c = 0; c < 3; c++)
-    for (int s = 0; s < ns; s++) {
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      if (s >= ns) continue;
       w.put(chan(px1[s], c) >> (8 - cp), cp);
       w.put(chan(px2[s], c) >> (8 - cp), cp);
     }
   if (ap) {
-    for (int s = 0; s < ns; s++) {
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      if (s >= ns) continue;
       w.put(chan(px1[s], 3) >> (8 - ap), ap);
       w.put(chan(px2[s], 3) >> (8 - ap), ap);
     }
   }
   if (a.pbit_type != PB_NONE) {
-    for (int s = 0; s < ns; s++) {
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      if (s >= ns) continue;
       const int pbc = (rec.meta >> (2 * s)) & 3;
       w.put((uint32_t)pbc_bit(a.pbit_type, pbc, 0), 1);
       if (a.pbit_type != PB_SHARED) w.put((uint32_t)pbc_bit(a.pbit_type, pbc, 1), 1);
@@ -2186,6 +2267,7 @@ __global__ void __launch_bounds__(256) label_fill_kernel(unsigned long long *bes
 __global__ void __launch_bounds__(64) label_kernel(BlockRange r, int mode, uint32_t nslots,
                                                    const uint32_t *sctab,
                                                    unsigned long long *best) {
+  __shared__ uint32_t s_px[64 * kPxStride];
   const uint64_t items = (uint64_t)r.num_blocks * nslots;
   for (uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; item < items;
        item += (uint64_t)gridDim.x * blockDim.x) {
@@ -2193,8 +2275,13 @@ __global__ void __launch_bounds__(64) label_kernel(BlockRange r, int mode, uint3
     const int slot = (int)(item % nslots);
     Cand k;
     slot_cand(mode, slot, k);
-    uint32_t px[16];
-    load_block(r, rel, px);
+    uint32_t *px = s_px + threadIdx.x * kPxStride;  // this lane's row only
+    {
+      uint32_t v[16];
+      load_block(r, rel, v);
+#pragma unroll
+      for (int i = 0; i < 16; i++) px[i] = v[i];
+    }
     CandRec rec;
     AState st;
     uint32_t total = 0;
