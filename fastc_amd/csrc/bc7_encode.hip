@@ -1148,6 +1148,179 @@ __device__ __forceinline__ void wg_enqueue(Work &w, bool valid, int type, uint32
   if (valid) w.queue[(size_t)type * w.qcap + s_base[type] + local] = id;
 }
 
+// Packed 16-bit lanes: channels 0 and 2 of a pixel are (pix & 0x00FF00FF),
+// channels 1 and 3 are ((pix >> 8) & 0x00FF00FF); every quantity of the
+// index search fits 16 bits (interpolation sums <= 255 * 64 + 32, channel
+// differences within +-255), so v_pk_* and v_dot2 give the exact integers.
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+__device__ __forceinline__ s16x2 as_s16x2(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
+__device__ __forceinline__ s16x2 as_s16x2(u16x2 v) { return __builtin_bit_cast(s16x2, v); }
+
+// Weight of the second endpoint at index j for nb = 4, 8, 16 entries:
+// (64 j + (nb - 1) / 2) / (nb - 1) by a multiply-shift, equal to
+// kInterpolationValues (Compressor.cpp:159-168); bc7_upload_tables checks
+// every entry on the host before the kernels run.
+struct WeightFn {
+  uint32_t h, m;
+};
+__host__ __device__ __forceinline__ WeightFn weight_fn(int nb) {
+  if (nb == 4) return {1u, 21846u};
+  if (nb == 8) return {3u, 9363u};
+  return {7u, 4370u};
+}
+__host__ __device__ __forceinline__ uint32_t weight_of(WeightFn fn, uint32_t j) {
+  // j < 16: with the mask the compiler sees both factors below 2^24 and
+  // uses the full-rate 24-bit multiply
+  return ((64u * (j & 15u) + fn.h) * fn.m) >> 16;
+}
+
+// Per-subset constants of one annealing step.  The interpolated value is
+// (e1 * (64 - w) + e2 * w + 32) >> 6 = e1 + ((e2 - e1) * w + 32) >> 6 (the
+// 64 * e1 term is a multiple of 64), so endpoint 1 and the difference suffice.
+struct SubConst {
+  uint32_t a02, a13;  // endpoint 1, channel pairs
+  uint32_t d02, d13;  // endpoint 2 - endpoint 1 (signed pairs)
+  float fl;           // squared endpoint distance (0: endpoints equal)
+};
+
+__device__ __forceinline__ SubConst sub_const(uint32_t e1, uint32_t e2) {
+  SubConst c;
+  c.a02 = e1 & 0x00FF00FFu;
+  c.a13 = (e1 >> 8) & 0x00FF00FFu;
+  const uint32_t b02 = e2 & 0x00FF00FFu, b13 = (e2 >> 8) & 0x00FF00FFu;
+  c.d02 = __builtin_bit_cast(uint32_t, as_s16x2(b02) - as_s16x2(c.a02));
+  c.d13 = __builtin_bit_cast(uint32_t, as_s16x2(b13) - as_s16x2(c.a13));
+  const int l = __builtin_amdgcn_sdot2(as_s16x2(c.d02), as_s16x2(c.d02),
+                                       __builtin_amdgcn_sdot2(as_s16x2(c.d13),
+                                                              as_s16x2(c.d13), 0, false),
+                                       false);
+  c.fl = (float)l;
+  return c;
+}
+
+// Squared error of the pixel pairs against index weight w1: the products
+// (e2 - e1) * w1 lie within +-255 * 64 and the interpolants within 0..255,
+// so signed 16-bit lanes hold every intermediate exactly.
+__device__ __forceinline__ uint32_t interp_err(uint32_t p02, uint32_t p13, const SubConst &c,
+                                               uint32_t w1) {
+  const s16x2 W = {(short)w1, (short)w1};
+  const s16x2 K = {32, 32};
+  const s16x2 S = {6, 6};
+  const s16x2 i02 = as_s16x2(c.a02) + ((as_s16x2(c.d02) * W + K) >> S);
+  const s16x2 i13 = as_s16x2(c.a13) + ((as_s16x2(c.d13) * W + K) >> S);
+  const s16x2 e02 = as_s16x2(p02) - i02;
+  const s16x2 e13 = as_s16x2(p13) - i13;
+  return (uint32_t)__builtin_amdgcn_sdot2(e02, e02, __builtin_amdgcn_sdot2(e13, e13, 0, false),
+                                          false);
+}
+
+// QuantizedError's per-pixel index search (RGBAEndpoints.cpp:240-300) for
+// pixel pix (projected as pt), without branches: both neighbouring indices
+// are scored and the second kept only when it exists and is strictly better;
+// equal endpoints give index 0.  Returns the error, the index in *best.
+__device__ __forceinline__ uint32_t pixel_err(uint32_t pix, uint32_t pt, const SubConst &c,
+                                              int nb, float kf, WeightFn wf, int *best) {
+  const uint32_t p02 = pix & 0x00FF00FFu, p13 = (pix >> 8) & 0x00FF00FFu;
+  const uint32_t t02 = pt & 0x00FF00FFu, t13 = (pt >> 8) & 0x00FF00FFu;
+  const int d = __builtin_amdgcn_sdot2(
+      as_s16x2(t02) - as_s16x2(c.a02), as_s16x2(c.d02),
+      __builtin_amdgcn_sdot2(as_s16x2(t13) - as_s16x2(c.a13), as_s16x2(c.d13), 0, false),
+      false);
+  const bool zero = c.fl == 0.0f;
+  const float pct = (float)d / (zero ? 1.0f : c.fl);
+  const float t = pct * kf;
+  int j1 = (int)floorf(t);
+  int j2 = (int)ceilf(t);
+  j1 = j1 < 0 ? 0 : j1;
+  j1 = j1 > nb - 1 ? nb - 1 : j1;
+  j2 = j2 > nb - 1 ? nb - 1 : j2;
+  j1 = zero ? 0 : j1;
+  j2 = zero ? 0 : j2;
+  const uint32_t e1 = interp_err(p02, p13, c, weight_of(wf, (uint32_t)j1));
+  const uint32_t e2 = interp_err(p02, p13, c, weight_of(wf, (uint32_t)j2));
+  const bool second = j2 > j1 && e2 < e1;
+  *best = second ? j2 : j1;
+  return second ? e2 : e1;
+}
+
+__device__ __forceinline__ SubConst pick_sub(bool c, const SubConst &x, const SubConst &y) {
+  SubConst r;
+  r.a02 = c ? x.a02 : y.a02;
+  r.a13 = c ? x.a13 : y.a13;
+  r.d02 = c ? x.d02 : y.d02;
+  r.d13 = c ? x.d13 : y.d13;
+  r.fl = c ? x.fl : y.fl;
+  return r;
+}
+
+// BoxSelection's estimate of one partition shape (EstimateTwoClusterError /
+// EstimateThreeClusterError, Compressor.cpp:1626-1655): the sum over subsets
+// of 0.0001 + QuantizedError of the subset's points between its bounding-box
+// corners (no quantisation, NB index entries), 0 for a one-colour subset.  Each
+// pixel is scored once, against its own subset's corners; per-subset errors
+// are exact integers, added in double in subset order as the reference does.
+template <int NS, int NB>
+__device__ __forceinline__ double shape_estimate(const uint32_t px[16], int shape) {
+  const uint32_t m1 = NS == 2 ? (uint32_t)kShape2[shape] : (uint32_t)kShape3[shape][0];
+  const uint32_t m2 = NS == 3 ? (uint32_t)kShape3[shape][1] : 0u;
+  uint32_t mn02[NS], mn13[NS], mx02[NS], mx13[NS];
+#pragma unroll
+  for (int t = 0; t < NS; t++) {
+    mn02[t] = mn13[t] = 0x00FF00FFu;
+    mx02[t] = mx13[t] = 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const uint32_t s = ((m1 >> i) & 1u) ? 1u + ((m2 >> i) & 1u) : 0u;
+    const u16x2 p02 = as_u16x2(px[i] & 0x00FF00FFu), p13 = as_u16x2((px[i] >> 8) & 0x00FF00FFu);
+#pragma unroll
+    for (int t = 0; t < NS; t++) {
+      const bool in = s == (uint32_t)t;
+      const u16x2 lo = {0xFF, 0xFF}, hi = {0, 0};
+      mn02[t] = __builtin_bit_cast(uint32_t,
+                                   __builtin_elementwise_min(as_u16x2(mn02[t]), in ? p02 : lo));
+      mn13[t] = __builtin_bit_cast(uint32_t,
+                                   __builtin_elementwise_min(as_u16x2(mn13[t]), in ? p13 : lo));
+      mx02[t] = __builtin_bit_cast(uint32_t,
+                                   __builtin_elementwise_max(as_u16x2(mx02[t]), in ? p02 : hi));
+      mx13[t] = __builtin_bit_cast(uint32_t,
+                                   __builtin_elementwise_max(as_u16x2(mx13[t]), in ? p13 : hi));
+    }
+  }
+  SubConst c[NS];
+  bool flat[NS];
+#pragma unroll
+  for (int t = 0; t < NS; t++) {
+    const uint32_t lo = mn02[t] | (mn13[t] << 8), hi = mx02[t] | (mx13[t] << 8);
+    c[t] = sub_const(lo, hi);
+    flat[t] = lo == hi;
+  }
+  const WeightFn wf = weight_fn(NB);
+  uint32_t e[NS];
+#pragma unroll
+  for (int t = 0; t < NS; t++) e[t] = 0;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const uint32_t s = ((m1 >> i) & 1u) ? 1u + ((m2 >> i) & 1u) : 0u;
+    const SubConst k = NS == 2 ? pick_sub(s != 0, c[1], c[0])
+                               : pick_sub(s == 0, c[0], pick_sub(s == 1, c[1], c[NS - 1]));
+    int best;
+    const uint32_t me = pixel_err(px[i], px[i], k, NB, (float)(NB - 1), wf, &best);
+#pragma unroll
+    for (int t = 0; t < NS; t++) e[t] += s == (uint32_t)t ? me : 0u;
+  }
+  double err = 0.0;
+#pragma unroll
+  for (int t = 0; t < NS; t++) {
+    if (!flat[t]) err += 0.0001 + (double)e[t];
+    else err += 0.0;
+  }
+  return err;
+}
+
 // ---------------------------------------------------------------------------
 // Kernel 1: classification and BoxSelection (Compressor.cpp:1670-1750,
 // 1819-1860), lane i scoring shape i; the candidates go to per-type queues.
@@ -1172,36 +1345,8 @@ __device__ uint32_t box_select_block(const uint32_t px[16], int lane, const Bc7P
   } else if (transparent) {
     sel = sel_pack(1, 0);
   } else {
-    double err2 = 0.0, err3 = 0.0;
-    for (int ns = 2; ns <= 3; ns++) {
-      if (ns == 3 && !opaque) break;
-      double err = 0.0;
-      for (int s = 0; s < ns; s++) {
-        const uint32_t m = subset_mask(lane, ns, s);
-        uint32_t mnc[4] = {255, 255, 255, 255}, mxc[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-          if (!((m >> i) & 1)) continue;
-#pragma unroll
-          for (int k = 0; k < 4; k++) {
-            const uint32_t c = chan(px[i], k);
-            mnc[k] = c < mnc[k] ? c : mnc[k];
-            mxc[k] = c > mxc[k] ? c : mxc[k];
-          }
-        }
-        const uint32_t mn = mnc[0] | (mnc[1] << 8) | (mnc[2] << 16) | (mnc[3] << 24);
-        const uint32_t mx = mxc[0] | (mxc[1] << 8) | (mxc[2] << 16) | (mxc[3] << 24);
-        // EstimateTwo/ThreeClusterError (Compressor.cpp:1626-1655)
-        if (mn != mx) {
-          uint64_t dummy;
-          err += 0.0001 + (double)qerr<16>(px, m, 0, mn, mx, ns == 2 ? 8 : 4, dummy);
-        } else {
-          err += 0.0;
-        }
-      }
-      if (ns == 2) err2 = err;
-      else err3 = err;
-    }
+    const double err2 = shape_estimate<2, 8>(px, lane);
+    const double err3 = opaque ? shape_estimate<3, 4>(px, lane) : 0.0;
     double v2 = err2, v3 = err3;
     int i2 = lane, i3 = lane;
 #pragma unroll
@@ -1642,114 +1787,6 @@ __device__ void lane_load(Lane &L, const BlockRange &r, const Bc7Params &prm, co
   L.extra = st.extra_err;
 }
 
-// Packed 16-bit lanes: channels 0 and 2 of a pixel are (pix & 0x00FF00FF),
-// channels 1 and 3 are ((pix >> 8) & 0x00FF00FF); every quantity of the
-// index search fits 16 bits (interpolation sums <= 255 * 64 + 32, channel
-// differences within +-255), so v_pk_* and v_dot2 give the exact integers.
-typedef short s16x2 __attribute__((ext_vector_type(2)));
-typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ u16x2 as_u16x2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
-__device__ __forceinline__ s16x2 as_s16x2(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
-__device__ __forceinline__ s16x2 as_s16x2(u16x2 v) { return __builtin_bit_cast(s16x2, v); }
-
-// Weight of the second endpoint at index j for nb = 4, 8, 16 entries:
-// (64 j + (nb - 1) / 2) / (nb - 1) by a multiply-shift, equal to
-// kInterpolationValues (Compressor.cpp:159-168); bc7_upload_tables checks
-// every entry on the host before the kernels run.
-struct WeightFn {
-  uint32_t h, m;
-};
-__host__ __device__ __forceinline__ WeightFn weight_fn(int nb) {
-  if (nb == 4) return {1u, 21846u};
-  if (nb == 8) return {3u, 9363u};
-  return {7u, 4370u};
-}
-__host__ __device__ __forceinline__ uint32_t weight_of(WeightFn fn, uint32_t j) {
-  // j < 16: with the mask the compiler sees both factors below 2^24 and
-  // uses the full-rate 24-bit multiply
-  return ((64u * (j & 15u) + fn.h) * fn.m) >> 16;
-}
-
-// Per-subset constants of one annealing step.  The interpolated value is
-// (e1 * (64 - w) + e2 * w + 32) >> 6 = e1 + ((e2 - e1) * w + 32) >> 6 (the
-// 64 * e1 term is a multiple of 64), so endpoint 1 and the difference suffice.
-struct SubConst {
-  uint32_t a02, a13;  // endpoint 1, channel pairs
-  uint32_t d02, d13;  // endpoint 2 - endpoint 1 (signed pairs)
-  float fl;           // squared endpoint distance (0: endpoints equal)
-};
-
-__device__ __forceinline__ SubConst sub_const(uint32_t e1, uint32_t e2) {
-  SubConst c;
-  c.a02 = e1 & 0x00FF00FFu;
-  c.a13 = (e1 >> 8) & 0x00FF00FFu;
-  const uint32_t b02 = e2 & 0x00FF00FFu, b13 = (e2 >> 8) & 0x00FF00FFu;
-  c.d02 = __builtin_bit_cast(uint32_t, as_s16x2(b02) - as_s16x2(c.a02));
-  c.d13 = __builtin_bit_cast(uint32_t, as_s16x2(b13) - as_s16x2(c.a13));
-  const int l = __builtin_amdgcn_sdot2(as_s16x2(c.d02), as_s16x2(c.d02),
-                                       __builtin_amdgcn_sdot2(as_s16x2(c.d13),
-                                                              as_s16x2(c.d13), 0, false),
-                                       false);
-  c.fl = (float)l;
-  return c;
-}
-
-// Squared error of the pixel pairs against index weight w1: the products
-// (e2 - e1) * w1 lie within +-255 * 64 and the interpolants within 0..255,
-// so signed 16-bit lanes hold every intermediate exactly.
-__device__ __forceinline__ uint32_t interp_err(uint32_t p02, uint32_t p13, const SubConst &c,
-                                               uint32_t w1) {
-  const s16x2 W = {(short)w1, (short)w1};
-  const s16x2 K = {32, 32};
-  const s16x2 S = {6, 6};
-  const s16x2 i02 = as_s16x2(c.a02) + ((as_s16x2(c.d02) * W + K) >> S);
-  const s16x2 i13 = as_s16x2(c.a13) + ((as_s16x2(c.d13) * W + K) >> S);
-  const s16x2 e02 = as_s16x2(p02) - i02;
-  const s16x2 e13 = as_s16x2(p13) - i13;
-  return (uint32_t)__builtin_amdgcn_sdot2(e02, e02, __builtin_amdgcn_sdot2(e13, e13, 0, false),
-                                          false);
-}
-
-// QuantizedError's per-pixel index search (RGBAEndpoints.cpp:240-300) for
-// pixel pix (projected as pt), without branches: both neighbouring indices
-// are scored and the second kept only when it exists and is strictly better;
-// equal endpoints give index 0.  Returns the error, the index in *best.
-__device__ __forceinline__ uint32_t pixel_err(uint32_t pix, uint32_t pt, const SubConst &c,
-                                              int nb, float kf, WeightFn wf, int *best) {
-  const uint32_t p02 = pix & 0x00FF00FFu, p13 = (pix >> 8) & 0x00FF00FFu;
-  const uint32_t t02 = pt & 0x00FF00FFu, t13 = (pt >> 8) & 0x00FF00FFu;
-  const int d = __builtin_amdgcn_sdot2(
-      as_s16x2(t02) - as_s16x2(c.a02), as_s16x2(c.d02),
-      __builtin_amdgcn_sdot2(as_s16x2(t13) - as_s16x2(c.a13), as_s16x2(c.d13), 0, false),
-      false);
-  const bool zero = c.fl == 0.0f;
-  const float pct = (float)d / (zero ? 1.0f : c.fl);
-  const float t = pct * kf;
-  int j1 = (int)floorf(t);
-  int j2 = (int)ceilf(t);
-  j1 = j1 < 0 ? 0 : j1;
-  j1 = j1 > nb - 1 ? nb - 1 : j1;
-  j2 = j2 > nb - 1 ? nb - 1 : j2;
-  j1 = zero ? 0 : j1;
-  j2 = zero ? 0 : j2;
-  const uint32_t e1 = interp_err(p02, p13, c, weight_of(wf, (uint32_t)j1));
-  const uint32_t e2 = interp_err(p02, p13, c, weight_of(wf, (uint32_t)j2));
-  const bool second = j2 > j1 && e2 < e1;
-  *best = second ? j2 : j1;
-  return second ? e2 : e1;
-}
-
-__device__ __forceinline__ SubConst pick_sub(bool c, const SubConst &x, const SubConst &y) {
-  SubConst r;
-  r.a02 = c ? x.a02 : y.a02;
-  r.a13 = c ? x.a13 : y.a13;
-  r.d02 = c ? x.d02 : y.d02;
-  r.d13 = c ? x.d13 : y.d13;
-  r.fl = c ? x.fl : y.fl;
-  return r;
-}
-
 // The 16-pixel pass for up to NSUB subsets: errors per subset and the
 // indices.  Every pixel is scored; errors of subsets that have stopped
 // annealing are not used.
@@ -1811,9 +1848,12 @@ __device__ bool lane_step(Lane &L, uint32_t q) {
   uint64_t nidx = 0;
   const float kf = (float)(nb - 1);
   const WeightFn wf = weight_fn(nb);
-  // one code path for every subset count: lanes of a wave hold candidates of
-  // different modes once they refill
-  score_pixels<3>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx);
+  // the widest subset count among the wave's running lanes picks the path
+  // (a wave-uniform branch): the list is ordered by mode, so most waves hold
+  // candidates of one subset count
+  if (__ballot(L.nsub == 3)) score_pixels<3>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx);
+  else if (__ballot(L.nsub == 2)) score_pixels<2>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx);
+  else score_pixels<1>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx);
 #pragma unroll
   for (int s = 0; s < 3; s++) {
     if (!((L.active >> s) & 1)) continue;
@@ -1829,10 +1869,21 @@ __device__ bool lane_step(Lane &L, uint32_t q) {
       const float rr = rng.frand();
       L.ctr[s] = rng.ctr;
       if (rr < 1.0f) {
-        const float temp = (float)L.energy[s] / (float)((int)q - 1);
-        const double pr =
-            exp(((double)0.1f * ((double)L.cur[s] - (double)err)) / (double)temp);
-        accept = (double)rr < pr;
+        // A single-precision estimate of the probability decides unless rr
+        // lies within 1e-4 (relative) of it, which covers the estimate's
+        // error; then the double expression of the reference decides.  The
+        // errors are integers below 2^24, so their float difference is exact.
+        const float diff = (float)L.cur[s] - (float)err;
+        const float pf =
+            __expf(__fdividef(0.1f * diff * (float)((int)q - 1), (float)L.energy[s]));
+        if (rr != 0.0f && rr < pf * 0.9999f) {
+          accept = true;
+        } else if (!(rr != 0.0f && rr > pf * 1.0001f)) {
+          const float temp = (float)L.energy[s] / (float)((int)q - 1);
+          const double pr =
+              exp(((double)0.1f * ((double)L.cur[s] - (double)err)) / (double)temp);
+          accept = (double)rr < pr;
+        }
       }
     }
     if (accept) {
@@ -1931,7 +1982,6 @@ __global__ void __launch_bounds__(64) anneal_kernel(BlockRange r, Bc7Params prm,
       wbase += min(avail, (uint32_t)__popcll(need));
     }
     if (!__ballot(L.has)) break;
-#pragma unroll
     for (int u = 0; u < UNROLL; u++) {
       if (L.has) {
         if (lane_step(L, prm.quality)) {
