@@ -1745,7 +1745,7 @@ struct Lane {
   int nsub, entry, shape;
   uint32_t sub2;       // subset of pixel i in bits 2i..2i+1
   int ptmode;
-  uint32_t px[16];
+  uint32_t *px;        // the block's 16 texels: this lane's row of LDS
   uint32_t active;
   uint32_t p1[3], p2[3], bp1[3], bp2[3], cur[3], best[3];
   int cpbc[3], bpbc[3], energy[3];
@@ -1771,7 +1771,12 @@ __device__ void lane_load(Lane &L, const BlockRange &r, const Bc7Params &prm, co
     for (int i = 0; i < 16; i++)
       if ((m >> i) & 1) L.sub2 |= (uint32_t)s << (2 * i);
   }
-  load_block(r, st.id >> 4, L.px);
+  {
+    uint32_t v[16];
+    load_block(r, st.id >> 4, v);
+#pragma unroll
+    for (int i = 0; i < 16; i++) L.px[i] = v[i];
+  }
   const uint64_t bid = prm.block_id_base + r.first_block + (st.id >> 4);
 #pragma unroll
   for (int s = 0; s < 3; s++) {
@@ -1940,12 +1945,19 @@ __device__ void lane_finish(Lane &L, Work &w) {
 }
 
 // UNROLL annealing steps run between refills of finished lanes.
-template <int UNROLL>
-__global__ void __launch_bounds__(64) anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
+// WAVES: the minimum waves per SIMD the register allocation must allow.  3
+// (168 VGPRs, a small spill) suits the long lists of the full search; the
+// learned search's short, filtered list runs faster with the uncapped
+// allocation (WAVES 1, two waves fit).
+template <int UNROLL, int WAVES>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES)))
+anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
   const int lane = threadIdx.x;
   const uint64_t lt = (1ull << lane) - 1ull;
   const uint32_t total = *(volatile uint32_t *)&w.counters[16];
+  __shared__ uint32_t s_px[64 * kPxStride];
   Lane L;
+  L.px = s_px + lane * kPxStride;
   L.has = false;
   uint32_t wbase = 0, wend = 0;  // wave-local window of the list
   bool exhausted = false;
@@ -2499,12 +2511,14 @@ hipError_t bc7_upload_tables(int device) {
 
 void launch_anneal(uint32_t grid, const BlockRange &r, const Bc7Params &p, const bc7::Work &w,
                    hipStream_t stream) {
-  if (p.anneal_unroll >= 4)
-    hipLaunchKernelGGL(bc7::anneal_kernel<4>, dim3(grid), dim3(64), 0, stream, r, p, w);
+  if (w.cerr)  // learned search
+    hipLaunchKernelGGL((bc7::anneal_kernel<1, 1>), dim3(grid), dim3(64), 0, stream, r, p, w);
+  else if (p.anneal_unroll >= 4)
+    hipLaunchKernelGGL((bc7::anneal_kernel<4, 3>), dim3(grid), dim3(64), 0, stream, r, p, w);
   else if (p.anneal_unroll == 2)
-    hipLaunchKernelGGL(bc7::anneal_kernel<2>, dim3(grid), dim3(64), 0, stream, r, p, w);
+    hipLaunchKernelGGL((bc7::anneal_kernel<2, 3>), dim3(grid), dim3(64), 0, stream, r, p, w);
   else
-    hipLaunchKernelGGL(bc7::anneal_kernel<1>, dim3(grid), dim3(64), 0, stream, r, p, w);
+    hipLaunchKernelGGL((bc7::anneal_kernel<1, 3>), dim3(grid), dim3(64), 0, stream, r, p, w);
 }
 
 hipError_t fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t stream) {
