@@ -1098,19 +1098,46 @@ struct AState {
 // annealed (annealing lowers a fit's error by a few percent at most).
 constexpr unsigned long long kAnnealMarginDiv = 20;
 
+// Learned search: the list of annealing states that pass the margin, in
+// astate order (the set-up kernels append by mode, so the list stays grouped
+// by mode and annealing waves stay uniform).  Persistent; each wave appends
+// its passing entries with one atomic, keeping their order.
+__global__ void __launch_bounds__(256) anneal_list_kernel(const AState *astate, uint32_t *counters,
+                                                          const unsigned long long *cerr,
+                                                          const unsigned long long *q0best,
+                                                          uint32_t *alist) {
+  const uint32_t total = *(volatile uint32_t *)&counters[16];
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    bool pass = false;
+    if (i < total) {
+      const uint32_t id = astate[i].id;
+      const unsigned long long best = q0best[id >> 4];
+      pass = cerr[id] <= best + best / kAnnealMarginDiv;
+    }
+    const uint64_t m = __ballot(pass);
+    uint32_t at = 0;
+    if (lane == 0 && m) at = atomicAdd(&counters[18], (uint32_t)__popcll(m));
+    at = __shfl(at, 0);
+    if (pass) alist[at + (uint32_t)__popcll(m & lt)] = i;
+  }
+}
+
 struct Work {
   uint32_t *sel;          // per block
   uint16_t *cdesc;        // per block x 16 candidate descriptors
   CandRec *rec;           // per block x 16
   uint32_t *queue;        // per-type candidate queues, 16 entries per block each
-  uint32_t *counters;     // [0..7] queue sizes, [8..15] set-up fetch, [16] anneal size, [17] anneal fetch
+  uint32_t *counters;     // [0..7] queue sizes, [8..15] set-up fetch, [16] anneal size,
+                          // [17] anneal fetch, [18] learned annealing list size
   AState *astate;
   uint32_t qcap;          // entries per type queue
   uint32_t astate_cap;
-  // learned search: per candidate slot (rel * 16 + slot) the decoded
-  // quality-0 error, per block the smallest; null in the reference search
-  const unsigned long long *cerr;
-  const unsigned long long *q0best;
+  // learned search: the annealing states to run (indices into astate, in
+  // astate order), counters[18] of them; null in the reference search
+  const uint32_t *alist;
 };
 
 __device__ __forceinline__ void load_block(const BlockRange &r, uint32_t rel, uint32_t px[16]) {
@@ -1954,7 +1981,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES))
 anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
   const int lane = threadIdx.x;
   const uint64_t lt = (1ull << lane) - 1ull;
-  const uint32_t total = *(volatile uint32_t *)&w.counters[16];
+  const uint32_t total = *(volatile uint32_t *)&w.counters[w.alist ? 18 : 16];
   __shared__ uint32_t s_px[64 * kPxStride];
   Lane L;
   L.px = s_px + lane * kPxStride;
@@ -1979,18 +2006,8 @@ anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
       }
       const uint32_t avail = wend - wbase;
       const uint32_t rank = (uint32_t)__popcll(need & lt);
-      if (!L.has && rank < avail) {
-        const AState &st = w.astate[wbase + rank];
-        // learned search: anneal only candidates whose quality-0 block
-        // decodes within kAnnealMargin of the block's best
-        bool take = true;
-        if (w.cerr) {
-          const uint32_t rel = st.id >> 4;
-          const unsigned long long best = w.q0best[rel];
-          take = w.cerr[st.id] <= best + best / kAnnealMarginDiv;
-        }
-        if (take) lane_load(L, r, prm, st);
-      }
+      if (!L.has && rank < avail)
+        lane_load(L, r, prm, w.astate[w.alist ? w.alist[wbase + rank] : wbase + rank]);
       wbase += min(avail, (uint32_t)__popcll(need));
     }
     if (!__ballot(L.has)) break;
@@ -2511,7 +2528,7 @@ hipError_t bc7_upload_tables(int device) {
 
 void launch_anneal(uint32_t grid, const BlockRange &r, const Bc7Params &p, const bc7::Work &w,
                    hipStream_t stream) {
-  if (w.cerr)  // learned search
+  if (w.alist)  // learned search
     hipLaunchKernelGGL((bc7::anneal_kernel<1, 1>), dim3(grid), dim3(64), 0, stream, r, p, w);
   else if (p.anneal_unroll >= 4)
     hipLaunchKernelGGL((bc7::anneal_kernel<4, 3>), dim3(grid), dim3(64), 0, stream, r, p, w);
@@ -2566,8 +2583,8 @@ size_t bc7_work_bytes(uint32_t num_blocks, bool learned) {
     // q0 errors, fallback list, one chunk of network activations and logits
     const size_t c = n < kBc7MlpChunk ? n : kBc7MlpChunk;
     const size_t rows = (c + 127) & ~(size_t)127;
-    bytes += n * 8 + n * bc7::kMaxCands * 8 + n * 4 + rows * (64 * 2 + 256 * 2 * 2 + 512 * 4) +
-             8 * 256;
+    bytes += n * 8 + n * bc7::kMaxCands * (8 + 4) + n * 4 +
+             rows * (64 * 2 + 256 * 2 * 2 + 512 * 4) + 9 * 256;
   }
   return bytes;
 }
@@ -2608,12 +2625,13 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   float *mlogits = nullptr;
   const uint32_t mchunk = n < kBc7MlpChunk ? n : kBc7MlpChunk;
   unsigned long long *cerr = nullptr;
-  w.cerr = nullptr;
-  w.q0best = nullptr;
+  w.alist = nullptr;
+  uint32_t *alist = nullptr;
   if (learned) {
     const size_t rows = ((size_t)mchunk + 127) & ~(size_t)127;
     q0err = reinterpret_cast<unsigned long long *>(carve((size_t)n * 8));
     cerr = reinterpret_cast<unsigned long long *>(carve(nc * 8));
+    alist = reinterpret_cast<uint32_t *>(carve(nc * 4));
     fallback = reinterpret_cast<uint32_t *>(carve((size_t)n * 4));
     mX = reinterpret_cast<uint16_t *>(carve(rows * 64 * 2));
     mh1 = reinterpret_cast<uint16_t *>(carve(rows * 256 * 2));
@@ -2679,9 +2697,10 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
                          q0err, cerr, 0);
       timing_end(t, kTagBc7Pack, stream);
       bc7::Work wa = w;
-      wa.cerr = cerr;
-      wa.q0best = q0err;
+      wa.alist = alist;
       t = timing_begin(stream);
+      hipLaunchKernelGGL(bc7::anneal_list_kernel, dim3((uint32_t)g_num_cus[dev] * 4u), dim3(256),
+                         0, stream, w.astate, w.counters, cerr, q0err, alist);
       launch_anneal(persistent, r, p, wa, stream);
       timing_end(t, kTagBc7Anneal, stream);
     }
