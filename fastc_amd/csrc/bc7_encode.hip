@@ -2580,11 +2580,10 @@ size_t bc7_work_bytes(uint32_t num_blocks, bool learned) {
                  n * bc7::kNumTypes * bc7::kMaxCands * 4 +
                  n * bc7::kMaxCands * sizeof(bc7::AState) + n * 16 + 8 * 256;
   if (learned) {
-    // q0 errors, fallback list, one chunk of network activations and logits
+    // q0 errors, candidate errors, annealing list, fallback list, one chunk of logits
     const size_t c = n < kBc7MlpChunk ? n : kBc7MlpChunk;
     const size_t rows = (c + 127) & ~(size_t)127;
-    bytes += n * 8 + n * bc7::kMaxCands * (8 + 4) + n * 4 +
-             rows * (64 * 2 + 256 * 2 * 2 + 512 * 4) + 9 * 256;
+    bytes += n * 8 + n * bc7::kMaxCands * (8 + 4) + n * 4 + rows * 512 * 4 + 9 * 256;
   }
   return bytes;
 }
@@ -2621,7 +2620,6 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   const bool learned = p.ranker != nullptr;
   unsigned long long *q0err = nullptr;
   uint32_t *fallback = nullptr;
-  uint16_t *mX = nullptr, *mh1 = nullptr, *mh2 = nullptr;
   float *mlogits = nullptr;
   const uint32_t mchunk = n < kBc7MlpChunk ? n : kBc7MlpChunk;
   unsigned long long *cerr = nullptr;
@@ -2633,9 +2631,6 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
     cerr = reinterpret_cast<unsigned long long *>(carve(nc * 8));
     alist = reinterpret_cast<uint32_t *>(carve(nc * 4));
     fallback = reinterpret_cast<uint32_t *>(carve((size_t)n * 4));
-    mX = reinterpret_cast<uint16_t *>(carve(rows * 64 * 2));
-    mh1 = reinterpret_cast<uint16_t *>(carve(rows * 256 * 2));
-    mh2 = reinterpret_cast<uint16_t *>(carve(rows * 256 * 2));
     mlogits = reinterpret_cast<float *>(carve(rows * 512 * 4));
   }
   uint32_t *fallback_count = w.counters + 20;
@@ -2661,7 +2656,7 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
       rc.first_block = r.first_block + c0;
       rc.num_blocks = cn;
       t = timing_begin(stream);
-      e = ranker_logits(*p.ranker, rc, cn, mX, mh1, mh2, mlogits, stream);
+      e = ranker_logits(*p.ranker, rc, cn, mlogits, stream);
       timing_end(t, kTagBc7Rank, stream);
       if (e != hipSuccess) return e;
       t = timing_begin(stream);

@@ -175,11 +175,11 @@ struct fh_ranker {
 
 namespace fh {
 
-// Inference with caller buffers (rows = n rounded up to 128): X rows x 64,
-// h1 and h2 rows x 256 bf16, logits rows x 512 fp32.  Reads only the
-// ranker's weights.
-hipError_t ranker_logits(const Ranker &rk, const BlockRange &r, uint32_t n, uint16_t *X,
-                         uint16_t *h1, uint16_t *h2, float *logits, hipStream_t s);
+// Fused inference of the n blocks of r into logits (n x 512 fp32), bit-identical
+// to the unfused forward pass; the hidden activations never leave the chip.
+// Reads only the ranker's weights.
+hipError_t ranker_logits(const Ranker &rk, const BlockRange &r, uint32_t n, float *logits,
+                         hipStream_t s);
 
 // Host-side table builders (tables.cpp); both mirror generator programs the
 // reference documents.
