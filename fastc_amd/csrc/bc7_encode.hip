@@ -1210,7 +1210,8 @@ __host__ __device__ __forceinline__ uint32_t weight_of(WeightFn fn, uint32_t j) 
 struct SubConst {
   uint32_t a02, a13;  // endpoint 1, channel pairs
   uint32_t d02, d13;  // endpoint 2 - endpoint 1 (signed pairs)
-  float fl;           // squared endpoint distance (0: endpoints equal)
+  int l;              // squared endpoint distance (0: endpoints equal)
+  float rl;           // approximately 1 / l
 };
 
 __device__ __forceinline__ SubConst sub_const(uint32_t e1, uint32_t e2) {
@@ -1224,7 +1225,8 @@ __device__ __forceinline__ SubConst sub_const(uint32_t e1, uint32_t e2) {
                                        __builtin_amdgcn_sdot2(as_s16x2(c.d13),
                                                               as_s16x2(c.d13), 0, false),
                                        false);
-  c.fl = (float)l;
+  c.l = l;
+  c.rl = __builtin_amdgcn_rcpf((float)(l > 0 ? l : 1));
   return c;
 }
 
@@ -1248,21 +1250,34 @@ __device__ __forceinline__ uint32_t interp_err(uint32_t p02, uint32_t p13, const
 // pixel pix (projected as pt), without branches: both neighbouring indices
 // are scored and the second kept only when it exists and is strictly better;
 // equal endpoints give index 0.  Returns the error, the index in *best.
+//
+// The reference takes j1 = floor(t), j2 = ceil(t) of t = fl(fl(d / l) * kf)
+// (float, d the integer projection, l the integer squared distance, kf =
+// nb - 1).  Those equal the floor and ceiling of the rational d kf / l: its
+// distance to an integer, when nonzero, is at least 1 / l >= 1 / (4 * 255^2)
+// = 3.8e-6, more than the float error of at most 2^-23 |t| <= 1.9e-6 for
+// |t| <= 16 (larger |t| is clamped away), and fl(fl(k / kf) kf) = k for every
+// integer 0 <= k <= kf in {3, 7, 15} (checked exhaustively).  So the quotient
+// is taken in integers: an estimate from a reciprocal, corrected by one step
+// with the exact remainder.
 __device__ __forceinline__ uint32_t pixel_err(uint32_t pix, uint32_t pt, const SubConst &c,
                                               int nb, float kf, WeightFn wf, int *best) {
+  (void)kf;
   const uint32_t p02 = pix & 0x00FF00FFu, p13 = (pix >> 8) & 0x00FF00FFu;
   const uint32_t t02 = pt & 0x00FF00FFu, t13 = (pt >> 8) & 0x00FF00FFu;
   const int d = __builtin_amdgcn_sdot2(
       as_s16x2(t02) - as_s16x2(c.a02), as_s16x2(c.d02),
       __builtin_amdgcn_sdot2(as_s16x2(t13) - as_s16x2(c.a13), as_s16x2(c.d13), 0, false),
       false);
-  const bool zero = c.fl == 0.0f;
-  const float pct = (float)d / (zero ? 1.0f : c.fl);
-  const float t = pct * kf;
-  int j1 = (int)floorf(t);
-  int j2 = (int)ceilf(t);
-  j1 = j1 < 0 ? 0 : j1;
-  j1 = j1 > nb - 1 ? nb - 1 : j1;
+  const bool zero = c.l == 0;
+  const int num = __mul24(d, nb - 1);  // |d| <= 4 * 255^2, so 24-bit operands suffice
+  int q = (int)floorf((float)num * c.rl);
+  int rem = num - __mul24(q, c.l);
+  const bool lo = rem < 0, hi = rem >= c.l;
+  q += lo ? -1 : (hi ? 1 : 0);
+  rem += lo ? c.l : (hi ? -c.l : 0);
+  int j1 = q < 0 ? 0 : (q > nb - 1 ? nb - 1 : q);
+  int j2 = q + (rem != 0 ? 1 : 0);
   j2 = j2 > nb - 1 ? nb - 1 : j2;
   j1 = zero ? 0 : j1;
   j2 = zero ? 0 : j2;
@@ -1279,7 +1294,8 @@ __device__ __forceinline__ SubConst pick_sub(bool c, const SubConst &x, const Su
   r.a13 = c ? x.a13 : y.a13;
   r.d02 = c ? x.d02 : y.d02;
   r.d13 = c ? x.d13 : y.d13;
-  r.fl = c ? x.fl : y.fl;
+  r.l = c ? x.l : y.l;
+  r.rl = c ? x.rl : y.rl;
   return r;
 }
 
@@ -1773,6 +1789,7 @@ struct Lane {
   uint32_t sub2;       // subset of pixel i in bits 2i..2i+1
   int ptmode;
   uint32_t *px;        // the block's 16 texels: this lane's row of LDS
+  uint32_t *pt;        // the points the RGB fit sees (rotation applied), same row
   uint32_t active;
   uint32_t p1[3], p2[3], bp1[3], bp2[3], cur[3], best[3];
   int cpbc[3], bpbc[3], energy[3];
@@ -1802,7 +1819,10 @@ __device__ void lane_load(Lane &L, const BlockRange &r, const Bc7Params &prm, co
     uint32_t v[16];
     load_block(r, st.id >> 4, v);
 #pragma unroll
-    for (int i = 0; i < 16; i++) L.px[i] = v[i];
+    for (int i = 0; i < 16; i++) {
+      L.px[i] = v[i];
+      L.pt[i] = point_of(v[i], L.ptmode);
+    }
   }
   const uint64_t bid = prm.block_id_base + r.first_block + (st.id >> 4);
 #pragma unroll
@@ -1827,10 +1847,6 @@ __device__ __forceinline__ void score_pixels(const Lane &L, const SubConst &sc0,
                                              const SubConst &sc1, const SubConst &sc2, int nb,
                                              float kf, WeightFn wf, uint32_t nerr[3],
                                              uint64_t &nidx) {
-  // the RGB fit of a rotation mode sees channel rot-1 swapped with A and A = 255
-  const uint32_t pt_or = L.ptmode ? 0xFF000000u : 0u;
-  const uint32_t sh = L.ptmode >= 2 ? 8u * (uint32_t)(L.ptmode - 2) : 0u;
-  const uint32_t swap = L.ptmode >= 2 ? 0xFFu << sh : 0u;
 #pragma unroll
   for (int i = 0; i < 16; i++) {
     const uint32_t s = NSUB == 1 ? 0u : (L.sub2 >> (2 * i)) & 3u;
@@ -1838,7 +1854,7 @@ __device__ __forceinline__ void score_pixels(const Lane &L, const SubConst &sc0,
                                  : (NSUB == 2 ? pick_sub(s != 0, sc1, sc0)
                                               : pick_sub(s == 0, sc0, pick_sub(s == 1, sc1, sc2)));
     const uint32_t pix = L.px[i];
-    const uint32_t pt = ((pix | pt_or) & ~swap) | (((pix >> 24) << sh) & swap);
+    const uint32_t pt = L.pt[i];
     int best;
     const uint32_t me = pixel_err(pix, pt, c, nb, kf, wf, &best);
     if (NSUB == 1) {
@@ -1982,9 +1998,11 @@ anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
   const int lane = threadIdx.x;
   const uint64_t lt = (1ull << lane) - 1ull;
   const uint32_t total = *(volatile uint32_t *)&w.counters[w.alist ? 18 : 16];
-  __shared__ uint32_t s_px[64 * kPxStride];
+  // per lane 16 texels then 16 points, odd row stride: conflict-free
+  __shared__ uint32_t s_px[64 * 33];
   Lane L;
-  L.px = s_px + lane * kPxStride;
+  L.px = s_px + lane * 33;
+  L.pt = L.px + 16;
   L.has = false;
   uint32_t wbase = 0, wend = 0;  // wave-local window of the list
   bool exhausted = false;
