@@ -2024,8 +2024,13 @@ anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
       }
       const uint32_t avail = wend - wbase;
       const uint32_t rank = (uint32_t)__popcll(need & lt);
-      if (!L.has && rank < avail)
-        lane_load(L, r, prm, w.astate[w.alist ? w.alist[wbase + rank] : wbase + rank]);
+      if (!L.has && rank < avail) {
+        // the list runs from its end: the set-up kernels append modes 0, 2,
+        // 1, 3, 7, 6, 4, 5, and the rotation modes' candidates anneal the
+        // longest, so starting with them keeps the tail of the launch short
+        const uint32_t item = total - 1u - (wbase + rank);
+        lane_load(L, r, prm, w.astate[w.alist ? w.alist[item] : item]);
+      }
       wbase += min(avail, (uint32_t)__popcll(need));
     }
     if (!__ballot(L.has)) break;
