@@ -1480,9 +1480,12 @@ __global__ void __launch_bounds__(256) select_kernel(BlockRange r, Bc7Params prm
   int type = 0;
   uint32_t rel = 0;
   if (base + blk < total && c < sel_count(s_sel[blk])) {
-    valid = true;
+    // mode 6's second shape entry has the same quality-0 fit as the first
+    // (its slot follows it): the set-up of the first writes both
+    const uint32_t d = s_cd[blk][c];
+    valid = !((d & 7) == 6 && ((d >> 12) & 1));
     rel = rel_of(base + blk);
-    type = kTypeOfMode[s_cd[blk][c] & 7];
+    type = kTypeOfMode[d & 7];
   }
   wg_enqueue(w, valid, type, (rel << 4) | (uint32_t)c, s_cnt, s_base);
 }
@@ -1762,12 +1765,25 @@ __global__ void __launch_bounds__(64) setup_kernel(BlockRange r, Bc7Params prm, 
       const uint32_t active = fit_candidate<MODE>(k, px, prm.quality, sctab, rec, st, total);
       st.id = id;
       w.rec[(size_t)rel * kMaxCands + c] = rec;
+      // mode 6, shape entry 0 followed by entry 1 (opaque blocks with two
+      // shapes): the second slot gets the same fit and its own annealing state
+      bool twin = false;
+      if (MODE == 6 && c + 1 < kMaxCands && c + 1 < sel_count(w.sel[rel])) {
+        const uint32_t d1 = w.cdesc[(size_t)rel * kMaxCands + c + 1];
+        twin = (d1 & 7) == 6 && ((d1 >> 12) & 1);
+      }
+      if (twin) w.rec[(size_t)rel * kMaxCands + c + 1] = rec;
       if (active) {
         st.info = (uint32_t)k.mode | ((uint32_t)k.rot << 3) | ((uint32_t)k.im << 5) |
                   ((uint32_t)k.nsub << 6) | ((uint32_t)k.shape << 8) | (active << 14) |
                   ((uint32_t)k.entry << 17);
-        const uint32_t slot = atomicAdd(&w.counters[16], 1u);
+        const uint32_t slot = atomicAdd(&w.counters[16], twin ? 2u : 1u);
         w.astate[slot] = st;
+        if (twin) {
+          st.id = id + 1;
+          st.info |= 1u << 17;
+          w.astate[slot + 1] = st;
+        }
       }
     }
   }
