@@ -1099,30 +1099,101 @@ struct AState {
 constexpr unsigned long long kAnnealMarginDiv = 20;
 
 // Learned search: the list of annealing states that pass the margin, in
-// astate order (the set-up kernels append by mode, so the list stays grouped
-// by mode and annealing waves stay uniform).  Persistent; each wave appends
-// its passing entries with one atomic, keeping their order.
-__global__ void __launch_bounds__(256) anneal_list_kernel(const AState *astate, uint32_t *counters,
-                                                          const unsigned long long *cerr,
-                                                          const unsigned long long *q0best,
-                                                          uint32_t *alist) {
-  const uint32_t total = *(volatile uint32_t *)&counters[16];
-  const int lane = threadIdx.x & 63;
-  const uint64_t lt = (1ull << lane) - 1ull;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
-    const uint32_t i = base + threadIdx.x;
-    bool pass = false;
+// astate order (the set-up kernels append by mode, so annealing waves stay
+// uniform and the list's end holds the rotation modes).  An ordered
+// compaction over chunks of 4096 states: per-chunk counts, their exclusive
+// scan, then each chunk writes its passing states from its offset.
+constexpr uint32_t kListChunk = 4096;
+
+__device__ __forceinline__ uint32_t list_flags(const uint32_t *aid, uint32_t total,
+                                               const unsigned long long *cerr,
+                                               const unsigned long long *q0best, uint32_t cb,
+                                               int t) {
+  // thread t checks states cb + 16 t .. cb + 16 t + 15
+  uint32_t flags = 0;
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const uint32_t i = cb + 16u * (uint32_t)t + (uint32_t)k;
     if (i < total) {
-      const uint32_t id = astate[i].id;
+      const uint32_t id = aid[i];
       const unsigned long long best = q0best[id >> 4];
-      pass = cerr[id] <= best + best / kAnnealMarginDiv;
+      if (cerr[id] <= best + best / kAnnealMarginDiv) flags |= 1u << k;
     }
-    const uint64_t m = __ballot(pass);
-    uint32_t at = 0;
-    if (lane == 0 && m) at = atomicAdd(&counters[18], (uint32_t)__popcll(m));
-    at = __shfl(at, 0);
-    if (pass) alist[at + (uint32_t)__popcll(m & lt)] = i;
   }
+  return flags;
+}
+
+// Exclusive prefix over the 256 threads of a workgroup; returns the total.
+__device__ __forceinline__ uint32_t wg_exclusive_scan(uint32_t v, uint32_t *excl,
+                                                      uint32_t *s_wave) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint32_t incl = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(incl, o);
+    if (lane >= o) incl += u;
+  }
+  if (lane == 63) s_wave[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    before += q < wave ? s_wave[q] : 0u;
+    all += s_wave[q];
+  }
+  *excl = before + incl - v;
+  return all;
+}
+
+__global__ void __launch_bounds__(256) list_count_kernel(const uint32_t *aid, const uint32_t *counters,
+                                                         const unsigned long long *cerr,
+                                                         const unsigned long long *q0best,
+                                                         uint32_t *chunk_count) {
+  __shared__ uint32_t s_wave[4];
+  const uint32_t total = counters[16];
+  const uint32_t cb = blockIdx.x * kListChunk;
+  if (cb >= total) return;  // whole workgroup
+  const uint32_t flags = list_flags(aid, total, cerr, q0best, cb, threadIdx.x);
+  uint32_t excl;
+  const uint32_t all = wg_exclusive_scan((uint32_t)__popc(flags), &excl, s_wave);
+  if (threadIdx.x == 0) chunk_count[blockIdx.x] = all;
+}
+
+// One workgroup: exclusive scan of the chunk counts (at most 64 per thread),
+// the list length into counters[18].
+__global__ void __launch_bounds__(256) list_scan_kernel(uint32_t *counters, uint32_t *chunk_count) {
+  __shared__ uint32_t s_wave[4];
+  const uint32_t nchunks = (counters[16] + kListChunk - 1) / kListChunk;
+  const uint32_t per = (nchunks + 255) / 256;
+  const uint32_t c0 = threadIdx.x * per;
+  uint32_t sum = 0;
+  for (uint32_t c = c0; c < c0 + per && c < nchunks; c++) sum += chunk_count[c];
+  uint32_t excl;
+  const uint32_t all = wg_exclusive_scan(sum, &excl, s_wave);
+  for (uint32_t c = c0; c < c0 + per && c < nchunks; c++) {
+    const uint32_t v = chunk_count[c];
+    chunk_count[c] = excl;
+    excl += v;
+  }
+  if (threadIdx.x == 0) counters[18] = all;
+}
+
+__global__ void __launch_bounds__(256) list_write_kernel(const uint32_t *aid, const uint32_t *counters,
+                                                         const unsigned long long *cerr,
+                                                         const unsigned long long *q0best,
+                                                         const uint32_t *chunk_offset,
+                                                         uint32_t *alist) {
+  __shared__ uint32_t s_wave[4];
+  const uint32_t total = counters[16];
+  const uint32_t cb = blockIdx.x * kListChunk;
+  if (cb >= total) return;  // whole workgroup
+  const uint32_t flags = list_flags(aid, total, cerr, q0best, cb, threadIdx.x);
+  uint32_t excl;
+  wg_exclusive_scan((uint32_t)__popc(flags), &excl, s_wave);
+  uint32_t pos = chunk_offset[blockIdx.x] + excl;
+#pragma unroll
+  for (int k = 0; k < 16; k++)
+    if ((flags >> k) & 1u) alist[pos++] = cb + 16u * threadIdx.x + (uint32_t)k;
 }
 
 struct Work {
@@ -1138,6 +1209,9 @@ struct Work {
   // learned search: the annealing states to run (indices into astate, in
   // astate order), counters[18] of them; null in the reference search
   const uint32_t *alist;
+  // learned search: astate[i].id for every state (read when the list is
+  // built, without touching the 64-byte states); null in the reference search
+  uint32_t *aid;
 };
 
 __device__ __forceinline__ void load_block(const BlockRange &r, uint32_t rel, uint32_t px[16]) {
@@ -1779,10 +1853,12 @@ __global__ void __launch_bounds__(64) setup_kernel(BlockRange r, Bc7Params prm, 
                   ((uint32_t)k.entry << 17);
         const uint32_t slot = atomicAdd(&w.counters[16], twin ? 2u : 1u);
         w.astate[slot] = st;
+        if (w.aid) w.aid[slot] = id;
         if (twin) {
           st.id = id + 1;
           st.info |= 1u << 17;
           w.astate[slot + 1] = st;
+          if (w.aid) w.aid[slot + 1] = id + 1;
         }
       }
     }
@@ -2622,7 +2698,8 @@ size_t bc7_work_bytes(uint32_t num_blocks, bool learned) {
     // q0 errors, candidate errors, annealing list, fallback list, one chunk of logits
     const size_t c = n < kBc7MlpChunk ? n : kBc7MlpChunk;
     const size_t rows = (c + 127) & ~(size_t)127;
-    bytes += n * 8 + n * bc7::kMaxCands * (8 + 4) + n * 4 + rows * 512 * 4 + 9 * 256;
+    bytes += n * 8 + n * bc7::kMaxCands * (8 + 4 + 4) + n * 4 + rows * 512 * 4 +
+             (n * bc7::kMaxCands / bc7::kListChunk + 1) * 4 + 11 * 256;
   }
   return bytes;
 }
@@ -2663,12 +2740,15 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   const uint32_t mchunk = n < kBc7MlpChunk ? n : kBc7MlpChunk;
   unsigned long long *cerr = nullptr;
   w.alist = nullptr;
-  uint32_t *alist = nullptr;
+  w.aid = nullptr;
+  uint32_t *alist = nullptr, *chunk_count = nullptr;
   if (learned) {
     const size_t rows = ((size_t)mchunk + 127) & ~(size_t)127;
     q0err = reinterpret_cast<unsigned long long *>(carve((size_t)n * 8));
     cerr = reinterpret_cast<unsigned long long *>(carve(nc * 8));
     alist = reinterpret_cast<uint32_t *>(carve(nc * 4));
+    w.aid = reinterpret_cast<uint32_t *>(carve(nc * 4));
+    chunk_count = reinterpret_cast<uint32_t *>(carve((nc / bc7::kListChunk + 1) * 4));
     fallback = reinterpret_cast<uint32_t *>(carve((size_t)n * 4));
     mlogits = reinterpret_cast<float *>(carve(rows * 512 * 4));
   }
@@ -2733,8 +2813,16 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
       bc7::Work wa = w;
       wa.alist = alist;
       t = timing_begin(stream);
-      hipLaunchKernelGGL(bc7::anneal_list_kernel, dim3((uint32_t)g_num_cus[dev] * 4u), dim3(256),
-                         0, stream, w.astate, w.counters, cerr, q0err, alist);
+      // ordered compaction of the annealing list (grid covers the largest list)
+      const uint32_t nlist = (uint32_t)((nc + bc7::kListChunk - 1) / bc7::kListChunk);
+      hipLaunchKernelGGL(bc7::list_count_kernel, dim3(nlist), dim3(256), 0, stream,
+                         (const uint32_t *)w.aid, (const uint32_t *)w.counters, cerr, q0err,
+                         chunk_count);
+      hipLaunchKernelGGL(bc7::list_scan_kernel, dim3(1), dim3(256), 0, stream, w.counters,
+                         chunk_count);
+      hipLaunchKernelGGL(bc7::list_write_kernel, dim3(nlist), dim3(256), 0, stream,
+                         (const uint32_t *)w.aid, (const uint32_t *)w.counters, cerr, q0err,
+                         (const uint32_t *)chunk_count, alist);
       launch_anneal(persistent, r, p, wa, stream);
       timing_end(t, kTagBc7Anneal, stream);
     }
