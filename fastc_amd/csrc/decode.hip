@@ -204,9 +204,12 @@ __global__ void __launch_bounds__(256)
     error_kernel(const uint32_t *orig, const void *enc, uint32_t width, uint32_t height,
                  unsigned long long *block_err, unsigned long long *sse) {
   const uint32_t bw = (width + 3) / 4, bh = (height + 3) / 4;
-  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t err = 0;
-  if (b < bw * bh) {
+  // persistent: a few workgroups per CU stride over the blocks, so that the
+  // total needs only one atomic accumulation per workgroup
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < bw * bh;
+       b += gridDim.x * blockDim.x) {
+    uint64_t be = 0;
     const uint32_t bx = b % bw, by = b / bw;
     const bool full = VEC && bx * 4 + 4 <= width;
     uint32_t px[16];
@@ -223,8 +226,8 @@ __global__ void __launch_bounds__(256)
           d = *reinterpret_cast<const uint4 *>(static_cast<const uint32_t *>(enc) + off);
         else
           d = make_uint4(px[4 * y], px[4 * y + 1], px[4 * y + 2], px[4 * y + 3]);
-        err += pixel_error64(o.x, d.x) + pixel_error64(o.y, d.y) + pixel_error64(o.z, d.z) +
-               pixel_error64(o.w, d.w);
+        be += pixel_error64(o.x, d.x) + pixel_error64(o.y, d.y) + pixel_error64(o.z, d.z) +
+              pixel_error64(o.w, d.w);
       } else {
 #pragma unroll
         for (int x = 0; x < 4; x++) {
@@ -234,11 +237,12 @@ __global__ void __launch_bounds__(256)
             d = static_cast<const uint32_t *>(enc)[off + x];
           else
             d = px[4 * y + x];
-          err += pixel_error64(orig[off + x], d);
+          be += pixel_error64(orig[off + x], d);
         }
       }
     }
-    if (block_err) block_err[b] = err;
+    if (block_err) block_err[b] = be;
+    err += be;
   }
   // workgroup sum, then one 128-bit accumulation per workgroup
 #pragma unroll
@@ -253,6 +257,21 @@ __global__ void __launch_bounds__(256)
       if (old + s < old) atomicAdd(&sse[1], 1ull);
     }
   }
+}
+
+// Compute units of the current device (cached per device).
+int num_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cus[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        v <= 0)
+      v = 256;
+    cus[dev] = v;
+  }
+  return cus[dev];
 }
 
 template <int FMT>
@@ -276,12 +295,15 @@ hipError_t launch_error(const uint32_t *orig, const void *enc, uint32_t w, uint3
   const uint32_t nb = ((w + 3) / 4) * ((h + 3) / 4);
   bool vec = (w % 4) == 0 && (reinterpret_cast<uintptr_t>(orig) & 15) == 0;
   if (FMT == kFormatRgba8) vec = vec && (reinterpret_cast<uintptr_t>(enc) & 15) == 0;
+  uint32_t grid = (nb + 255) / 256;
+  const uint32_t cap = (uint32_t)num_cus() * 4u;
+  if (grid > cap) grid = cap;
   if (vec)
-    hipLaunchKernelGGL((error_kernel<FMT, true>), dim3((nb + 255) / 256), dim3(256), 0, stream,
-                       orig, enc, w, h, block_err, sse);
+    hipLaunchKernelGGL((error_kernel<FMT, true>), dim3(grid), dim3(256), 0, stream, orig, enc, w,
+                       h, block_err, sse);
   else
-    hipLaunchKernelGGL((error_kernel<FMT, false>), dim3((nb + 255) / 256), dim3(256), 0,
-                       stream, orig, enc, w, h, block_err, sse);
+    hipLaunchKernelGGL((error_kernel<FMT, false>), dim3(grid), dim3(256), 0, stream, orig, enc,
+                       w, h, block_err, sse);
   return hipGetLastError();
 }
 
