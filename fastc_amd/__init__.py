@@ -145,8 +145,12 @@ def lib():
         L.fh_psnr.argtypes = [vp, vp, u32, u32, ctypes.POINTER(ctypes.c_double)]
         L.fh_psnr_from_sse.argtypes = [ctypes.POINTER(ctypes.c_ulonglong), u32, u32]
         L.fh_psnr_from_sse.restype = ctypes.c_double
+        L.fh_mip_level_device.argtypes = [vp, u32, u32, vp, vp]
+        L.fh_pad4_device.argtypes = [vp, u32, u32, vp, vp]
+        L.fh_ktx_write.argtypes = [u32, u32, u32, u32, ctypes.POINTER(ctypes.c_void_p), vp,
+                                   ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         for name in ("fh_decompress_device", "fh_block_error_device", "fh_image_error_device",
-                     "fh_psnr"):
+                     "fh_psnr", "fh_mip_level_device", "fh_pad4_device", "fh_ktx_write"):
             getattr(L, name).restype = ctypes.c_int
         for name in ("fh_batch_create", "fh_batch_destroy", "fh_batch_run", "fh_batch_autotune",
                      "fh_batch_blocks", "fh_batch_set_tune"):
@@ -168,7 +172,7 @@ EXPORTED_SYMBOLS = ["fh_compress", "fh_compress_device", "fh_decompress",
                     "fh_batch_create", "fh_batch_destroy", "fh_batch_run", "fh_batch_autotune",
                     "fh_batch_blocks", "fh_batch_set_tune", "fh_decompress_device",
                     "fh_block_error_device", "fh_image_error_device", "fh_psnr_from_sse",
-                    "fh_psnr"]
+                    "fh_psnr", "fh_mip_level_device", "fh_pad4_device", "fh_ktx_write"]
 
 
 def _check(status):
@@ -425,3 +429,28 @@ def psnr(a, b):
     _check(lib().fh_psnr(a.ctypes.data, b.ctypes.data, a.shape[1], a.shape[0],
                          ctypes.byref(out)))
     return out.value
+
+
+def mip_level_device(src_ptr, width, height, dst_ptr, stream=0):
+    """Next mip level of a device RGBA8 image into dst (max(1, w/2) x max(1, h/2))."""
+    _check(lib().fh_mip_level_device(ctypes.c_void_p(src_ptr), width, height,
+                                     ctypes.c_void_p(dst_ptr), ctypes.c_void_p(stream)))
+
+
+def pad4_device(src_ptr, width, height, dst_ptr, stream=0):
+    """Copy a device image into a ceil4(w) x ceil4(h) one padded with transparent black."""
+    _check(lib().fh_pad4_device(ctypes.c_void_p(src_ptr), width, height,
+                                ctypes.c_void_p(dst_ptr), ctypes.c_void_p(stream)))
+
+
+def ktx_write(fmt, width, height, levels):
+    """KTX 1.1 file bytes of an encoded texture; levels is a list of per-level
+    encoded block arrays (level 0 first)."""
+    datas = [np.ascontiguousarray(l, dtype=np.uint8) for l in levels]
+    ptrs = (ctypes.c_void_p * len(datas))(*[d.ctypes.data for d in datas])
+    size = ctypes.c_size_t(0)
+    _check(lib().fh_ktx_write(fmt, width, height, len(datas), ptrs, None, 0, ctypes.byref(size)))
+    out = np.zeros(size.value, dtype=np.uint8)
+    _check(lib().fh_ktx_write(fmt, width, height, len(datas), ptrs, out.ctypes.data, out.nbytes,
+                              ctypes.byref(size)))
+    return out.tobytes()

@@ -415,6 +415,29 @@ fh_status fh_image_error_device(const uint32_t *a, const uint32_t *b, uint32_t w
   return FH_OK;
 }
 
+fh_status fh_mip_level_device(const uint32_t *src, uint32_t width, uint32_t height, uint32_t *dst,
+                              void *hip_stream) {
+  if (!src || !dst || width == 0 || height == 0)
+    return fail(FH_ERR_INVALID_ARGUMENT, "null buffer or empty image");
+  if (width == 1 && height == 1) return fail(FH_ERR_INVALID_ARGUMENT, "a 1x1 level has no next level");
+  if (!aligned(src, 4) || !aligned(dst, 4)) return fail(FH_ERR_INVALID_ARGUMENT, "misaligned buffer");
+  hipError_t e = mip_level(src, width, height, dst, reinterpret_cast<hipStream_t>(hip_stream));
+  if (e != hipSuccess) return fail_hip(e, "mip launch");
+  return FH_OK;
+}
+
+fh_status fh_pad4_device(const uint32_t *src, uint32_t width, uint32_t height, uint32_t *dst,
+                         void *hip_stream) {
+  if (!src || !dst || width == 0 || height == 0)
+    return fail(FH_ERR_INVALID_ARGUMENT, "null buffer or empty image");
+  if (width > 0xFFFFFFFCu || height > 0xFFFFFFFCu)
+    return fail(FH_ERR_INVALID_ARGUMENT, "image too large");
+  if (!aligned(src, 4) || !aligned(dst, 4)) return fail(FH_ERR_INVALID_ARGUMENT, "misaligned buffer");
+  hipError_t e = pad4(src, width, height, dst, reinterpret_cast<hipStream_t>(hip_stream));
+  if (e != hipSuccess) return fail_hip(e, "pad launch");
+  return FH_OK;
+}
+
 double fh_psnr_from_sse(const unsigned long long sse[2], uint32_t width, uint32_t height) {
   // sse is 255^2 times the reference's sum of squared differences
   const double total = (double)sse[1] * 18446744073709551616.0 + (double)sse[0];

@@ -124,6 +124,12 @@ hipError_t bc7_scatter(const SurfDesc *surfs, const uint32_t *block_surf, uint32
 // per-block premultiplied errors (block_err, optional) and their 128-bit
 // total in sse[0] (low) and sse[1] (high), both overwritten.
 constexpr int kFormatRgba8 = -1;
+
+// Mip levels and padding (mips.hip): dst = max(1, w/2) x max(1, h/2) box-filtered
+// texels; pad4 copies w x h into ceil4(w) x ceil4(h) with transparent black.
+hipError_t mip_level(const uint32_t *src, uint32_t w, uint32_t h, uint32_t *dst,
+                     hipStream_t stream);
+hipError_t pad4(const uint32_t *src, uint32_t w, uint32_t h, uint32_t *dst, hipStream_t stream);
 hipError_t texture_decode(uint32_t format, const uint8_t *in, uint32_t *out, uint32_t width,
                           uint32_t height, hipStream_t stream);
 hipError_t block_error(int format, const uint32_t *original, const void *encoded, uint32_t width,

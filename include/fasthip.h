@@ -158,6 +158,32 @@ double fh_psnr_from_sse(const unsigned long long sse[2], uint32_t width, uint32_
 fh_status fh_psnr(const uint8_t *a, const uint8_t *b, uint32_t width, uint32_t height,
                   double *psnr);
 
+/* Mip chains and containers (the reference encodes the levels it is given
+ * and writes one level).
+ *
+ * fh_mip_level_device: the next mip level of a width x height RGBA8 image,
+ * max(1, width/2) x max(1, height/2) texels (sides halve rounding down), each
+ * the per-channel rounded mean of its 2x2 source texels (2x1 or 1x2 once a
+ * side is 1): (a + b + c + d + 2) >> 2, (a + b + 1) >> 1.  No counterpart in
+ * the reference.  fh_pad4_device: copies width x height texels into a
+ * ceil4(width) x ceil4(height) image padded with transparent black, as
+ * CompressImage pads (Core/src/TexComp.cpp:382-393).  Device buffers,
+ * asynchronous on hip_stream. */
+fh_status fh_mip_level_device(const uint32_t *src, uint32_t width, uint32_t height, uint32_t *dst,
+                              void *hip_stream);
+fh_status fh_pad4_device(const uint32_t *src, uint32_t width, uint32_t height, uint32_t *dst,
+                         void *hip_stream);
+/* KTX 1.1 file image of an encoded texture: ImageWriterKTX::WriteImage
+ * (IO/src/ImageWriterKTX.cpp:69-160) byte for byte for one level of BPTC,
+ * DXT1 or DXT5 whose sides are multiples of 4; levels > 1 writes a mip chain
+ * (level i is max(1, width >> i) x max(1, height >> i) texels, level_data[i]
+ * its grid of whole blocks) and ETC1 is written as GL_ETC1_RGB8_OES, both
+ * beyond the reference.  Host buffers.  *out_size receives the file size;
+ * with out == NULL nothing is written (size query). */
+fh_status fh_ktx_write(uint32_t format, uint32_t width, uint32_t height, uint32_t levels,
+                       const uint8_t *const *level_data, uint8_t *out, size_t out_capacity,
+                       size_t *out_size);
+
 fh_status fh_compressed_size(uint32_t format, uint32_t width, uint32_t height, size_t *size);
 
 const char *fh_last_error(void);
