@@ -149,8 +149,12 @@ def lib():
         L.fh_pad4_device.argtypes = [vp, u32, u32, vp, vp]
         L.fh_ktx_write.argtypes = [u32, u32, u32, u32, ctypes.POINTER(ctypes.c_void_p), vp,
                                    ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+        L.fh_compress_with_stats.argtypes = [ctypes.POINTER(FhJob), vp, ctypes.c_size_t]
+        L.fh_bc7_stats_format.argtypes = [vp, ctypes.c_uint64, vp, ctypes.c_size_t,
+                                          ctypes.POINTER(ctypes.c_size_t)]
         for name in ("fh_decompress_device", "fh_block_error_device", "fh_image_error_device",
-                     "fh_psnr", "fh_mip_level_device", "fh_pad4_device", "fh_ktx_write"):
+                     "fh_psnr", "fh_mip_level_device", "fh_pad4_device", "fh_ktx_write",
+                     "fh_compress_with_stats", "fh_bc7_stats_format"):
             getattr(L, name).restype = ctypes.c_int
         for name in ("fh_batch_create", "fh_batch_destroy", "fh_batch_run", "fh_batch_autotune",
                      "fh_batch_blocks", "fh_batch_set_tune"):
@@ -172,7 +176,14 @@ EXPORTED_SYMBOLS = ["fh_compress", "fh_compress_device", "fh_decompress",
                     "fh_batch_create", "fh_batch_destroy", "fh_batch_run", "fh_batch_autotune",
                     "fh_batch_blocks", "fh_batch_set_tune", "fh_decompress_device",
                     "fh_block_error_device", "fh_image_error_device", "fh_psnr_from_sse",
-                    "fh_psnr", "fh_mip_level_device", "fh_pad4_device", "fh_ktx_write"]
+                    "fh_psnr", "fh_mip_level_device", "fh_pad4_device", "fh_ktx_write",
+                    "fh_compress_with_stats", "fh_bc7_stats_format"]
+
+# fh_bc7_block_stats (include/fasthip.h)
+BLOCK_STATS_DTYPE = np.dtype([("path", "<i4"), ("mode", "<i4"), ("num_two_shape", "<u4"),
+                              ("num_three_shape", "<u4"), ("mode_estimate", "<f8", (8,)),
+                              ("mode_error", "<f8", (8,)), ("two_shape", "<f8", (64,)),
+                              ("three_shape", "<f8", (64,))])
 
 
 def _check(status):
@@ -238,6 +249,33 @@ def compress(img, fmt=eCompressionFormat_BPTC, quality=0, seed=0, block_modes=0x
              block_modes, wm_counter, q0_guard, ranker, topk, threshold)
     _check(lib().fh_compress(ctypes.byref(j)))
     return out
+
+
+def compress_with_stats(img, quality=0, seed=0, block_modes=0xFF, rect=None, wm_counter=None,
+                        q0_guard=True):
+    """BPTCC::CompressWithStats on the GPU (fh_compress_with_stats): returns
+    (compressed bytes, statistics), the statistics a BLOCK_STATS_DTYPE array
+    with one record per block the job processes, in processing order."""
+    img = np.ascontiguousarray(img, dtype=np.uint32)
+    h, w = img.shape
+    out = np.zeros(compressed_size(eCompressionFormat_BPTC, w, h), dtype=np.uint8)
+    stats = np.zeros((h // 4) * (w // 4), dtype=BLOCK_STATS_DTYPE)
+    j = _job(eCompressionFormat_BPTC, img.ctypes.data, out.ctypes.data, out.nbytes, w, h, rect,
+             quality, seed, block_modes, wm_counter, q0_guard, None, 4, 0.6)
+    _check(lib().fh_compress_with_stats(ctypes.byref(j), stats.ctypes.data, stats.size))
+    return out, stats
+
+
+def stats_format(stats, block_index):
+    """The log text CompressWithStats writes for one block
+    (fh_bc7_stats_format): `stats` one BLOCK_STATS_DTYPE record."""
+    rec = np.ascontiguousarray(np.asarray(stats, dtype=BLOCK_STATS_DTYPE).reshape(1))
+    size = ctypes.c_size_t(0)
+    _check(lib().fh_bc7_stats_format(rec.ctypes.data, block_index, None, 0, ctypes.byref(size)))
+    buf = ctypes.create_string_buffer(size.value + 1)
+    _check(lib().fh_bc7_stats_format(rec.ctypes.data, block_index, buf, size.value + 1,
+                                     ctypes.byref(size)))
+    return buf.value.decode()
 
 
 def compress_device(in_ptr, out_ptr, out_size, width, height, fmt=eCompressionFormat_BPTC,

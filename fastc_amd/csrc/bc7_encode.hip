@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/fasthip.h"
 #include "bc7_block_decode.h"
 #include "internal.h"
 
@@ -1004,17 +1005,32 @@ __device__ __forceinline__ uint32_t alpha_fit(const Fit &f, const uint32_t px[16
 // (Compressor.cpp:1771-1809), with the rotation modes split per (rotation,
 // index mode), stored in the order the reference tries them (modes 0, 2, 1,
 // 3, 7, 4, 5, 6; shape entries; rotations; index modes), so that the first
-// strict minimum over the slots is the reference's choice.  Up to 16 slots
-// per block, each a 16-bit descriptor:
+// strict minimum over the slots is the reference's choice.  Up to 32 slots
+// per block (16 in Compress's search; CompressWithStats runs modes 4, 5 and
+// 6 once per shape entry), each a 16-bit descriptor:
 //   mode 3 | shape 6 << 3 | rotation 2 << 9 | index mode 1 << 11 | entry 1 << 12
 // (entry = the BoxSelection shape entry, which keys the annealing stream).
 // Candidate types (work queues, sorted so that a wave runs one code path):
 //   0 mode 0, 1 mode 2, 2 mode 1, 3 mode 3, 4 mode 7, 5 mode 6, 6 mode 4, 7 mode 5.
 constexpr int kNumTypes = 8;
-constexpr int kMaxCands = 16;
+constexpr int kSlotBits = 5;
+constexpr int kMaxCands = 1 << kSlotBits;
+constexpr int kMaxBoxCands = 16;  // slots of a BoxSelection or learned block
+// most candidates of one type a block can have (mode 4: 8 per shape entry)
+constexpr int kMaxPerType = 16;
+
+// Candidate id: relative block << kSlotBits | slot; also the candidate's
+// index in the per-slot arrays (block * kMaxCands + slot).
+__host__ __device__ __forceinline__ uint32_t cand_id(uint32_t rel, uint32_t slot) {
+  return (rel << kSlotBits) | slot;
+}
+__host__ __device__ __forceinline__ uint32_t id_block(uint32_t id) { return id >> kSlotBits; }
+__host__ __device__ __forceinline__ uint32_t id_slot(uint32_t id) {
+  return id & (kMaxCands - 1u);
+}
 __constant__ int kTypeOfMode[8] = {0, 2, 1, 3, 6, 7, 5, 4};
 
-// Block class and candidate count, packed: cls 2 | count 5 << 2.
+// Block class and candidate count, packed: cls 2 | count 5 << 2 (at most 31).
 __device__ __forceinline__ uint32_t sel_pack(int cls, int count) {
   return (uint32_t)cls | ((uint32_t)count << 2);
 }
@@ -1042,7 +1058,7 @@ __device__ __forceinline__ void cand_from_desc(uint32_t d, Cand &k) {
 // The candidate list of a searched block after BoxSelection, in the
 // reference's order (see above).  Returns the count.
 __device__ int box_candidates(bool opaque_layout, int nshapes, int s2, int s3, uint32_t modes,
-                              uint32_t q, uint16_t out[kMaxCands]) {
+                              uint32_t q, uint16_t out[kMaxBoxCands]) {
   int n = 0;
   if (opaque_layout) {
     if (nshapes >= 2) {
@@ -1083,7 +1099,7 @@ struct CandRec {
 
 // Annealing work item (start state from the set-up kernel).
 struct AState {
-  uint32_t id;        // relative block << 4 | candidate
+  uint32_t id;        // cand_id(relative block, slot)
   uint32_t info;      // mode | rot << 3 | im << 5 | nsub << 6 | shape << 8 | active << 14 | entry << 17
   uint32_t p1[3], p2[3];
   uint32_t err[3];
@@ -1116,7 +1132,7 @@ __device__ __forceinline__ uint32_t list_flags(const uint32_t *aid, uint32_t tot
     const uint32_t i = cb + 16u * (uint32_t)t + (uint32_t)k;
     if (i < total) {
       const uint32_t id = aid[i];
-      const unsigned long long best = q0best[id >> 4];
+      const unsigned long long best = q0best[id_block(id)];
       if (cerr[id] <= best + best / kAnnealMarginDiv) flags |= 1u << k;
     }
   }
@@ -1200,7 +1216,7 @@ struct Work {
   uint32_t *sel;          // per block
   uint16_t *cdesc;        // per block x 16 candidate descriptors
   CandRec *rec;           // per block x 16
-  uint32_t *queue;        // per-type candidate queues, 16 entries per block each
+  uint32_t *queue;        // per-type candidate queues, kMaxPerType entries per block each
   uint32_t *counters;     // [0..7] queue sizes, [8..15] set-up fetch, [16] anneal size,
                           // [17] anneal fetch, [18] learned annealing list size
   AState *astate;
@@ -1373,6 +1389,47 @@ __device__ __forceinline__ SubConst pick_sub(bool c, const SubConst &x, const Su
   return r;
 }
 
+// Subset of pixel i under shape masks m1, m2 (0 when the bit of m1 is clear).
+__device__ __forceinline__ uint32_t subset_of(uint32_t m1, uint32_t m2, int i) {
+  return ((m1 >> i) & 1u) ? 1u + ((m2 >> i) & 1u) : 0u;
+}
+
+// Bounding-box corners of every subset of a shape (GetBoundingBox after
+// SetPartition, RGBAEndpoints.h:205-226), as packed RGBA8.
+template <int NS>
+__device__ __forceinline__ void subset_corners(const uint32_t px[16], uint32_t m1, uint32_t m2,
+                                               uint32_t lo[NS], uint32_t hi[NS]) {
+  uint32_t mn02[NS], mn13[NS], mx02[NS], mx13[NS];
+#pragma unroll
+  for (int t = 0; t < NS; t++) {
+    mn02[t] = mn13[t] = 0x00FF00FFu;
+    mx02[t] = mx13[t] = 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const uint32_t s = subset_of(m1, m2, i);
+    const u16x2 p02 = as_u16x2(px[i] & 0x00FF00FFu), p13 = as_u16x2((px[i] >> 8) & 0x00FF00FFu);
+#pragma unroll
+    for (int t = 0; t < NS; t++) {
+      const bool in = s == (uint32_t)t;
+      const u16x2 top = {0xFF, 0xFF}, bottom = {0, 0};
+      mn02[t] = __builtin_bit_cast(uint32_t,
+                                   __builtin_elementwise_min(as_u16x2(mn02[t]), in ? p02 : top));
+      mn13[t] = __builtin_bit_cast(uint32_t,
+                                   __builtin_elementwise_min(as_u16x2(mn13[t]), in ? p13 : top));
+      mx02[t] = __builtin_bit_cast(uint32_t,
+                                   __builtin_elementwise_max(as_u16x2(mx02[t]), in ? p02 : bottom));
+      mx13[t] = __builtin_bit_cast(uint32_t,
+                                   __builtin_elementwise_max(as_u16x2(mx13[t]), in ? p13 : bottom));
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NS; t++) {
+    lo[t] = mn02[t] | (mn13[t] << 8);
+    hi[t] = mx02[t] | (mx13[t] << 8);
+  }
+}
+
 // BoxSelection's estimate of one partition shape (EstimateTwoClusterError /
 // EstimateThreeClusterError, Compressor.cpp:1626-1655): the sum over subsets
 // of 0.0001 + QuantizedError of the subset's points between its bounding-box
@@ -1383,37 +1440,14 @@ template <int NS, int NB>
 __device__ __forceinline__ double shape_estimate(const uint32_t px[16], int shape) {
   const uint32_t m1 = NS == 2 ? (uint32_t)kShape2[shape] : (uint32_t)kShape3[shape][0];
   const uint32_t m2 = NS == 3 ? (uint32_t)kShape3[shape][1] : 0u;
-  uint32_t mn02[NS], mn13[NS], mx02[NS], mx13[NS];
-#pragma unroll
-  for (int t = 0; t < NS; t++) {
-    mn02[t] = mn13[t] = 0x00FF00FFu;
-    mx02[t] = mx13[t] = 0u;
-  }
-#pragma unroll
-  for (int i = 0; i < 16; i++) {
-    const uint32_t s = ((m1 >> i) & 1u) ? 1u + ((m2 >> i) & 1u) : 0u;
-    const u16x2 p02 = as_u16x2(px[i] & 0x00FF00FFu), p13 = as_u16x2((px[i] >> 8) & 0x00FF00FFu);
-#pragma unroll
-    for (int t = 0; t < NS; t++) {
-      const bool in = s == (uint32_t)t;
-      const u16x2 lo = {0xFF, 0xFF}, hi = {0, 0};
-      mn02[t] = __builtin_bit_cast(uint32_t,
-                                   __builtin_elementwise_min(as_u16x2(mn02[t]), in ? p02 : lo));
-      mn13[t] = __builtin_bit_cast(uint32_t,
-                                   __builtin_elementwise_min(as_u16x2(mn13[t]), in ? p13 : lo));
-      mx02[t] = __builtin_bit_cast(uint32_t,
-                                   __builtin_elementwise_max(as_u16x2(mx02[t]), in ? p02 : hi));
-      mx13[t] = __builtin_bit_cast(uint32_t,
-                                   __builtin_elementwise_max(as_u16x2(mx13[t]), in ? p13 : hi));
-    }
-  }
+  uint32_t lo[NS], hi[NS];
+  subset_corners<NS>(px, m1, m2, lo, hi);
   SubConst c[NS];
   bool flat[NS];
 #pragma unroll
   for (int t = 0; t < NS; t++) {
-    const uint32_t lo = mn02[t] | (mn13[t] << 8), hi = mx02[t] | (mx13[t] << 8);
-    c[t] = sub_const(lo, hi);
-    flat[t] = lo == hi;
+    c[t] = sub_const(lo[t], hi[t]);
+    flat[t] = lo[t] == hi[t];
   }
   const WeightFn wf = weight_fn(NB);
   uint32_t e[NS];
@@ -1447,7 +1481,7 @@ __device__ __forceinline__ double shape_estimate(const uint32_t px[16], int shap
 // block's candidates in CompressClusters' order go to cd.  Returns the
 // packed class and count.
 __device__ uint32_t box_select_block(const uint32_t px[16], int lane, const Bc7Params &prm,
-                                     uint16_t cd[kMaxCands]) {
+                                     uint16_t cd[kMaxBoxCands]) {
   bool solid = true, transparent = true, opaque = true;
 #pragma unroll
   for (int i = 0; i < 16; i++) {
@@ -1502,16 +1536,244 @@ __device__ uint32_t box_select_block(const uint32_t px[16], int lane, const Bc7P
   return sel;
 }
 
+// ---------------------------------------------------------------------------
+// CompressWithStats' block path (Compressor.cpp:1954-2183): its own shape
+// scan and the statistics it logs.
+// ---------------------------------------------------------------------------
+// QuantizedError of every subset of a shape between its bounding-box corners
+// quantised with endpoint masks mask_a and mask_b (ToPixel without p-bit,
+// RGBAEndpoints.cpp:167-177), NB index entries: ea/eb per subset (exact
+// integers), flat when the subset is one colour.  Each pixel is scored once
+// per mask, against its own subset's endpoints.
+template <int NS, int NB>
+__device__ __forceinline__ void masked_subset_errors(const uint32_t px[16], uint32_t m1,
+                                                     uint32_t m2, uint32_t mask_a,
+                                                     uint32_t mask_b, bool flat[NS],
+                                                     uint32_t ea[NS], uint32_t eb[NS]) {
+  uint32_t lo[NS], hi[NS];
+  subset_corners<NS>(px, m1, m2, lo, hi);
+  SubConst ca[NS], cb[NS];
+#pragma unroll
+  for (int t = 0; t < NS; t++) {
+    flat[t] = lo[t] == hi[t];
+    ca[t] = sub_const(to_pixel_int(lo[t], mask_a, -1), to_pixel_int(hi[t], mask_a, -1));
+    cb[t] = sub_const(to_pixel_int(lo[t], mask_b, -1), to_pixel_int(hi[t], mask_b, -1));
+    ea[t] = eb[t] = 0;
+  }
+  const WeightFn wf = weight_fn(NB);
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const uint32_t sub = subset_of(m1, m2, i);
+    SubConst ka = ca[0], kb = cb[0];
+#pragma unroll
+    for (int t = 1; t < NS; t++) {
+      ka = pick_sub(sub == (uint32_t)t, ca[t], ka);
+      kb = pick_sub(sub == (uint32_t)t, cb[t], kb);
+    }
+    int best;
+    const uint32_t xa = pixel_err(px[i], px[i], ka, NB, (float)(NB - 1), wf, &best);
+    const uint32_t xb = pixel_err(px[i], px[i], kb, NB, (float)(NB - 1), wf, &best);
+#pragma unroll
+    for (int t = 0; t < NS; t++) {
+      ea[t] += sub == (uint32_t)t ? xa : 0u;
+      eb[t] += sub == (uint32_t)t ? xb : 0u;
+    }
+  }
+}
+
+// std::min(a, b): b only when strictly smaller.
+__device__ __forceinline__ double min_ref(double a, double b) { return b < a ? b : a; }
+
+// One shape of the statistics scan (EstimateTwoClusterErrorStats and
+// EstimateThreeClusterErrorStats, Compressor.cpp:1862-1935, summed over the
+// subsets as at :2074-2092 and :2124-2141): the selection error and the two
+// per-mode estimates (modes 1 and 3 for two subsets, 0 and 2 for three), in
+// the reference's double operations and order.
+template <int NS>
+__device__ __forceinline__ double stats_shape(const uint32_t px[16], int shape, double &est_a,
+                                              double &est_b) {
+  constexpr int NB = NS == 2 ? 8 : 4;
+  const uint32_t m1 = NS == 2 ? (uint32_t)kShape2[shape] : (uint32_t)kShape3[shape][0];
+  const uint32_t m2 = NS == 3 ? (uint32_t)kShape3[shape][1] : 0u;
+  const uint32_t mask_a = NS == 2 ? 0xFFFCFCFCu : 0xFFF0F0F0u;
+  const uint32_t mask_b = NS == 2 ? 0xFFFEFEFEu : 0xFFF8F8F8u;
+  bool flat[NS];
+  uint32_t ea[NS], eb[NS];
+  masked_subset_errors<NS, NB>(px, m1, m2, mask_a, mask_b, flat, ea, eb);
+  double err = 0.0;
+#pragma unroll
+  for (int t = 0; t < NS; t++) {
+    double x = 0.0, sa = 0.0, sb = 0.0;
+    if (!flat[t]) {
+      // two subsets: the estimates are the errors; three: 0.0001 + the errors
+      sa = NS == 2 ? (double)ea[t] : 0.0001 + (double)ea[t];
+      sb = NS == 2 ? (double)eb[t] : 0.0001 + (double)eb[t];
+      x = 0.0001;
+      x += min_ref(sa, sb);
+    }
+    err += x;
+    est_a = t == 0 ? sa : est_a + sa;
+    est_b = t == 0 ? sb : est_b + sb;
+  }
+  return err;
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+  return v;
+}
+
+// The scan over 64 shapes, lane i holding shape i's error: the scan stops
+// after the first shape below 1e-9 (brk, 64 if none); a shape is logged when
+// strictly below every earlier scanned one (bestError starts at DBL_MAX).
+// The logged errors go to log[], in shape order, the rest of its 64 entries
+// are zeroed; returns the mask of logged shapes (the last of them is the
+// chosen shape).
+__device__ __forceinline__ uint64_t stats_scan(double err, int lane, int &brk, double *log) {
+  const uint64_t under = __ballot(err < 1e-9);
+  brk = under ? __ffsll((unsigned long long)under) - 1 : 64;
+  const bool scanned = lane <= brk;
+  double pm = scanned ? err : DBL_MAX;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double u = __shfl_up(pm, o);
+    if (lane >= o) pm = fmin(pm, u);
+  }
+  double before = __shfl_up(pm, 1);
+  if (lane == 0) before = DBL_MAX;
+  const bool logged = scanned && err < before;
+  const uint64_t mask = __ballot(logged);
+  if (logged) log[__popcll(mask & ((1ull << lane) - 1ull))] = err;
+  if (lane >= __popcll(mask)) log[lane] = 0.0;
+  return mask;
+}
+
+// The candidates of a statistics block in CompressClusters' order
+// (Compressor.cpp:1771-1809) for the shape entries (2 subsets, s2) and, on
+// opaque blocks, (3 subsets, s3): a single-subset mode runs once per entry;
+// without annealing its second entry repeats the first exactly and cannot
+// win (strict <), so it is left out.
+__device__ int stats_candidates(int nshapes, int s2, int s3, uint32_t modes, uint32_t q,
+                                uint16_t out[kMaxCands]) {
+  constexpr int kOrder[8] = {0, 2, 1, 3, 7, 4, 5, 6};
+  int n = 0;
+#pragma unroll
+  for (int mi = 0; mi < 8; mi++) {
+    const int mode = kOrder[mi];
+    if (!((modes >> mode) & 1)) continue;
+    for (int e = 0; e < nshapes; e++) {
+      const int parts = e == 0 ? 2 : 3, idx = e == 0 ? s2 : s3;
+      const int nsub = kModes[mode].subsets;
+      if (nsub != 1 && nsub != parts) continue;
+      if (mode == 0 && idx >= 16) continue;
+      if (nsub == 1 && e > 0 && q == 0) continue;
+      if (mode == 4) {
+        for (int c = 0; c < 8; c++) out[n++] = (uint16_t)cdesc_pack(4, 0, c >> 1, c & 1, e);
+      } else if (mode == 5) {
+        for (int c = 0; c < 4; c++) out[n++] = (uint16_t)cdesc_pack(5, 0, c, 0, e);
+      } else {
+        out[n++] = (uint16_t)cdesc_pack(mode, idx, 0, 0, e);
+      }
+    }
+  }
+  return n;
+}
+
+// Classification, the statistics scan and the candidates of one block by a
+// whole wave (lane i scores shape i); writes the block's statistics except
+// the mode (known once the block is packed).  Returns the packed class and
+// count.
+__device__ uint32_t stats_select_block(const uint32_t px[16], int lane, const Bc7Params &prm,
+                                       uint16_t cd[kMaxCands], fh_bc7_block_stats *bs) {
+  bool solid = true, transparent = true, opaque = true;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    solid = solid && px[i] == px[0];
+    transparent = transparent && (px[i] >> 24) == 0;
+    opaque = opaque && (px[i] >> 24) == 255;
+  }
+  // every field of the record is written once (the caller's buffer may hold
+  // anything): estimates -1 unless set, unused shape entries 0
+  if (lane < 8) bs->mode_error[lane] = -1.0;
+  if (!opaque || solid || transparent) bs->three_shape[lane] = 0.0;
+  if (solid || transparent) {
+    bs->two_shape[lane] = 0.0;
+    if (lane < 8) bs->mode_estimate[lane] = -1.0;
+    if (lane == 0) {
+      bs->path = solid ? 0 : 1;
+      bs->num_two_shape = bs->num_three_shape = 0;
+    }
+    return sel_pack(solid ? 0 : 1, 0);
+  }
+  // mode 6: the whole block as one line, four index values, 7-bit
+  // endpoints (Compressor.cpp:2046-2061); never one colour here
+  double est6;
+  {
+    bool flat[1];
+    uint32_t ea[1], eb[1];
+    masked_subset_errors<1, 4>(px, 0u, 0u, 0xFEFEFEFEu, 0xFEFEFEFEu, flat, ea, eb);
+    est6 = flat[0] ? 0.0 : 0.0001 + (double)ea[0];
+  }
+  double e1, e3;
+  const double err2 = stats_shape<2>(px, lane, e1, e3);
+  int brk2;
+  const uint64_t log2 = stats_scan(err2, lane, brk2, bs->two_shape);
+  const int s2 = 63 - __clzll((long long)log2);
+  uint32_t modes = 0xFFu;
+  int path = 3;
+  if (brk2 < 64) {
+    path = 2;
+    modes = (1u << 1) | (1u << 3) | (1u << 7);
+  }
+  // UpdateErrorEstimate keeps the minimum over the scanned shapes
+  const double est1 = wave_min(lane <= brk2 ? e1 : DBL_MAX);
+  const double est3 = wave_min(lane <= brk2 ? e3 : DBL_MAX);
+  int s3 = 0, nshapes = 1;
+  uint64_t log3 = 0;
+  double est0 = -1.0, est2 = -1.0;
+  if (opaque) {
+    // runs even when the two-subset scan stopped early
+    double e0, e2;
+    const double err3 = stats_shape<3>(px, lane, e0, e2);
+    int brk3;
+    log3 = stats_scan(err3, lane, brk3, bs->three_shape);
+    s3 = 63 - __clzll((long long)log3);
+    nshapes = 2;
+    if (brk3 < 64) {
+      path = 2;
+      modes = (1u << 0) | (1u << 2);
+    }
+    est0 = wave_min(lane <= brk3 ? e0 : DBL_MAX);
+    est2 = wave_min(lane <= brk3 ? e2 : DBL_MAX);
+  }
+  if (lane < 8) {
+    // modes 4, 5 and 7 get no estimate on this path
+    const double est = lane == 0 ? est0 : lane == 1 ? est1 : lane == 2 ? est2
+                     : lane == 3 ? est3 : lane == 6 ? est6 : -1.0;
+    bs->mode_estimate[lane] = est;
+  }
+  if (lane == 0) {
+    bs->path = path;
+    bs->num_two_shape = (uint32_t)__popcll(log2);
+    bs->num_three_shape = (uint32_t)__popcll(log3);
+  }
+  const int count = stats_candidates(nshapes, s2, s3, modes & prm.block_modes, prm.quality, cd);
+  return sel_pack(2, count);
+}
+
 // Kernel 1: one workgroup of four waves selects 16 blocks, each wave four
 // in turn; the workgroup's candidates go to the type queues with one global
 // atomic per type.  list, if given, holds the blocks to select (list_count
 // of them, a device word); otherwise all blocks of the range.
+template <bool STATS>
 __global__ void __launch_bounds__(256) select_kernel(BlockRange r, Bc7Params prm, Work w,
                                                      const uint32_t *list,
                                                      const uint32_t *list_count) {
+  constexpr int kSlots = STATS ? kMaxCands : kMaxBoxCands;
   __shared__ uint32_t s_px[16][16];
   __shared__ uint32_t s_sel[16];
-  __shared__ uint16_t s_cd[16][kMaxCands];
+  __shared__ uint16_t s_cd[16][kSlots];
   __shared__ uint32_t s_cnt[kNumTypes], s_base[kNumTypes];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t base = blockIdx.x * 16;  // position in the list or the range
@@ -1536,8 +1798,9 @@ __global__ void __launch_bounds__(256) select_kernel(BlockRange r, Bc7Params prm
     uint32_t px[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) px[i] = s_px[blk][i];
-    uint16_t cd[kMaxCands];
-    const uint32_t sel = box_select_block(px, lane, prm, cd);
+    uint16_t cd[kSlots];
+    const uint32_t sel = STATS ? stats_select_block(px, lane, prm, cd, prm.stats + rel)
+                               : box_select_block(px, lane, prm, cd);
     const int count = sel_count(sel);
     if (lane == 0) {
       s_sel[blk] = sel;
@@ -1545,23 +1808,35 @@ __global__ void __launch_bounds__(256) select_kernel(BlockRange r, Bc7Params prm
     }
     // slot c of this block's descriptors goes out from lane c
     if (lane < count) w.cdesc[(size_t)rel * kMaxCands + lane] = cd[lane];
-    if (lane < kMaxCands) s_cd[blk][lane] = lane < count ? cd[lane] : (uint16_t)0;
+    if (lane < kSlots) s_cd[blk][lane] = lane < count ? cd[lane] : (uint16_t)0;
   }
   __syncthreads();
-  // enqueue: thread = (block, candidate slot)
-  const int blk = tid >> 4, c = tid & 15;
-  bool valid = false;
-  int type = 0;
-  uint32_t rel = 0;
-  if (base + blk < total && c < sel_count(s_sel[blk])) {
-    // mode 6's second shape entry has the same quality-0 fit as the first
-    // (its slot follows it): the set-up of the first writes both
-    const uint32_t d = s_cd[blk][c];
-    valid = !((d & 7) == 6 && ((d >> 12) & 1));
-    rel = rel_of(base + blk);
-    type = kTypeOfMode[d & 7];
+  // enqueue: thread = (block, candidate slot), 16 slots per round
+#pragma unroll
+  for (int half = 0; half < kSlots / 16; half++) {
+    const int blk = tid >> 4, c = (tid & 15) + 16 * half;
+    bool valid = false;
+    int type = 0;
+    uint32_t rel = 0;
+    if (base + blk < total && c < sel_count(s_sel[blk])) {
+      // mode 6's second shape entry has the same quality-0 fit as the first
+      // (its slot follows it): the set-up of the first writes both
+      const uint32_t d = s_cd[blk][c];
+      valid = !((d & 7) == 6 && ((d >> 12) & 1));
+      rel = rel_of(base + blk);
+      type = kTypeOfMode[d & 7];
+    }
+    wg_enqueue(w, valid, type, cand_id(rel, (uint32_t)c), s_cnt, s_base);
   }
-  wg_enqueue(w, valid, type, (rel << 4) | (uint32_t)c, s_cnt, s_base);
+}
+
+// The mode of each written block (the lowest set bit of its first byte; 8
+// for the reserved encoding) into its statistics.
+__global__ void __launch_bounds__(256) stats_mode_kernel(BlockRange r, fh_bc7_block_stats *stats) {
+  const uint32_t rel = blockIdx.x * 256 + threadIdx.x;
+  if (rel >= r.num_blocks) return;
+  const uint32_t b0 = r.out[(size_t)(r.first_block + rel) * 16];
+  stats[rel].mode = b0 ? __ffs(b0) - 1 : 8;
 }
 
 // ---------------------------------------------------------------------------
@@ -1684,7 +1959,7 @@ __global__ void __launch_bounds__(1024) learned_select_kernel(BlockRange r, Bc7P
   if (threadIdx.x == 0 && s_fb) s_fbbase = atomicAdd(fallback_count, s_fb);
   __syncthreads();
   if (fell_back && lane == 0) fallback[s_fbbase + fb_slot] = rel;
-  wg_enqueue(w, lane < count, kTypeOfMode[my_desc & 7], (rel << 4) | (uint32_t)lane, s_cnt,
+  wg_enqueue(w, lane < count, kTypeOfMode[my_desc & 7], cand_id(rel, (uint32_t)lane), s_cnt,
              s_base);
 }
 
@@ -1819,8 +2094,8 @@ __global__ void __launch_bounds__(64) setup_kernel(BlockRange r, Bc7Params prm, 
       const uint32_t item = b0 + lane;
       if (item >= count) continue;
       const uint32_t id = w.queue[(size_t)t * w.qcap + item];
-      const uint32_t rel = id >> 4;
-      const int c = id & 15;
+      const uint32_t rel = id_block(id);
+      const int c = (int)id_slot(id);
       Cand k;
       cand_from_desc(w.cdesc[(size_t)rel * kMaxCands + c], k);
       // the lane's texels live in LDS: the fit reads them many times, and
@@ -1909,14 +2184,14 @@ __device__ void lane_load(Lane &L, const BlockRange &r, const Bc7Params &prm, co
   }
   {
     uint32_t v[16];
-    load_block(r, st.id >> 4, v);
+    load_block(r, id_block(st.id), v);
 #pragma unroll
     for (int i = 0; i < 16; i++) {
       L.px[i] = v[i];
       L.pt[i] = point_of(v[i], L.ptmode);
     }
   }
-  const uint64_t bid = prm.block_id_base + r.first_block + (st.id >> 4);
+  const uint64_t bid = prm.block_id_base + r.first_block + id_block(st.id);
 #pragma unroll
   for (int s = 0; s < 3; s++) {
     L.p1[s] = L.bp1[s] = st.p1[s];
@@ -2052,7 +2327,7 @@ __device__ bool lane_step(Lane &L, uint32_t q) {
 }
 
 __device__ void lane_finish(Lane &L, Work &w) {
-  CandRec *rc = &w.rec[(size_t)(L.id >> 4) * 16 + (L.id & 15)];
+  CandRec *rc = &w.rec[L.id];
   const Fit &f = L.f;
   uint32_t total = L.extra;
   uint32_t pbcs = rc->meta & 0x3Fu;
@@ -2692,7 +2967,7 @@ size_t bc7_work_bytes(uint32_t num_blocks, bool learned) {
   // counters, sel, cdesc, rec, queues, astate, q0, alignment slack
   size_t bytes = 64 * 4 + n * 4 + n * bc7::kMaxCands * 2 +
                  n * bc7::kMaxCands * sizeof(bc7::CandRec) +
-                 n * bc7::kNumTypes * bc7::kMaxCands * 4 +
+                 n * bc7::kNumTypes * bc7::kMaxPerType * 4 +
                  n * bc7::kMaxCands * sizeof(bc7::AState) + n * 16 + 8 * 256;
   if (learned) {
     // q0 errors, candidate errors, annealing list, fallback list, one chunk of logits
@@ -2728,8 +3003,8 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   w.sel = reinterpret_cast<uint32_t *>(carve((size_t)n * 4));
   w.cdesc = reinterpret_cast<uint16_t *>(carve(nc * 2));
   w.rec = reinterpret_cast<bc7::CandRec *>(carve(nc * sizeof(bc7::CandRec)));
-  w.qcap = (uint32_t)nc;
-  w.queue = reinterpret_cast<uint32_t *>(carve(nc * bc7::kNumTypes * 4));
+  w.qcap = (uint32_t)((size_t)n * bc7::kMaxPerType);
+  w.queue = reinterpret_cast<uint32_t *>(carve((size_t)w.qcap * bc7::kNumTypes * 4));
   w.astate = reinterpret_cast<bc7::AState *>(carve(nc * sizeof(bc7::AState)));
   w.astate_cap = (uint32_t)nc;
   uint4 *q0 = reinterpret_cast<uint4 *>(carve((size_t)n * 16));
@@ -2784,12 +3059,17 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
       timing_end(t, kTagBc7Select, stream);
     }
     t = timing_begin(stream);
-    hipLaunchKernelGGL(bc7::select_kernel, dim3(ngroups), dim3(256), 0, stream, r, p, w,
+    hipLaunchKernelGGL(bc7::select_kernel<false>, dim3(ngroups), dim3(256), 0, stream, r, p, w,
                        (const uint32_t *)fallback, (const uint32_t *)fallback_count);
+    timing_end(t, kTagBc7Select, stream);
+  } else if (p.stats) {
+    t = timing_begin(stream);
+    hipLaunchKernelGGL(bc7::select_kernel<true>, dim3(ngroups), dim3(256), 0, stream, r, p, w,
+                       (const uint32_t *)nullptr, (const uint32_t *)nullptr);
     timing_end(t, kTagBc7Select, stream);
   } else {
     t = timing_begin(stream);
-    hipLaunchKernelGGL(bc7::select_kernel, dim3(ngroups), dim3(256), 0, stream, r, p, w,
+    hipLaunchKernelGGL(bc7::select_kernel<false>, dim3(ngroups), dim3(256), 0, stream, r, p, w,
                        (const uint32_t *)nullptr, (const uint32_t *)nullptr);
     timing_end(t, kTagBc7Select, stream);
   }
@@ -2852,6 +3132,8 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
     hipLaunchKernelGGL(bc7::guard_kernel, pack_grid, dim3(256), 0, stream, r, q0);
     timing_end(t, kTagBc7Guard, stream);
   }
+  if (p.stats)
+    hipLaunchKernelGGL(bc7::stats_mode_kernel, pack_grid, dim3(256), 0, stream, r, p.stats);
   return hipGetLastError();
 }
 

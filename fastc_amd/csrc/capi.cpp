@@ -47,6 +47,8 @@ struct DeviceState {
   uint8_t *out_buf = nullptr;
   size_t out_size = 0;
   unsigned long long *sse = nullptr;  // 2 words, fh_psnr's total
+  void *stats_buf = nullptr;          // fh_compress_with_stats' records
+  size_t stats_size = 0;
   bool init = false;
 };
 
@@ -178,7 +180,8 @@ fh_status validate(const fh_job *j, bool device) {
 
 // Runs the encoder on device buffers.  Caller holds st->mu.
 fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8_t *dout,
-                     hipStream_t stream, uint32_t first, uint32_t count) {
+                     hipStream_t stream, uint32_t first, uint32_t count,
+                     fh_bc7_block_stats *dstats = nullptr) {
   BlockRange r;
   r.img = reinterpret_cast<const uint32_t *>(din);
   r.out = dout;
@@ -224,6 +227,7 @@ fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8
       p.threshold = j->ranker_threshold;
       p.wg_per_cu = 0;
       p.anneal_unroll = 0;
+      p.stats = nullptr;
       // jobs larger than one call's work area run as consecutive calls on the
       // stream; the watermark counter carries across them in order
       for (uint32_t done = 0; done < count; done += chunk) {
@@ -231,6 +235,7 @@ fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8
         rc.first_block = first + done;
         rc.num_blocks = count - done < chunk ? count - done : chunk;
         s.num_chunks = (rc.num_blocks + 255) / 256;
+        if (dstats) p.stats = dstats + done;
         e = bc7_encode(rc, p, s, work, stream);
         if (e != hipSuccess) return fail_hip(e, "BC7 encode launch");
       }
@@ -270,6 +275,54 @@ int current_device(fh_status *s) {
   return dev;
 }
 
+// fh_compress and fh_compress_with_stats on host buffers: stage the image,
+// encode, copy the blocks (and statistics) back.  Caller-facing checks as in
+// CompressImageData.
+fh_status compress_host(const fh_job *j, bool with_stats, fh_bc7_block_stats *stats,
+                        size_t stats_count) {
+  fh_status s = validate(j, false);
+  if (s != FH_OK) return s;
+  uint32_t first = 0, count = 0;
+  s = job_range(j, &first, &count);
+  if (s != FH_OK) return s;
+  if (with_stats && count && (!stats || stats_count < count))
+    return fail(FH_ERR_BUFFER_TOO_SMALL, "statistics buffer too small");
+  const int dev = current_device(&s);
+  if (dev < 0) return s;
+  DeviceState *st = device_state(dev);
+  std::lock_guard<std::mutex> g(st->mu);
+  hipError_t e = ensure_init(st);
+  if (e != hipSuccess) return fail_hip(e, "device initialisation");
+  const size_t in_bytes = (size_t)j->width * j->height * 4;
+  const uint32_t bb = block_bytes(j->format);
+  const size_t out_bytes = (size_t)(j->width / 4) * (j->height / 4) * bb;
+  e = grow(reinterpret_cast<void **>(&st->in_buf), &st->in_size, in_bytes);
+  if (e == hipSuccess)
+    e = grow(reinterpret_cast<void **>(&st->out_buf), &st->out_size, out_bytes);
+  const size_t stats_bytes = with_stats ? (size_t)count * sizeof(fh_bc7_block_stats) : 0;
+  if (e == hipSuccess && stats_bytes) e = grow(&st->stats_buf, &st->stats_size, stats_bytes);
+  if (e != hipSuccess) return fail_hip(e, "device buffer allocation");
+  fh_bc7_block_stats *dstats =
+      stats_bytes ? reinterpret_cast<fh_bc7_block_stats *>(st->stats_buf) : nullptr;
+  hipStream_t stream = nullptr;
+  e = hipMemcpyAsync(st->in_buf, j->in, in_bytes, hipMemcpyHostToDevice, stream);
+  if (e != hipSuccess) return fail_hip(e, "input upload");
+  s = run_device(st, j, st->in_buf, st->out_buf, stream, first, count, dstats);
+  if (s != FH_OK) return s;
+  if (dstats) {
+    e = hipMemcpyAsync(stats, dstats, stats_bytes, hipMemcpyDeviceToHost, stream);
+    if (e != hipSuccess) return fail_hip(e, "statistics download");
+  }
+  if (count) {
+    e = hipMemcpyAsync(j->out + (size_t)first * bb, st->out_buf + (size_t)first * bb,
+                       (size_t)count * bb, hipMemcpyDeviceToHost, stream);
+    if (e != hipSuccess) return fail_hip(e, "output download");
+  }
+  e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) return fail_hip(e, "encode");
+  return FH_OK;
+}
+
 }  // namespace
 
 void set_error(const char *msg) { t_last_error = msg; }
@@ -292,37 +345,53 @@ using namespace fh;
 
 extern "C" {
 
-fh_status fh_compress(const fh_job *j) {
-  fh_status s = validate(j, false);
-  if (s != FH_OK) return s;
-  uint32_t first = 0, count = 0;
-  s = job_range(j, &first, &count);
-  if (s != FH_OK) return s;
-  const int dev = current_device(&s);
-  if (dev < 0) return s;
-  DeviceState *st = device_state(dev);
-  std::lock_guard<std::mutex> g(st->mu);
-  hipError_t e = ensure_init(st);
-  if (e != hipSuccess) return fail_hip(e, "device initialisation");
-  const size_t in_bytes = (size_t)j->width * j->height * 4;
-  const uint32_t bb = block_bytes(j->format);
-  const size_t out_bytes = (size_t)(j->width / 4) * (j->height / 4) * bb;
-  e = grow(reinterpret_cast<void **>(&st->in_buf), &st->in_size, in_bytes);
-  if (e == hipSuccess)
-    e = grow(reinterpret_cast<void **>(&st->out_buf), &st->out_size, out_bytes);
-  if (e != hipSuccess) return fail_hip(e, "device buffer allocation");
-  hipStream_t stream = nullptr;
-  e = hipMemcpyAsync(st->in_buf, j->in, in_bytes, hipMemcpyHostToDevice, stream);
-  if (e != hipSuccess) return fail_hip(e, "input upload");
-  s = run_device(st, j, st->in_buf, st->out_buf, stream, first, count);
-  if (s != FH_OK) return s;
-  if (count) {
-    e = hipMemcpyAsync(j->out + (size_t)first * bb, st->out_buf + (size_t)first * bb,
-                       (size_t)count * bb, hipMemcpyDeviceToHost, stream);
-    if (e != hipSuccess) return fail_hip(e, "output download");
+fh_status fh_compress(const fh_job *j) { return compress_host(j, false, nullptr, 0); }
+
+fh_status fh_compress_with_stats(const fh_job *j, fh_bc7_block_stats *stats, size_t stats_count) {
+  if (j && j->format != FH_FORMAT_BPTC && block_bytes(j->format) != 0)
+    return fail(FH_ERR_UNSUPPORTED, "statistics are only collected for BPTC");
+  if (j && (j->flags & FH_FLAG_LEARNED))
+    return fail(FH_ERR_INVALID_ARGUMENT, "statistics are not collected by the learned search");
+  return compress_host(j, true, stats, stats_count);
+}
+
+fh_status fh_bc7_stats_format(const fh_bc7_block_stats *st, uint64_t block_index, char *out,
+                              size_t out_capacity, size_t *out_size) {
+  if (!st || !out_size) return fail(FH_ERR_INVALID_ARGUMENT, "null statistics or size");
+  if (st->num_two_shape > 64 || st->num_three_shape > 64)
+    return fail(FH_ERR_INVALID_ARGUMENT, "more than 64 shape estimates");
+  // kBlockStatString (Compressor.cpp:106-131) spells the mode numbers out
+  static const char *const kNum[8] = {"Zero", "One", "Two", "Three",
+                                      "Four", "Five", "Six", "Seven"};
+  std::string text;
+  char line[192];
+  // BlockLogger prefixes "<index>: " to each "<name> -- <value>\n" that
+  // PrintStat builds with an ostream (doubles: default format, i.e. %g)
+  auto add_d = [&](const char *name, const char *suffix, double v) {
+    snprintf(line, sizeof(line), "%llu: BlockStat_%s%s -- %g\n", (unsigned long long)block_index,
+             name, suffix, v);
+    text += line;
+  };
+  auto add_i = [&](const char *name, int v) {
+    snprintf(line, sizeof(line), "%llu: BlockStat_%s -- %d\n", (unsigned long long)block_index,
+             name, v);
+    text += line;
+  };
+  for (uint32_t k = 0; k < st->num_two_shape; k++) add_d("TwoShapeEstimate", "", st->two_shape[k]);
+  for (uint32_t k = 0; k < st->num_three_shape; k++)
+    add_d("ThreeShapeEstimate", "", st->three_shape[k]);
+  add_i("Path", st->path);
+  add_i("Mode", st->mode);
+  char name[32];
+  for (int m = 0; m < 8; m++) {
+    snprintf(name, sizeof(name), "Mode%s", kNum[m]);
+    add_d(name, "Estimate", st->mode_estimate[m]);
+    add_d(name, "Error", st->mode_error[m]);
   }
-  e = hipStreamSynchronize(stream);
-  if (e != hipSuccess) return fail_hip(e, "encode");
+  *out_size = text.size();
+  if (!out) return FH_OK;
+  if (out_capacity <= text.size()) return fail(FH_ERR_BUFFER_TOO_SMALL, "text buffer too small");
+  memcpy(out, text.c_str(), text.size() + 1);
   return FH_OK;
 }
 

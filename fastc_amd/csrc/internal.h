@@ -7,6 +7,8 @@
 
 #include <mutex>
 
+struct fh_bc7_block_stats;  // include/fasthip.h
+
 namespace fh {
 
 // Launch parameters shared by the block encoders.  Blocks are addressed in
@@ -34,6 +36,9 @@ struct Bc7Params {
   float threshold;       // below this top softmax probability: BoxSelection
   uint32_t wg_per_cu;    // persistent workgroups per CU (0 = 16)
   uint32_t anneal_unroll;  // annealing steps between refills (0 = 1)
+  // CompressWithStats' block path: one record per block of the range (null =
+  // Compress's BoxSelection search); not combined with the ranker
+  fh_bc7_block_stats *stats;
 };
 
 // Device scratch needed by the BC7 watermark prefix scan.

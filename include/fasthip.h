@@ -23,6 +23,10 @@
  *   fh_psnr, fh_block_error_device, fh_image_error_device
  *                        Image::ComputePSNR (Base/src/Image.cpp:205-255) and
  *                        its per-block terms, fused with decoding.
+ *   fh_compress_with_stats, fh_bc7_stats_format
+ *                        BPTCC::CompressWithStats (BPTCCompressor.h:173,
+ *                        BPTCEncoder/src/Compressor.cpp:1577-1623) and the
+ *                        BlockLogger lines it writes (Compressor.cpp:1373-1390).
  *   fh_compressed_size   CompressedImage::GetCompressedSize
  *                        (Core/src/CompressedImage.cpp:136-149).
  *   fh_last_error        the stderr message CompressImageData prints
@@ -116,6 +120,51 @@ typedef struct fh_job {
 
 fh_status fh_compress(const fh_job *job);
 fh_status fh_compress_device(const fh_job *job, void *hip_stream);
+
+/* One block's statistics, in the vocabulary of the log CompressWithStats
+ * writes (EBlockStats, BPTCEncoder/src/Compressor.cpp:77-131; the statistics
+ * version of CompressBC7Block, :1954-2183). */
+typedef struct fh_bc7_block_stats {
+  int32_t path;             /* BlockStat_Path: 0 solid, 1 transparent, 2 a shape
+                               scan stopped at an estimate below 1e-9, 3 otherwise */
+  int32_t mode;             /* BlockStat_Mode: the mode of the block written */
+  uint32_t num_two_shape;   /* BlockStat_TwoShapeEstimate lines: the two-subset
+                               scan's strict improvements, in two_shape */
+  uint32_t num_three_shape; /* BlockStat_ThreeShapeEstimate lines, in three_shape */
+  double mode_estimate[8];  /* BlockStat_Mode<N>Estimate; -1 when never set */
+  double mode_error[8];     /* BlockStat_Mode<N>Error: always -1, as in the
+                               reference, whose CompressClusters keeps
+                               std::min(error, -1.0) (Compressor.cpp:1800-1801) */
+  double two_shape[64];
+  double three_shape[64];
+} fh_bc7_block_stats;
+
+/* BPTCC::CompressWithStats (BPTCEncoder/include/FasTC/BPTCCompressor.h:173,
+ * Compressor.cpp:1577-1623) for a BPTC job on host buffers: the same job
+ * rules and output layout as fh_compress, with the statistics block path
+ * (Compressor.cpp:1954-2183), whose shape scan differs from Compress's
+ * BoxSelection: it estimates two-subset shapes with mode 1 and mode 3
+ * endpoints and three-subset shapes with mode 0 and mode 2 endpoints, calls a
+ * block opaque only if every alpha is 255, and keeps modes 4 and 5 for opaque
+ * blocks (each single-subset mode then runs once per shape entry).  So the
+ * bytes differ from fh_compress's.  stats[k] receives the k-th block the job
+ * processes (raster order from (x_start, y_start)); stats_count must cover
+ * them.  FH_FLAG_LEARNED is rejected; other formats return
+ * FH_ERR_UNSUPPORTED, as the reference only collects BPTC statistics
+ * (Core/src/TexComp.cpp:87-108). */
+fh_status fh_compress_with_stats(const fh_job *job, fh_bc7_block_stats *stats,
+                                 size_t stats_count);
+
+/* The text BlockLogger writes for one block (Compressor.cpp:1373-1390,
+ * 1945-1950): a line "<block_index>: BlockStat_<name> -- <value>" per
+ * statistic, in the order CompressWithStats writes them (shape estimates,
+ * Path, Mode, then Mode<N>Estimate and Mode<N>Error for N = 0..7), doubles in
+ * the ostream default format.  block_index is the block's raster index in the
+ * image (CompressionJob::CoordsToBlockIdx).  Writes the text and a NUL when
+ * out is non-null and large enough; *out_size receives the text's length
+ * (without the NUL).  With out null only the size is computed. */
+fh_status fh_bc7_stats_format(const fh_bc7_block_stats *stats, uint64_t block_index, char *out,
+                              size_t out_capacity, size_t *out_size);
 
 /* Decodes a width x height surface of any format to RGBA8 (host buffers:
  * in holds ceil(w/4) * ceil(h/4) blocks, out w * h * 4 bytes), bit-exact

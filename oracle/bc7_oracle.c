@@ -1600,6 +1600,29 @@ int ora_bc7_exhaustive_label(const uint32_t px[16], uint64_t *err_out) {
 }
 
 
+/* The quality guard (not in the reference): with quality > 0, the block of
+ * the same selection at quality 0 replaces the annealed one when it decodes
+ * to a strictly smaller premultiplied error.  Returns 1 if it did. */
+static int q0_guard(const shape_sel *sel, const uint32_t px[16], const ora_bc7_settings *s,
+                    uint64_t block_id, uint8_t out[16]) {
+  if (!s->q0_guard || s->quality == 0) return 0;
+  ora_bc7_settings s0 = *s;
+  s0.quality = 0;
+  uint8_t out0[16];
+  memset(out0, 0, 16);
+  compress_clusters(sel, px, &s0, block_id, out0, NULL);
+  if (memcmp(out0, out, 16) == 0) return 0;
+  uint32_t d0[16], d1[16];
+  decode_block(out0, d0, 0);
+  decode_block(out, d1, 0);
+  if (ora_block_premul_err(px, d0) < ora_block_premul_err(px, d1)) {
+    memcpy(out, out0, 16);
+    return 1;
+  }
+  return 0;
+}
+
+
 void ora_bc7_compress_block(const uint32_t px[16], uint8_t out[16], const ora_bc7_settings *s,
                             uint32_t wm_index, uint64_t block_id, ora_bc7_block_stats *st) {
   if (st) {
@@ -1628,23 +1651,8 @@ void ora_bc7_compress_block(const uint32_t px[16], uint8_t out[16], const ora_bc
   shape_sel sel = box_selection(px);
   sel.modes &= s->block_modes;
   compress_clusters(&sel, px, s, block_id, out, st);
-  if (st) st->q0_kept = 0;
-  if (s->q0_guard && s->quality > 0) {
-    ora_bc7_settings s0 = *s;
-    s0.quality = 0;
-    uint8_t out0[16];
-    memset(out0, 0, 16);
-    compress_clusters(&sel, px, &s0, block_id, out0, NULL);
-    if (memcmp(out0, out, 16) != 0) {
-      uint32_t d0[16], d1[16];
-      decode_block(out0, d0, 0);
-      decode_block(out, d1, 0);
-      if (ora_block_premul_err(px, d0) < ora_block_premul_err(px, d1)) {
-        memcpy(out, out0, 16);
-        if (st) st->q0_kept = 1;
-      }
-    }
-  }
+  const int kept = q0_guard(&sel, px, s, block_id, out);
+  if (st) st->q0_kept = kept;
 }
 
 /* BPTCC::Compress (Compressor.cpp:1473-1520) with GetBlock (1460-1466). */
@@ -1667,6 +1675,198 @@ void ora_bc7_compress_job(const uint8_t *in, uint8_t *out, uint32_t width, uint3
         wm = *wm_counter;
       }
       ora_bc7_compress_block(blk, o, s, wm, (uint64_t)(j / 4) * bw + i / 4, NULL);
+      o += 16;
+    }
+    sx = 0;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* CompressWithStats (Compressor.cpp:1577-1623, 1862-2183).                 */
+/* ------------------------------------------------------------------------ */
+
+/* std::min(a, b): b only when strictly smaller. */
+static double min_ref(double a, double b) { return b < a ? b : a; }
+
+/* UpdateErrorEstimate (Compressor.cpp:1937-1944). */
+static void update_estimate(double *est, int mode, double e) {
+  if (est[mode] == -1.0 || e < est[mode]) est[mode] = e;
+}
+
+/* EstimateTwoClusterErrorStats (Compressor.cpp:1862-1898): QuantizedError
+ * of the subset between its bounding-box corners, quantised as mode 1 (mask
+ * 0xFFFCFCFC) and mode 3 (0xFFFEFEFE) endpoints; the estimates are the two
+ * errors, the result 0.0001 + the smaller; 0 for a one-colour subset. */
+static double est_two_stats(const clus *c, double se[2]) {
+  const v4 v = v4_sub(c->mx, c->mn);
+  if (v4_dot(v, v) == 0) {
+    se[0] = se[1] = 0.0;
+    return 0.0;
+  }
+  const double e1 = qerr(c, c->mn, c->mx, 8, 0xFFFCFCFCu, v4_splat(1.0f), NULL, NULL);
+  se[0] = e1 >= 0.0 ? e1 : min_ref(se[0], e1);
+  const double e3 = qerr(c, c->mn, c->mx, 8, 0xFFFEFEFEu, v4_splat(1.0f), NULL, NULL);
+  se[1] = e3 >= 0.0 ? e3 : min_ref(se[1], e3);
+  double error = 0.0001;
+  error += min_ref(e1, e3);
+  return error;
+}
+
+/* EstimateThreeClusterErrorStats (Compressor.cpp:1900-1935): as above with
+ * mode 0 (0xFFF0F0F0) and mode 2 (0xFFF8F8F8) endpoints and four index
+ * values; here the estimates already include the 0.0001. */
+static double est_three_stats(const clus *c, double se[2]) {
+  const v4 v = v4_sub(c->mx, c->mn);
+  if (v4_dot(v, v) == 0) {
+    se[0] = se[1] = 0.0;
+    return 0.0;
+  }
+  const double e0 = 0.0001 + qerr(c, c->mn, c->mx, 4, 0xFFF0F0F0u, v4_splat(1.0f), NULL, NULL);
+  se[0] = e0 >= 0.0 ? e0 : min_ref(se[0], e0);
+  const double e2 = 0.0001 + qerr(c, c->mn, c->mx, 4, 0xFFF8F8F8u, v4_splat(1.0f), NULL, NULL);
+  se[1] = e2 >= 0.0 ? e2 : min_ref(se[1], e2);
+  double error = 0.0001;
+  error += min_ref(e0, e2);
+  return error;
+}
+
+/* The shape scan of the statistics block path (Compressor.cpp:2063-2176). */
+static shape_sel stats_selection(const uint32_t px[16], ora_bc7_log_stats *ls) {
+  shape_sel r;
+  r.modes = 0xFF;
+  r.sh_idx[0] = r.sh_idx[1] = 0;
+  r.sh_parts[0] = r.sh_parts[1] = 0;
+  clus c;
+  clus_init(&c, px);
+  int opaque = 1;
+  for (uint32_t i = 0; i < c.n; i++)
+    if (fabs(c.pts[i].v[3] - 255.0f) > 1e-10) opaque = 0;
+  /* mode 6's estimate: the whole block as one line (:2046-2061) */
+  {
+    const v4 v = v4_sub(c.mx, c.mn);
+    if (v4_dot(v, v) == 0) {
+      ls->mode_estimate[6] = 0.0;
+    } else {
+      const double e =
+          0.0001 + qerr(&c, c.mn, c.mx, 4, 0xFEFEFEFEu, v4_splat(1.0f), NULL, NULL);
+      update_estimate(ls->mode_estimate, 6, e);
+    }
+  }
+  double best[2] = {DBL_MAX, DBL_MAX};
+  int path = 0;
+  r.nshapes = 1;
+  for (int i = 0; i < 64; i++) {
+    c.shape = (uint32_t)i;
+    c.nparts = 2;
+    double err = 0.0, est[2] = {-1.0, -1.0};
+    for (int ci = 0; ci < 2; ci++) {
+      clus_set_partition(&c, (uint32_t)ci);
+      double se[2] = {-1.0, -1.0};
+      err += est_two_stats(&c, se);
+      for (int ei = 0; ei < 2; ei++)
+        if (se[ei] >= 0.0) est[ei] = est[ei] == -1.0 ? se[ei] : est[ei] + se[ei];
+    }
+    if (est[0] != -1.0) update_estimate(ls->mode_estimate, 1, est[0]);
+    if (est[1] != -1.0) update_estimate(ls->mode_estimate, 3, est[1]);
+    if (err < best[0]) {
+      ls->two_shape[ls->num_two_shape++] = err;
+      best[0] = err;
+      r.sh_idx[0] = i;
+      r.sh_parts[0] = 2;
+    }
+    if (err < 1e-9) {
+      path = 2;
+      r.modes = kTwoPartModes;
+      break;
+    }
+  }
+  /* the three-subset scan runs for opaque blocks even after the two-subset
+   * scan stopped early */
+  if (opaque) {
+    r.nshapes++;
+    for (int i = 0; i < 64; i++) {
+      c.shape = (uint32_t)i;
+      c.nparts = 3;
+      double err = 0.0, est[2] = {-1.0, -1.0};
+      for (int ci = 0; ci < 3; ci++) {
+        clus_set_partition(&c, (uint32_t)ci);
+        double se[2] = {-1.0, -1.0};
+        err += est_three_stats(&c, se);
+        for (int ei = 0; ei < 2; ei++)
+          if (se[ei] >= 0.0) est[ei] = est[ei] == -1.0 ? se[ei] : est[ei] + se[ei];
+      }
+      if (est[0] != -1.0) update_estimate(ls->mode_estimate, 0, est[0]);
+      if (est[1] != -1.0) update_estimate(ls->mode_estimate, 2, est[1]);
+      if (err < best[1]) {
+        ls->three_shape[ls->num_three_shape++] = err;
+        best[1] = err;
+        r.sh_idx[1] = i;
+        r.sh_parts[1] = 3;
+      }
+      if (err < 1e-9) {
+        path = 2;
+        r.modes = kThreePartModes;
+        break;
+      }
+    }
+  }
+  ls->path = path == 0 ? 3 : path;
+  return r;
+}
+
+static int written_mode(const uint8_t blk[16]) {
+  for (int m = 0; m < 8; m++)
+    if ((blk[0] >> m) & 1) return m;
+  return 8;
+}
+
+void ora_bc7_compress_block_logged(const uint32_t px[16], uint8_t out[16],
+                                   const ora_bc7_settings *s, uint32_t wm_index,
+                                   uint64_t block_id, ora_bc7_log_stats *ls) {
+  memset(ls, 0, sizeof(*ls));
+  for (int i = 0; i < 8; i++) ls->mode_estimate[i] = ls->mode_error[i] = -1.0;
+  memset(out, 0, 16);
+  if (all_one_color(px)) {
+    write_solid(px[0], out, wm_index);
+    ls->path = 0;
+  } else {
+    int transparent = 1;
+    for (int i = 0; i < 16; i++)
+      if ((float)((px[i] >> 24) & 0xFF) > 0.0f) transparent = 0;
+    if (transparent) {
+      write_transparent(out);
+      ls->path = 1;
+    } else {
+      shape_sel sel = stats_selection(px, ls);
+      sel.modes &= s->block_modes;
+      compress_clusters(&sel, px, s, block_id, out, NULL);
+      q0_guard(&sel, px, s, block_id, out);
+    }
+  }
+  ls->mode = written_mode(out);
+}
+
+void ora_bc7_compress_job_logged(const uint8_t *in, uint8_t *out, uint32_t width,
+                                 uint32_t height, uint32_t x_start, uint32_t y_start,
+                                 uint32_t x_end, uint32_t y_end, const ora_bc7_settings *s,
+                                 uint32_t *wm_counter, ora_bc7_log_stats *stats) {
+  const uint32_t *pix = (const uint32_t *)in;
+  const uint32_t bw = width / 4;
+  uint8_t *o = out + ((y_start / 4) * bw + x_start / 4) * 16;
+  const uint32_t end_y = y_end < height - 4 ? y_end : height - 4;
+  uint32_t sx = x_start;
+  size_t k = 0;
+  for (uint32_t j = y_start; j <= end_y; j += 4) {
+    const uint32_t end_x = j == y_end ? x_end : width;
+    for (uint32_t i = sx; i < end_x; i += 4) {
+      uint32_t blk[16];
+      for (int r = 0; r < 4; r++) memcpy(blk + 4 * r, pix + (j + r) * width + i, 16);
+      uint32_t wm = 0;
+      if (all_one_color(blk)) {
+        *wm_counter = (*wm_counter + 1) % 9;
+        wm = *wm_counter;
+      }
+      ora_bc7_compress_block_logged(blk, o, s, wm, (uint64_t)(j / 4) * bw + i / 4, &stats[k++]);
       o += 16;
     }
     sx = 0;

@@ -88,6 +88,45 @@ void ora_bc7_compress_job(const uint8_t *in, uint8_t *out, uint32_t width,
                           uint32_t x_end, uint32_t y_end,
                           const ora_bc7_settings *s, uint32_t *wm_counter);
 
+/* One block's statistics from BPTCC::CompressWithStats, in the vocabulary
+ * of its log (EBlockStats, Compressor.cpp:77-131; the statistics version of
+ * CompressBC7Block, :1954-2183).  Same layout as fh_bc7_block_stats.
+ *  path            BlockStat_Path: 0 solid, 1 transparent, 2 a shape scan
+ *                  stopped at an estimate below 1e-9, 3 otherwise
+ *  mode            BlockStat_Mode: the mode of the block written
+ *  num_two_shape   how many BlockStat_TwoShapeEstimate lines (one per strict
+ *                  improvement of the two-subset scan), values in two_shape
+ *  num_three_shape the same for BlockStat_ThreeShapeEstimate
+ *  mode_estimate   BlockStat_Mode<N>Estimate, -1 when never set
+ *  mode_error      BlockStat_Mode<N>Error: always -1, since CompressClusters
+ *                  keeps std::min(error, -1.0) (Compressor.cpp:1800-1801) */
+typedef struct {
+  int32_t path, mode;
+  uint32_t num_two_shape, num_three_shape;
+  double mode_estimate[8];
+  double mode_error[8];
+  double two_shape[64];
+  double three_shape[64];
+} ora_bc7_log_stats;
+
+/* The statistics version of CompressBC7Block (Compressor.cpp:1954-2183):
+ * its own shape scan (EstimateTwo/ThreeClusterErrorStats, :1862-1935; the
+ * opaque test is A == 255, modes 4 and 5 stay selected for opaque blocks),
+ * then CompressClusters and, as in ora_bc7_compress_block, the optional
+ * quality guard. */
+void ora_bc7_compress_block_logged(const uint32_t px[16], uint8_t out[16],
+                                   const ora_bc7_settings *s, uint32_t wm_index,
+                                   uint64_t block_id, ora_bc7_log_stats *ls);
+
+/* BPTCC::CompressWithStats over a (sub-)job (Compressor.cpp:1577-1623):
+ * the loop of ora_bc7_compress_job; stats[k] receives the k-th block
+ * processed. */
+void ora_bc7_compress_job_logged(const uint8_t *in, uint8_t *out, uint32_t width,
+                                 uint32_t height, uint32_t x_start, uint32_t y_start,
+                                 uint32_t x_end, uint32_t y_end,
+                                 const ora_bc7_settings *s, uint32_t *wm_counter,
+                                 ora_bc7_log_stats *stats);
+
 /* BPTCC::Decompress (Decompressor.cpp:345-367), including the reference's
  * treatment of mode 4 with index mode 1 (Decompressor.cpp:263-295 interpolate
  * with the per-mode bit counts, not the swapped ones). */

@@ -32,6 +32,14 @@ class BlockStats(ctypes.Structure):
                 ("q0_kept", ctypes.c_int)]
 
 
+# ora_bc7_log_stats / fh_bc7_block_stats
+LOG_STATS_DTYPE = np.dtype([("path", "<i4"), ("mode", "<i4"), ("num_two_shape", "<u4"),
+                            ("num_three_shape", "<u4"), ("mode_estimate", "<f8", (8,)),
+                            ("mode_error", "<f8", (8,)), ("two_shape", "<f8", (64,)),
+                            ("three_shape", "<f8", (64,))])
+STAT_NAMES = ["Zero", "One", "Two", "Three", "Four", "Five", "Six", "Seven"]
+
+
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
@@ -50,6 +58,8 @@ def lib():
                                              ctypes.c_uint64, ctypes.POINTER(BlockStats)]
         L.ora_bc7_compress_job.argtypes = [vp, vp] + [ctypes.c_uint32] * 6 + [
             ctypes.POINTER(Settings), ctypes.POINTER(ctypes.c_uint32)]
+        L.ora_bc7_compress_job_logged.argtypes = [vp, vp] + [ctypes.c_uint32] * 6 + [
+            ctypes.POINTER(Settings), ctypes.POINTER(ctypes.c_uint32), vp]
         L.ora_bc7_decompress.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32]
         L.ora_bc7_decode_block_spec.argtypes = [vp, vp]
         L.ora_psnr.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32]
@@ -113,6 +123,44 @@ def bc7_compress(img, quality=0, seed=0, rng=RNG_COUNTER, lcg_seed=0, block_mode
     lib().ora_bc7_compress_job(_p(img), _p(out), w, h, x0, y0, x1, y1, ctypes.byref(s),
                                ctypes.byref(wm))
     return out, wm.value
+
+
+def bc7_compress_logged(img, quality=0, seed=0, rng=RNG_COUNTER, lcg_seed=0, block_modes=0xFF,
+                        rect=None, wm_counter=0xFFFFFFFF, q0_guard=False):
+    """BPTCC::CompressWithStats over an image or sub-rectangle.  Returns
+    (bytes, wm, stats), stats a LOG_STATS_DTYPE array, one record per block
+    processed."""
+    img = np.ascontiguousarray(img, dtype=np.uint32)
+    h, w = img.shape
+    out = np.zeros((h // 4) * (w // 4) * 16, dtype=np.uint8)
+    stats = np.zeros((h // 4) * (w // 4), dtype=LOG_STATS_DTYPE)
+    lcg = ctypes.c_uint32(lcg_seed)
+    s = Settings(quality, block_modes, rng, ctypes.pointer(lcg), seed, int(q0_guard))
+    wm = ctypes.c_uint32(wm_counter)
+    x0, y0, x1, y1 = rect if rect is not None else (0, 0, w, h)
+    lib().ora_bc7_compress_job_logged(_p(img), _p(out), w, h, x0, y0, x1, y1, ctypes.byref(s),
+                                      ctypes.byref(wm), _p(stats))
+    return out, wm.value, stats
+
+
+def format_block_stats(rec, block_index):
+    """The lines BlockLogger writes for one block (Compressor.cpp:1373-1390,
+    1945-1950, 1954-2183): "<index>: BlockStat_<name> -- <value>", values
+    written by an ostream with its default format (printf %g for doubles)."""
+    def line(name, v):
+        v = ("%d" % v) if isinstance(v, (int, np.integer)) else ("%g" % v)
+        return "%d: BlockStat_%s -- %s\n" % (block_index, name, v)
+    out = []
+    for k in range(int(rec["num_two_shape"])):
+        out.append(line("TwoShapeEstimate", float(rec["two_shape"][k])))
+    for k in range(int(rec["num_three_shape"])):
+        out.append(line("ThreeShapeEstimate", float(rec["three_shape"][k])))
+    out.append(line("Path", int(rec["path"])))
+    out.append(line("Mode", int(rec["mode"])))
+    for m in range(8):
+        out.append(line("Mode%sEstimate" % STAT_NAMES[m], float(rec["mode_estimate"][m])))
+        out.append(line("Mode%sError" % STAT_NAMES[m], float(rec["mode_error"][m])))
+    return "".join(out)
 
 
 def bc7_compress_block(px16, quality=0, seed=0, rng=RNG_COUNTER, lcg_seed=0, wm_index=0,
