@@ -64,10 +64,11 @@ def parse():
     ap.add_argument("--topk", type=int, default=4)
     ap.add_argument("--threshold", type=float, default=0.6)
     ap.add_argument("--workload", default="bc7",
-                    choices=["bc7", "batch", "dxt1", "dxt5", "etc1", "decode"],
+                    choices=["bc7", "batch", "dxt1", "dxt5", "etc1", "decode", "stats"],
                     help="bc7: the headline BC7 q=50 4096^2 line (default); batch: the "
                          "512-texture mixed batch; dxt1/dxt5/etc1: HBM-bound encodes; "
-                         "decode: decoders and fused error reductions, all formats")
+                         "decode: decoders and fused error reductions, all formats; "
+                         "stats: BC7 CompressWithStats (statistics search and records)")
     ap.add_argument("--textures", type=int, default=512)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a device-to-device copy of 512 MiB (3 times), the "
@@ -275,6 +276,74 @@ def run_decode(args):
     print(json.dumps(line), flush=True)
 
 
+def run_stats(args):
+    """BPTCC::CompressWithStats on the GPU (fh_compress_with_stats_device) on
+    the headline image (4096x4096, q=50, seed 3): the statistics shape scan,
+    CompressClusters over its candidates (modes 4 and 5 on opaque blocks too,
+    each single-subset mode once per shape entry), annealing, packing and one
+    1168-byte BlockStat record per block, all device-resident."""
+    import torch
+    import fastc_amd
+    from fastc_amd import synth
+
+    fastc_amd.lib()
+    n = args.size
+    img = synth.generate(n, n, args.seed)
+    d_img = torch.from_numpy(img.view(np.int32)).cuda()
+    nb = n * n // 16
+    enc = torch.zeros(n * n, dtype=torch.uint8, device="cuda")
+    rec = torch.zeros(nb * fastc_amd.BLOCK_STATS_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+
+    def step():
+        fastc_amd.compress_with_stats_device(d_img.data_ptr(), enc.data_ptr(), enc.numel(),
+                                             rec.data_ptr(), nb, n, n, quality=args.quality,
+                                             seed=args.seed, stream=sh)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    print("[bench] stats timed", file=sys.stderr, flush=True)
+    fastc_amd.set_kernel_timing(True)
+    step()
+    torch.cuda.synchronize()
+    kernels = {}
+    for k in ("bc7_scan", "bc7_select", "bc7_setup", "bc7_anneal", "bc7_pack", "bc7_guard"):
+        tot, launches = fastc_amd.kernel_time(k)
+        if launches:
+            kernels[k] = tot
+    fastc_amd.set_kernel_timing(False)
+    blk = torch.zeros(nb, dtype=torch.int64, device="cuda")
+    sse = torch.zeros(2, dtype=torch.int64, device="cuda")
+    fastc_amd.block_error_device(d_img.data_ptr(), enc.data_ptr(), n, n, sse.data_ptr(),
+                                 blk.data_ptr(), fmt=3, stream=sh)
+    st = rec.cpu().numpy().view(fastc_amd.BLOCK_STATS_DTYPE)
+    psnr_db = fastc_amd.psnr_from_sse(sse.cpu().numpy().view(np.uint64), n, n)
+    line = {
+        "metric": "BC7 CompressWithStats Mtexels/s (4096², q=50)",
+        "value": n * n / (ms / 1e3) / 1e6, "unit": "Mtexels/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"BC7 CompressWithStats q={args.quality}, {n}x{n} RGBA8, "
+                               f"seed {args.seed}, records on the device"},
+        "psnr_db": psnr_db, "kernels_ms": kernels,
+        "records": {"bytes": int(rec.numel()),
+                    "path": np.bincount(st["path"], minlength=4).tolist(),
+                    "mode": np.bincount(st["mode"], minlength=9).tolist(),
+                    "mean_shape_lines": float((st["num_two_shape"] +
+                                               st["num_three_shape"]).mean())},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def run_simple(args, fmt):
     """DXT1 / DXT5 / ETC1 on a ring of 16 distinct 2048x2048 images (seeds
     2..17, 5% hard edges): one batch graph of 16 surfaces, so the working set
@@ -344,6 +413,8 @@ def main():
             run_batch(args)
         elif args.workload == "decode":
             run_decode(args)
+        elif args.workload == "stats":
+            run_stats(args)
         else:
             run_simple(args, {"dxt1": 0, "dxt5": 1, "etc1": 2}[args.workload])
         return

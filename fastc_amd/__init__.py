@@ -150,11 +150,14 @@ def lib():
         L.fh_ktx_write.argtypes = [u32, u32, u32, u32, ctypes.POINTER(ctypes.c_void_p), vp,
                                    ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         L.fh_compress_with_stats.argtypes = [ctypes.POINTER(FhJob), vp, ctypes.c_size_t]
+        L.fh_compress_with_stats_device.argtypes = [ctypes.POINTER(FhJob), vp, ctypes.c_size_t,
+                                                    vp]
         L.fh_bc7_stats_format.argtypes = [vp, ctypes.c_uint64, vp, ctypes.c_size_t,
                                           ctypes.POINTER(ctypes.c_size_t)]
         for name in ("fh_decompress_device", "fh_block_error_device", "fh_image_error_device",
                      "fh_psnr", "fh_mip_level_device", "fh_pad4_device", "fh_ktx_write",
-                     "fh_compress_with_stats", "fh_bc7_stats_format"):
+                     "fh_compress_with_stats", "fh_compress_with_stats_device",
+                     "fh_bc7_stats_format"):
             getattr(L, name).restype = ctypes.c_int
         for name in ("fh_batch_create", "fh_batch_destroy", "fh_batch_run", "fh_batch_autotune",
                      "fh_batch_blocks", "fh_batch_set_tune"):
@@ -177,7 +180,8 @@ EXPORTED_SYMBOLS = ["fh_compress", "fh_compress_device", "fh_decompress",
                     "fh_batch_blocks", "fh_batch_set_tune", "fh_decompress_device",
                     "fh_block_error_device", "fh_image_error_device", "fh_psnr_from_sse",
                     "fh_psnr", "fh_mip_level_device", "fh_pad4_device", "fh_ktx_write",
-                    "fh_compress_with_stats", "fh_bc7_stats_format"]
+                    "fh_compress_with_stats", "fh_compress_with_stats_device",
+                    "fh_bc7_stats_format"]
 
 # fh_bc7_block_stats (include/fasthip.h)
 BLOCK_STATS_DTYPE = np.dtype([("path", "<i4"), ("mode", "<i4"), ("num_two_shape", "<u4"),
@@ -264,6 +268,18 @@ def compress_with_stats(img, quality=0, seed=0, block_modes=0xFF, rect=None, wm_
              quality, seed, block_modes, wm_counter, q0_guard, None, 4, 0.6)
     _check(lib().fh_compress_with_stats(ctypes.byref(j), stats.ctypes.data, stats.size))
     return out, stats
+
+
+def compress_with_stats_device(in_ptr, out_ptr, out_size, stats_ptr, stats_count, width,
+                               height, quality=0, seed=0, block_modes=0xFF, stream=0,
+                               wm_counter=None, q0_guard=True):
+    """fh_compress_with_stats_device: device pointers for the image, the
+    blocks and `stats_count` BLOCK_STATS_DTYPE records; asynchronous on
+    `stream`."""
+    j = _job(eCompressionFormat_BPTC, in_ptr, out_ptr, out_size, width, height, None, quality,
+             seed, block_modes, wm_counter, q0_guard, None, 4, 0.6)
+    _check(lib().fh_compress_with_stats_device(ctypes.byref(j), ctypes.c_void_p(stats_ptr),
+                                               stats_count, ctypes.c_void_p(stream)))
 
 
 def stats_format(stats, block_index):

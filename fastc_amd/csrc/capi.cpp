@@ -275,6 +275,15 @@ int current_device(fh_status *s) {
   return dev;
 }
 
+// The statistics entry points take BPTC jobs of the reference search only.
+fh_status stats_job_ok(const fh_job *j) {
+  if (j && j->format != FH_FORMAT_BPTC && block_bytes(j->format) != 0)
+    return fail(FH_ERR_UNSUPPORTED, "statistics are only collected for BPTC");
+  if (j && (j->flags & FH_FLAG_LEARNED))
+    return fail(FH_ERR_INVALID_ARGUMENT, "statistics are not collected by the learned search");
+  return FH_OK;
+}
+
 // fh_compress and fh_compress_with_stats on host buffers: stage the image,
 // encode, copy the blocks (and statistics) back.  Caller-facing checks as in
 // CompressImageData.
@@ -348,11 +357,30 @@ extern "C" {
 fh_status fh_compress(const fh_job *j) { return compress_host(j, false, nullptr, 0); }
 
 fh_status fh_compress_with_stats(const fh_job *j, fh_bc7_block_stats *stats, size_t stats_count) {
-  if (j && j->format != FH_FORMAT_BPTC && block_bytes(j->format) != 0)
-    return fail(FH_ERR_UNSUPPORTED, "statistics are only collected for BPTC");
-  if (j && (j->flags & FH_FLAG_LEARNED))
-    return fail(FH_ERR_INVALID_ARGUMENT, "statistics are not collected by the learned search");
+  const fh_status s = stats_job_ok(j);
+  if (s != FH_OK) return s;
   return compress_host(j, true, stats, stats_count);
+}
+
+fh_status fh_compress_with_stats_device(const fh_job *j, fh_bc7_block_stats *stats,
+                                        size_t stats_count, void *hip_stream) {
+  fh_status s = stats_job_ok(j);
+  if (s != FH_OK) return s;
+  s = validate(j, true);
+  if (s != FH_OK) return s;
+  uint32_t first = 0, count = 0;
+  s = job_range(j, &first, &count);
+  if (s != FH_OK) return s;
+  if (count && (!stats || stats_count < count))
+    return fail(FH_ERR_BUFFER_TOO_SMALL, "statistics buffer too small");
+  const int dev = current_device(&s);
+  if (dev < 0) return s;
+  DeviceState *st = device_state(dev);
+  std::lock_guard<std::mutex> g(st->mu);
+  hipError_t e = ensure_init(st);
+  if (e != hipSuccess) return fail_hip(e, "device initialisation");
+  return run_device(st, j, j->in, j->out, reinterpret_cast<hipStream_t>(hip_stream), first,
+                    count, stats);
 }
 
 fh_status fh_bc7_stats_format(const fh_bc7_block_stats *st, uint64_t block_index, char *out,

@@ -131,3 +131,24 @@ def test_stats_1024_properties(fh, oracle):
         assert np.array_equal(got[16 * k:16 * k + 16], ref)
         for f in st.dtype.names:
             assert np.array_equal(st[f][k], rst[f][0]), (k, f)
+
+
+def test_stats_device_entry_point(fh, oracle):
+    """fh_compress_with_stats_device on torch buffers equals the oracle, and
+    rejects a short record buffer."""
+    import torch
+    img = _near_opaque(synth.generate(128, 64, 8), 8)
+    print("stats device entry point", flush=True)
+    ref, _, rst = oracle.bc7_compress_logged(img, quality=4, seed=2, q0_guard=True)
+    d_img = torch.from_numpy(img.view(np.int32)).cuda()
+    enc = torch.zeros(128 * 64, dtype=torch.uint8, device="cuda")
+    nb = 128 * 64 // 16
+    rec = torch.full((nb * fh.BLOCK_STATS_DTYPE.itemsize,), 0x7F, dtype=torch.uint8,
+                     device="cuda")
+    fh.compress_with_stats_device(d_img.data_ptr(), enc.data_ptr(), enc.numel(), rec.data_ptr(),
+                                  nb, 128, 64, quality=4, seed=2, wm_counter=_wm())
+    torch.cuda.synchronize()
+    _compare(enc.cpu().numpy(), ref, rec.cpu().numpy().view(fh.BLOCK_STATS_DTYPE), rst)
+    with pytest.raises(fh.FastHipError):
+        fh.compress_with_stats_device(d_img.data_ptr(), enc.data_ptr(), enc.numel(),
+                                      rec.data_ptr(), nb - 1, 128, 64, wm_counter=_wm())
