@@ -149,7 +149,10 @@ typedef struct fh_bc7_block_stats {
  * blocks (each single-subset mode then runs once per shape entry).  So the
  * bytes differ from fh_compress's.  stats[k] receives the k-th block the job
  * processes (raster order from (x_start, y_start)); stats_count must cover
- * them.  FH_FLAG_LEARNED is rejected; other formats return
+ * them.  Rows follow fh_compress's rule (Compressor.cpp:1478-1482): the
+ * reference's statistics loop (:1583) lacks the min(YEnd, Height - 4) clamp
+ * and would run one row past a whole-image job, out of bounds.
+ * FH_FLAG_LEARNED is rejected; other formats return
  * FH_ERR_UNSUPPORTED, as the reference only collects BPTC statistics
  * (Core/src/TexComp.cpp:87-108). */
 fh_status fh_compress_with_stats(const fh_job *job, fh_bc7_block_stats *stats,

@@ -8,6 +8,7 @@
 //   fasthip::CompressImageData           Core/include/FasTC/TexComp.h:81-88
 //   fasthip::BPTCC::CompressionSettings  BPTCEncoder/include/FasTC/BPTCCompressor.h:123-158
 //   fasthip::BPTCC::Compress/Decompress  BPTCCompressor.h:166-167, 216
+//   fasthip::BPTCC::CompressWithStats    BPTCCompressor.h:173-174
 //   fasthip::DXTC::CompressImageDXT1/5   DXTEncoder/include/FasTC/DXTCompressor.h:24-25
 //   fasthip::ETCC::Compress_RG           ETCEncoder/include/FasTC/ETCCompressor.h:35
 // Everything lives in namespace fasthip so it can be linked next to FasTC;
@@ -24,8 +25,10 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <ostream>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../fasthip.h"
 
@@ -126,6 +129,18 @@ inline fh_job make_job(const FasTC::CompressionJob &cj) {
   return j;
 }
 
+// Blocks a job processes, by Compress's row rule (Compressor.cpp:1478-1482).
+inline uint32_t job_blocks(const FasTC::CompressionJob &cj) {
+  if (cj.Height() < 4) return 0;
+  const uint32_t end_y = cj.YEnd() < cj.Height() - 4 ? cj.YEnd() : cj.Height() - 4;
+  uint32_t n = 0, sx = cj.XStart();
+  for (uint32_t y = cj.YStart(); y <= end_y; y += 4, sx = 0) {
+    const uint32_t end_x = y == cj.YEnd() ? cj.XEnd() : cj.Width();
+    if (end_x > sx) n += (end_x - sx + 3) / 4;
+  }
+  return n;
+}
+
 }  // namespace detail
 
 namespace BPTCC {
@@ -149,6 +164,31 @@ inline void Compress(const FasTC::CompressionJob &cj,
   j.error_metric = (uint32_t)settings.m_ErrorMetric;
   j.seed = settings.m_Seed;
   detail::check(fh_compress(&j));
+}
+
+// BPTCC::CompressWithStats (BPTCCompressor.h:173-174, Compressor.cpp:1577-1623):
+// the statistics search, and with a log stream the BlockLogger lines of
+// every block (fh_bc7_stats_format).  Rows follow Compress's rule: the
+// reference's statistics loop has no min(YEnd, Height - 4) clamp
+// (Compressor.cpp:1583) and would run a row past a whole-image job.
+inline void CompressWithStats(const FasTC::CompressionJob &cj, std::ostream *logStream,
+                              CompressionSettings settings = CompressionSettings()) {
+  fh_job j = detail::make_job(cj);
+  j.quality = settings.m_NumSimulatedAnnealingSteps;
+  j.block_modes = settings.m_BlockModes;
+  j.error_metric = (uint32_t)settings.m_ErrorMetric;
+  j.seed = settings.m_Seed;
+  const uint32_t n = detail::job_blocks(cj);
+  std::vector<fh_bc7_block_stats> st(n > 0 ? n : 1);
+  detail::check(fh_compress_with_stats(&j, st.data(), st.size()));
+  if (!logStream) return;
+  std::vector<char> text(16384);
+  uint64_t idx = cj.CoordsToBlockIdx(cj.XStart(), cj.YStart());
+  for (uint32_t k = 0; k < n; k++, idx++) {
+    size_t len = 0;
+    detail::check(fh_bc7_stats_format(&st[k], idx, text.data(), text.size(), &len));
+    logStream->write(text.data(), (std::streamsize)len);
+  }
 }
 
 inline void Decompress(const FasTC::DecompressionJob &dj) {
@@ -193,6 +233,7 @@ struct SCompressionSettings {
   bool bUsePVRTexLib = false;
   bool bUseNVTT = false;
   bool bUseGPU = true;   // route the encode to the MI355X kernels
+  std::ostream *logStream = nullptr;  // BPTC: CompressWithStats and its log
   uint64_t seed = 0;     // annealing stream key
 };
 
@@ -222,11 +263,22 @@ inline bool CompressImageData(const unsigned char *data, unsigned int width, uns
   j.quality = (uint32_t)(q < 0 ? 0 : (q > 256 ? 256 : q));
   j.block_modes = 0xFF;
   j.seed = settings.seed;
+  // with a log stream BPTC runs CompressWithStats (TexComp.cpp:165-181);
+  // the reference has no statistics for the other formats and encodes them
+  const bool stats = settings.logStream && settings.format == FasTC::eCompressionFormat_BPTC;
+  const uint32_t nblocks = (width / 4) * (height / 4);
+  std::vector<fh_bc7_block_stats> st(stats && nblocks ? nblocks : 0);
   for (int i = 0; i < (settings.iNumCompressions > 0 ? settings.iNumCompressions : 1); i++) {
-    const fh_status s = fh_compress(&j);
+    const fh_status s = stats ? fh_compress_with_stats(&j, st.data(), st.size()) : fh_compress(&j);
     if (s != FH_OK) {
       fprintf(stderr, "TexComp -- %s\n", fh_last_error());
       return false;
+    }
+    std::vector<char> text(16384);
+    for (size_t k = 0; k < st.size(); k++) {
+      size_t len = 0;
+      if (fh_bc7_stats_format(&st[k], k, text.data(), text.size(), &len) == FH_OK)
+        settings.logStream->write(text.data(), (std::streamsize)len);
     }
   }
   return true;

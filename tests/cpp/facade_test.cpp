@@ -3,10 +3,14 @@
 //   facade_test validate                 -> checks error behaviour (no GPU needed)
 //   facade_test encode in.raw w h outdir -> BPTC(q=0), DXT1, DXT5, ETC1 via the
 //                                           codec entry points and CompressImageData
+//   facade_test stats in.raw w h outdir  -> BPTCC::CompressWithStats (q=0) on a
+//                                           sub-job and CompressImageData with a
+//                                           log stream: blocks and log text
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -75,10 +79,38 @@ static int encode(const char *in, unsigned w, unsigned h, const std::string &dir
   return 0;
 }
 
+static int stats(const char *in, unsigned w, unsigned h, const std::string &dir) {
+  std::vector<uint8_t> img((size_t)w * h * 4);
+  FILE *f = fopen(in, "rb");
+  if (!f || fread(img.data(), 1, img.size(), f) != img.size()) return 2;
+  fclose(f);
+  std::vector<uint8_t> sub((size_t)w * h, 0), whole((size_t)w * h, 0);
+  BPTCC::CompressionSettings bs;
+  bs.m_NumSimulatedAnnealingSteps = 0;
+  std::ostringstream log_sub, log_whole;
+  BPTCC::CompressWithStats(FasTC::CompressionJob(FasTC::eCompressionFormat_BPTC, img.data(),
+                                                 sub.data(), w, h, 8, 4, 40, 48),
+                           &log_sub, bs);
+  SCompressionSettings s;
+  s.format = FasTC::eCompressionFormat_BPTC;
+  s.iQuality = 0;
+  s.logStream = &log_whole;
+  if (!CompressImageData(img.data(), w, h, whole.data(), (unsigned)whole.size(), s)) return 3;
+  save(dir + "/stats_sub.bin", sub);
+  save(dir + "/stats_whole.bin", whole);
+  const std::string a = log_sub.str(), b = log_whole.str();
+  save(dir + "/stats_sub.log", std::vector<uint8_t>(a.begin(), a.end()));
+  save(dir + "/stats_whole.log", std::vector<uint8_t>(b.begin(), b.end()));
+  printf("stats ok\n");
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc >= 2 && !strcmp(argv[1], "validate")) return validate();
   if (argc >= 6 && !strcmp(argv[1], "encode"))
     return encode(argv[2], (unsigned)atoi(argv[3]), (unsigned)atoi(argv[4]), argv[5]);
-  fprintf(stderr, "usage: facade_test validate | encode in.raw w h outdir\n");
+  if (argc >= 6 && !strcmp(argv[1], "stats"))
+    return stats(argv[2], (unsigned)atoi(argv[3]), (unsigned)atoi(argv[4]), argv[5]);
+  fprintf(stderr, "usage: facade_test validate | encode|stats in.raw w h outdir\n");
   return 2;
 }

@@ -54,3 +54,25 @@ def test_facade_matches_c_abi(fh, facade_bin, tmp_path):
     dec_b = pyoracle.bc7_decompress(want, 64, 64)
     assert np.array_equal(dec_a, dec_b)
     assert np.array_equal(pyoracle.bc7_decompress(rd("bptc_via_cid.bin"), 64, 64), dec_b)
+
+
+@pytest.mark.gpu
+def test_facade_compress_with_stats(fh, oracle, facade_bin, tmp_path):
+    """BPTCC::CompressWithStats on a sub-job and CompressImageData with a log
+    stream: the blocks and the log text the oracle gives (watermark entries
+    aside, which follow the process-wide counter)."""
+    img = synth.generate(64, 64, 2)
+    raw = tmp_path / "in.raw"
+    img.tofile(raw)
+    r = subprocess.run([facade_bin, "stats", str(raw), "64", "64", str(tmp_path)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rd = lambda n: np.fromfile(tmp_path / n, dtype=np.uint8)
+    for name, rect, first in (("sub", (8, 4, 40, 48), 1 * 16 + 2), ("whole", None, 0)):
+        ref, _, rst = oracle.bc7_compress_logged(img, rect=rect)
+        got = rd(f"stats_{name}.bin")
+        assert np.array_equal(oracle.bc7_decompress(got, 64, 64),
+                              oracle.bc7_decompress(ref, 64, 64)), name
+        n = int(np.count_nonzero((rst["mode"] != 0) | (rst["path"] != 0)))
+        want = "".join(oracle.format_block_stats(rst[k], first + k) for k in range(n))
+        assert (tmp_path / f"stats_{name}.log").read_text() == want, name
