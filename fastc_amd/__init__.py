@@ -150,6 +150,7 @@ def lib():
         L.fh_ktx_write.argtypes = [u32, u32, u32, u32, ctypes.POINTER(ctypes.c_void_p), vp,
                                    ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         L.fh_compress_with_stats.argtypes = [ctypes.POINTER(FhJob), vp, ctypes.c_size_t]
+        L.fh_compress_with_selection.argtypes = [ctypes.POINTER(FhJob), vp, ctypes.c_size_t]
         L.fh_compress_with_stats_device.argtypes = [ctypes.POINTER(FhJob), vp, ctypes.c_size_t,
                                                     vp]
         L.fh_bc7_stats_format.argtypes = [vp, ctypes.c_uint64, vp, ctypes.c_size_t,
@@ -157,7 +158,7 @@ def lib():
         for name in ("fh_decompress_device", "fh_block_error_device", "fh_image_error_device",
                      "fh_psnr", "fh_mip_level_device", "fh_pad4_device", "fh_ktx_write",
                      "fh_compress_with_stats", "fh_compress_with_stats_device",
-                     "fh_bc7_stats_format"):
+                     "fh_bc7_stats_format", "fh_compress_with_selection"):
             getattr(L, name).restype = ctypes.c_int
         for name in ("fh_batch_create", "fh_batch_destroy", "fh_batch_run", "fh_batch_autotune",
                      "fh_batch_blocks", "fh_batch_set_tune"):
@@ -181,7 +182,13 @@ EXPORTED_SYMBOLS = ["fh_compress", "fh_compress_device", "fh_decompress",
                     "fh_block_error_device", "fh_image_error_device", "fh_psnr_from_sse",
                     "fh_psnr", "fh_mip_level_device", "fh_pad4_device", "fh_ktx_write",
                     "fh_compress_with_stats", "fh_compress_with_stats_device",
-                    "fh_bc7_stats_format"]
+                    "fh_bc7_stats_format", "fh_compress_with_selection"]
+
+# fh_shape_selection (include/fasthip.h): BPTCC::ShapeSelection
+SHAPE_SELECTION_DTYPE = np.dtype([("num_shapes", "<u4"),
+                                  ("shapes", [("num_partitions", "<u4"), ("index", "<u4")],
+                                   (10,)),
+                                  ("selected_modes", "<u4")])
 
 # fh_bc7_block_stats (include/fasthip.h)
 BLOCK_STATS_DTYPE = np.dtype([("path", "<i4"), ("mode", "<i4"), ("num_two_shape", "<u4"),
@@ -280,6 +287,21 @@ def compress_with_stats_device(in_ptr, out_ptr, out_size, stats_ptr, stats_count
              seed, block_modes, wm_counter, q0_guard, None, 4, 0.6)
     _check(lib().fh_compress_with_stats_device(ctypes.byref(j), ctypes.c_void_p(stats_ptr),
                                                stats_count, ctypes.c_void_p(stream)))
+
+
+def compress_with_selection(img, selections, quality=0, seed=0, block_modes=0xFF, rect=None,
+                            wm_counter=None, q0_guard=True):
+    """BPTCC::Compress with a caller's shape selection function
+    (fh_compress_with_selection): `selections` is a SHAPE_SELECTION_DTYPE
+    array, one entry per block the job processes, in processing order."""
+    img = np.ascontiguousarray(img, dtype=np.uint32)
+    h, w = img.shape
+    sels = np.ascontiguousarray(selections, dtype=SHAPE_SELECTION_DTYPE)
+    out = np.zeros(compressed_size(eCompressionFormat_BPTC, w, h), dtype=np.uint8)
+    j = _job(eCompressionFormat_BPTC, img.ctypes.data, out.ctypes.data, out.nbytes, w, h, rect,
+             quality, seed, block_modes, wm_counter, q0_guard, None, 4, 0.6)
+    _check(lib().fh_compress_with_selection(ctypes.byref(j), sels.ctypes.data, sels.size))
+    return out
 
 
 def stats_format(stats, block_index):

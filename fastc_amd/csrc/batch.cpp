@@ -120,6 +120,7 @@ hipError_t enqueue(fh_batch *b, hipStream_t s) {
     p.wg_per_cu = b->tune.wg_per_cu;
     p.anneal_unroll = b->tune.anneal_unroll;
     p.stats = nullptr;
+    p.selections = nullptr;
     Bc7Scratch sc;
     const size_t prefix_bytes = ((size_t)b->bc7_chunk + 255) & ~(size_t)255;
     sc.local_prefix = b->bc7_scan;

@@ -1008,8 +1008,8 @@ __device__ __forceinline__ uint32_t alpha_fit(const Fit &f, const uint32_t px[16
 // strict minimum over the slots is the reference's choice.  Up to 32 slots
 // per block (16 in Compress's search; CompressWithStats runs modes 4, 5 and
 // 6 once per shape entry), each a 16-bit descriptor:
-//   mode 3 | shape 6 << 3 | rotation 2 << 9 | index mode 1 << 11 | entry 1 << 12
-// (entry = the BoxSelection shape entry, which keys the annealing stream).
+//   mode 3 | shape 6 << 3 | rotation 2 << 9 | index mode 1 << 11 | entry 3 << 12
+// (entry = the shape selection's entry, 0..4, which keys the annealing stream).
 // Candidate types (work queues, sorted so that a wave runs one code path):
 //   0 mode 0, 1 mode 2, 2 mode 1, 3 mode 3, 4 mode 7, 5 mode 6, 6 mode 4, 7 mode 5.
 constexpr int kNumTypes = 8;
@@ -1051,7 +1051,7 @@ __device__ __forceinline__ void cand_from_desc(uint32_t d, Cand &k) {
   k.shape = (d >> 3) & 63;
   k.rot = (d >> 9) & 3;
   k.im = (d >> 11) & 1;
-  k.entry = (d >> 12) & 1;
+  k.entry = (d >> 12) & 7;
   k.nsub = kModes[k.mode].subsets;
 }
 
@@ -1100,7 +1100,7 @@ struct CandRec {
 // Annealing work item (start state from the set-up kernel).
 struct AState {
   uint32_t id;        // cand_id(relative block, slot)
-  uint32_t info;      // mode | rot << 3 | im << 5 | nsub << 6 | shape << 8 | active << 14 | entry << 17
+  uint32_t info;      // mode | rot << 3 | im << 5 | nsub << 6 | shape << 8 | active << 14 | entry 3 << 17
   uint32_t p1[3], p2[3];
   uint32_t err[3];
   uint32_t pbc;       // 2 bits per subset
@@ -1244,9 +1244,10 @@ __device__ __forceinline__ void load_block(const BlockRange &r, uint32_t rel, ui
   }
 }
 
+// Key of one fit's annealing stream; shape entries 0..4 take bits 3-4 and 10.
 __device__ __forceinline__ uint32_t cand_tag(int mode, int entry, int subset, int rot, int im) {
-  return (uint32_t)mode | ((uint32_t)entry << 3) | ((uint32_t)subset << 5) | ((uint32_t)rot << 7) |
-         ((uint32_t)im << 9);
+  return (uint32_t)mode | ((uint32_t)(entry & 3) << 3) | ((uint32_t)subset << 5) |
+         ((uint32_t)rot << 7) | ((uint32_t)im << 9) | ((uint32_t)(entry >> 2) << 10);
 }
 
 // Appends the workgroup's valid candidates to their type queues: slots are
@@ -1762,15 +1763,73 @@ __device__ uint32_t stats_select_block(const uint32_t px[16], int lane, const Bc
   return sel_pack(2, count);
 }
 
+// The candidates of a caller's ShapeSelection (CompressionSettings::
+// m_ShapeSelectionFn, Compressor.cpp:1840-1858) in CompressClusters' order
+// (:1752-1809): the first min(5, num_shapes) entries, or one entry and no
+// multi-subset mode when there are none.  As in the statistics search, a
+// single-subset mode's entries after the first are left out without
+// annealing.  At most 31 slots (the host checks every selection first).
+__device__ int given_candidates(const fh_shape_selection &g, uint32_t block_modes, uint32_t q,
+                                uint16_t out[kMaxCands]) {
+  constexpr int kOrder[8] = {0, 2, 1, 3, 7, 4, 5, 6};
+  uint32_t modes = g.selected_modes & block_modes;
+  int nsh = g.num_shapes < 5 ? (int)g.num_shapes : 5;
+  if (nsh == 0) {
+    nsh = 1;
+    modes &= ~((1u << 0) | (1u << 1) | (1u << 2) | (1u << 3) | (1u << 7));
+  }
+  int n = 0;
+  auto emit = [&](uint32_t d) {
+    if (n < kMaxCands - 1) out[n++] = (uint16_t)d;
+  };
+#pragma unroll
+  for (int mi = 0; mi < 8; mi++) {
+    const int mode = kOrder[mi];
+    if (!((modes >> mode) & 1)) continue;
+    for (int e = 0; e < nsh; e++) {
+      const uint32_t parts = g.shapes[e].num_partitions, idx = g.shapes[e].index & 63u;
+      const int nsub = kModes[mode].subsets;
+      if (nsub != 1 && (uint32_t)nsub != parts) continue;
+      if (mode == 0 && g.shapes[e].index >= 16) continue;
+      if (nsub == 1 && e > 0 && q == 0) continue;
+      if (mode == 4) {
+        for (int c = 0; c < 8; c++) emit(cdesc_pack(4, 0, c >> 1, c & 1, e));
+      } else if (mode == 5) {
+        for (int c = 0; c < 4; c++) emit(cdesc_pack(5, 0, c, 0, e));
+      } else {
+        emit(cdesc_pack(mode, (int)idx, 0, 0, e));
+      }
+    }
+  }
+  return n;
+}
+
+// Classification and a caller's selection of one block (solid and
+// transparent blocks never reach the selection function).
+__device__ uint32_t given_select_block(const uint32_t px[16], const Bc7Params &prm,
+                                       const fh_shape_selection &g, uint16_t cd[kMaxCands]) {
+  bool solid = true, transparent = true;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    solid = solid && px[i] == px[0];
+    transparent = transparent && (px[i] >> 24) == 0;
+  }
+  if (solid || transparent) return sel_pack(solid ? 0 : 1, 0);
+  return sel_pack(2, given_candidates(g, prm.block_modes, prm.quality, cd));
+}
+
 // Kernel 1: one workgroup of four waves selects 16 blocks, each wave four
 // in turn; the workgroup's candidates go to the type queues with one global
 // atomic per type.  list, if given, holds the blocks to select (list_count
 // of them, a device word); otherwise all blocks of the range.
-template <bool STATS>
+// SEL: which selection (BoxSelection, the statistics scan, the caller's).
+enum { kSelBox = 0, kSelStats = 1, kSelGiven = 2 };
+
+template <int SEL>
 __global__ void __launch_bounds__(256) select_kernel(BlockRange r, Bc7Params prm, Work w,
                                                      const uint32_t *list,
                                                      const uint32_t *list_count) {
-  constexpr int kSlots = STATS ? kMaxCands : kMaxBoxCands;
+  constexpr int kSlots = SEL == kSelBox ? kMaxBoxCands : kMaxCands;
   __shared__ uint32_t s_px[16][16];
   __shared__ uint32_t s_sel[16];
   __shared__ uint16_t s_cd[16][kSlots];
@@ -1799,8 +1858,9 @@ __global__ void __launch_bounds__(256) select_kernel(BlockRange r, Bc7Params prm
 #pragma unroll
     for (int i = 0; i < 16; i++) px[i] = s_px[blk][i];
     uint16_t cd[kSlots];
-    const uint32_t sel = STATS ? stats_select_block(px, lane, prm, cd, prm.stats + rel)
-                               : box_select_block(px, lane, prm, cd);
+    const uint32_t sel = SEL == kSelStats ? stats_select_block(px, lane, prm, cd, prm.stats + rel)
+                       : SEL == kSelGiven ? given_select_block(px, prm, prm.selections[rel], cd)
+                                          : box_select_block(px, lane, prm, cd);
     const int count = sel_count(sel);
     if (lane == 0) {
       s_sel[blk] = sel;
@@ -1822,7 +1882,7 @@ __global__ void __launch_bounds__(256) select_kernel(BlockRange r, Bc7Params prm
       // mode 6's second shape entry has the same quality-0 fit as the first
       // (its slot follows it): the set-up of the first writes both
       const uint32_t d = s_cd[blk][c];
-      valid = !((d & 7) == 6 && ((d >> 12) & 1));
+      valid = !((d & 7) == 6 && ((d >> 12) & 7) == 1);
       rel = rel_of(base + blk);
       type = kTypeOfMode[d & 7];
     }
@@ -2119,7 +2179,7 @@ __global__ void __launch_bounds__(64) setup_kernel(BlockRange r, Bc7Params prm, 
       bool twin = false;
       if (MODE == 6 && c + 1 < kMaxCands && c + 1 < sel_count(w.sel[rel])) {
         const uint32_t d1 = w.cdesc[(size_t)rel * kMaxCands + c + 1];
-        twin = (d1 & 7) == 6 && ((d1 >> 12) & 1);
+        twin = (d1 & 7) == 6 && ((d1 >> 12) & 7) == 1;
       }
       if (twin) w.rec[(size_t)rel * kMaxCands + c + 1] = rec;
       if (active) {
@@ -2172,7 +2232,7 @@ __device__ void lane_load(Lane &L, const BlockRange &r, const Bc7Params &prm, co
   L.nsub = (st.info >> 6) & 3;
   L.shape = (st.info >> 8) & 63;
   L.active = (st.info >> 14) & 7;
-  L.entry = (st.info >> 17) & 1;
+  L.entry = (st.info >> 17) & 7;
   set_fit(L.f, mode, rot, im);
   L.ptmode = kModes[mode].has_rot ? rot + 1 : 0;
   L.sub2 = 0;
@@ -3059,17 +3119,22 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
       timing_end(t, kTagBc7Select, stream);
     }
     t = timing_begin(stream);
-    hipLaunchKernelGGL(bc7::select_kernel<false>, dim3(ngroups), dim3(256), 0, stream, r, p, w,
+    hipLaunchKernelGGL(bc7::select_kernel<bc7::kSelBox>, dim3(ngroups), dim3(256), 0, stream, r, p, w,
                        (const uint32_t *)fallback, (const uint32_t *)fallback_count);
     timing_end(t, kTagBc7Select, stream);
   } else if (p.stats) {
     t = timing_begin(stream);
-    hipLaunchKernelGGL(bc7::select_kernel<true>, dim3(ngroups), dim3(256), 0, stream, r, p, w,
-                       (const uint32_t *)nullptr, (const uint32_t *)nullptr);
+    hipLaunchKernelGGL(bc7::select_kernel<bc7::kSelStats>, dim3(ngroups), dim3(256), 0, stream,
+                       r, p, w, (const uint32_t *)nullptr, (const uint32_t *)nullptr);
+    timing_end(t, kTagBc7Select, stream);
+  } else if (p.selections) {
+    t = timing_begin(stream);
+    hipLaunchKernelGGL(bc7::select_kernel<bc7::kSelGiven>, dim3(ngroups), dim3(256), 0, stream,
+                       r, p, w, (const uint32_t *)nullptr, (const uint32_t *)nullptr);
     timing_end(t, kTagBc7Select, stream);
   } else {
     t = timing_begin(stream);
-    hipLaunchKernelGGL(bc7::select_kernel<false>, dim3(ngroups), dim3(256), 0, stream, r, p, w,
+    hipLaunchKernelGGL(bc7::select_kernel<bc7::kSelBox>, dim3(ngroups), dim3(256), 0, stream, r, p, w,
                        (const uint32_t *)nullptr, (const uint32_t *)nullptr);
     timing_end(t, kTagBc7Select, stream);
   }

@@ -49,6 +49,8 @@ struct DeviceState {
   unsigned long long *sse = nullptr;  // 2 words, fh_psnr's total
   void *stats_buf = nullptr;          // fh_compress_with_stats' records
   size_t stats_size = 0;
+  void *sel_buf = nullptr;            // fh_compress_with_selection's selections
+  size_t sel_size = 0;
   bool init = false;
 };
 
@@ -181,7 +183,8 @@ fh_status validate(const fh_job *j, bool device) {
 // Runs the encoder on device buffers.  Caller holds st->mu.
 fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8_t *dout,
                      hipStream_t stream, uint32_t first, uint32_t count,
-                     fh_bc7_block_stats *dstats = nullptr) {
+                     fh_bc7_block_stats *dstats = nullptr,
+                     const fh_shape_selection *dsels = nullptr) {
   BlockRange r;
   r.img = reinterpret_cast<const uint32_t *>(din);
   r.out = dout;
@@ -228,6 +231,7 @@ fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8
       p.wg_per_cu = 0;
       p.anneal_unroll = 0;
       p.stats = nullptr;
+      p.selections = nullptr;
       // jobs larger than one call's work area run as consecutive calls on the
       // stream; the watermark counter carries across them in order
       for (uint32_t done = 0; done < count; done += chunk) {
@@ -236,6 +240,7 @@ fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8
         rc.num_blocks = count - done < chunk ? count - done : chunk;
         s.num_chunks = (rc.num_blocks + 255) / 256;
         if (dstats) p.stats = dstats + done;
+        if (dsels) p.selections = dsels + done;
         e = bc7_encode(rc, p, s, work, stream);
         if (e != hipSuccess) return fail_hip(e, "BC7 encode launch");
       }
@@ -275,12 +280,55 @@ int current_device(fh_status *s) {
   return dev;
 }
 
-// The statistics entry points take BPTC jobs of the reference search only.
+// Candidates of one caller selection, counted as the select kernel builds
+// them (CompressClusters' loop, Compressor.cpp:1771-1809): rotation modes 8
+// and 4 slots, a single-subset mode once per entry only when annealing.
+int selection_candidates(const fh_shape_selection &g, uint32_t block_modes, uint32_t quality) {
+  static const int kSubsets[8] = {3, 2, 3, 2, 1, 1, 1, 2};
+  static const int kSlots[8] = {1, 1, 1, 1, 8, 4, 1, 1};
+  uint32_t modes = g.selected_modes & block_modes;
+  uint32_t nsh = g.num_shapes < 5 ? g.num_shapes : 5;
+  if (nsh == 0) {
+    nsh = 1;
+    modes &= ~0x8Fu;  // modes 0, 1, 2, 3 and 7
+  }
+  int n = 0;
+  for (int mode = 0; mode < 8; mode++) {
+    if (!((modes >> mode) & 1)) continue;
+    for (uint32_t e = 0; e < nsh; e++) {
+      const int nsub = kSubsets[mode];
+      if (nsub != 1 && (uint32_t)nsub != g.shapes[e].num_partitions) continue;
+      if (mode == 0 && g.shapes[e].index >= 16) continue;
+      if (nsub == 1 && e > 0 && quality == 0) continue;
+      n += kSlots[mode];
+    }
+  }
+  return n;
+}
+
+fh_status check_selections(const fh_shape_selection *sels, uint32_t count, const fh_job *j) {
+  const uint32_t modes = j->block_modes ? (j->block_modes & 0xFFu) : 0xFFu;
+  for (uint32_t k = 0; k < count; k++) {
+    const fh_shape_selection &g = sels[k];
+    const uint32_t nsh = g.num_shapes < 5 ? g.num_shapes : 5;
+    for (uint32_t e = 0; e < nsh; e++) {
+      const uint32_t parts = g.shapes[e].num_partitions;
+      if ((parts == 2 || parts == 3) && g.shapes[e].index >= 64)
+        return fail(FH_ERR_INVALID_ARGUMENT, "shape index beyond the partition tables");
+    }
+    if (selection_candidates(g, modes, j->quality) > 31)
+      return fail(FH_ERR_UNSUPPORTED, "a shape selection with more than 31 candidates");
+  }
+  return FH_OK;
+}
+
+// The statistics and shape-selection entry points take BPTC jobs of the
+// reference search only.
 fh_status stats_job_ok(const fh_job *j) {
   if (j && j->format != FH_FORMAT_BPTC && block_bytes(j->format) != 0)
-    return fail(FH_ERR_UNSUPPORTED, "statistics are only collected for BPTC");
+    return fail(FH_ERR_UNSUPPORTED, "statistics and shape selections are BPTC only");
   if (j && (j->flags & FH_FLAG_LEARNED))
-    return fail(FH_ERR_INVALID_ARGUMENT, "statistics are not collected by the learned search");
+    return fail(FH_ERR_INVALID_ARGUMENT, "not combined with the learned search");
   return FH_OK;
 }
 
@@ -288,7 +336,8 @@ fh_status stats_job_ok(const fh_job *j) {
 // encode, copy the blocks (and statistics) back.  Caller-facing checks as in
 // CompressImageData.
 fh_status compress_host(const fh_job *j, bool with_stats, fh_bc7_block_stats *stats,
-                        size_t stats_count) {
+                        size_t stats_count, const fh_shape_selection *sels = nullptr,
+                        size_t sels_count = 0) {
   fh_status s = validate(j, false);
   if (s != FH_OK) return s;
   uint32_t first = 0, count = 0;
@@ -296,6 +345,11 @@ fh_status compress_host(const fh_job *j, bool with_stats, fh_bc7_block_stats *st
   if (s != FH_OK) return s;
   if (with_stats && count && (!stats || stats_count < count))
     return fail(FH_ERR_BUFFER_TOO_SMALL, "statistics buffer too small");
+  if (sels) {
+    if (sels_count < count) return fail(FH_ERR_BUFFER_TOO_SMALL, "too few shape selections");
+    s = check_selections(sels, count, j);
+    if (s != FH_OK) return s;
+  }
   const int dev = current_device(&s);
   if (dev < 0) return s;
   DeviceState *st = device_state(dev);
@@ -310,13 +364,18 @@ fh_status compress_host(const fh_job *j, bool with_stats, fh_bc7_block_stats *st
     e = grow(reinterpret_cast<void **>(&st->out_buf), &st->out_size, out_bytes);
   const size_t stats_bytes = with_stats ? (size_t)count * sizeof(fh_bc7_block_stats) : 0;
   if (e == hipSuccess && stats_bytes) e = grow(&st->stats_buf, &st->stats_size, stats_bytes);
+  const size_t sel_bytes = sels ? (size_t)count * sizeof(fh_shape_selection) : 0;
+  if (e == hipSuccess && sel_bytes) e = grow(&st->sel_buf, &st->sel_size, sel_bytes);
   if (e != hipSuccess) return fail_hip(e, "device buffer allocation");
   fh_bc7_block_stats *dstats =
       stats_bytes ? reinterpret_cast<fh_bc7_block_stats *>(st->stats_buf) : nullptr;
   hipStream_t stream = nullptr;
   e = hipMemcpyAsync(st->in_buf, j->in, in_bytes, hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess && sel_bytes)
+    e = hipMemcpyAsync(st->sel_buf, sels, sel_bytes, hipMemcpyHostToDevice, stream);
   if (e != hipSuccess) return fail_hip(e, "input upload");
-  s = run_device(st, j, st->in_buf, st->out_buf, stream, first, count, dstats);
+  s = run_device(st, j, st->in_buf, st->out_buf, stream, first, count, dstats,
+                 sel_bytes ? reinterpret_cast<const fh_shape_selection *>(st->sel_buf) : nullptr);
   if (s != FH_OK) return s;
   if (dstats) {
     e = hipMemcpyAsync(stats, dstats, stats_bytes, hipMemcpyDeviceToHost, stream);
@@ -381,6 +440,14 @@ fh_status fh_compress_with_stats_device(const fh_job *j, fh_bc7_block_stats *sta
   if (e != hipSuccess) return fail_hip(e, "device initialisation");
   return run_device(st, j, j->in, j->out, reinterpret_cast<hipStream_t>(hip_stream), first,
                     count, stats);
+}
+
+fh_status fh_compress_with_selection(const fh_job *j, const fh_shape_selection *selections,
+                                     size_t selections_count) {
+  const fh_status s = stats_job_ok(j);
+  if (s != FH_OK) return s;
+  if (!selections) return fail(FH_ERR_INVALID_ARGUMENT, "null shape selections");
+  return compress_host(j, false, nullptr, 0, selections, selections_count);
 }
 
 fh_status fh_bc7_stats_format(const fh_bc7_block_stats *st, uint64_t block_index, char *out,

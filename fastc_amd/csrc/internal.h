@@ -8,6 +8,7 @@
 #include <mutex>
 
 struct fh_bc7_block_stats;  // include/fasthip.h
+struct fh_shape_selection;
 
 namespace fh {
 
@@ -39,6 +40,9 @@ struct Bc7Params {
   // CompressWithStats' block path: one record per block of the range (null =
   // Compress's BoxSelection search); not combined with the ranker
   fh_bc7_block_stats *stats;
+  // a caller's shape selection per block of the range (device; null = the
+  // library's own search); not combined with the ranker or statistics
+  const fh_shape_selection *selections;
 };
 
 // Device scratch needed by the BC7 watermark prefix scan.

@@ -27,6 +27,9 @@
  *                        BPTCC::CompressWithStats (BPTCCompressor.h:173,
  *                        BPTCEncoder/src/Compressor.cpp:1577-1623) and the
  *                        BlockLogger lines it writes (Compressor.cpp:1373-1390).
+ *   fh_compress_with_selection
+ *                        BPTCC::Compress with a caller's m_ShapeSelectionFn
+ *                        (BPTCCompressor.h:123-134, Compressor.cpp:1840-1858).
  *   fh_compressed_size   CompressedImage::GetCompressedSize
  *                        (Core/src/CompressedImage.cpp:136-149).
  *   fh_last_error        the stderr message CompressImageData prints
@@ -161,6 +164,32 @@ fh_status fh_compress_with_stats(const fh_job *job, fh_bc7_block_stats *stats,
  * asynchronous on hip_stream, as fh_compress_device. */
 fh_status fh_compress_with_stats_device(const fh_job *job, fh_bc7_block_stats *stats,
                                         size_t stats_count, void *hip_stream);
+
+/* BPTCC::ShapeSelection (BPTCEncoder/include/FasTC/BPTCCompressor.h:85-112),
+ * field for field: the shapes CompressClusters tries for one block and the
+ * modes it allows. */
+typedef struct fh_shape_selection {
+  uint32_t num_shapes;      /* m_NumShapesToSearch; CompressClusters tries at most 5 */
+  struct {
+    uint32_t num_partitions;  /* Shape::m_NumPartitions */
+    uint32_t index;           /* Shape::m_Index, < 64 when num_partitions is 2 or 3 */
+  } shapes[10];
+  uint32_t selected_modes;  /* m_SelectedModes, and-ed with fh_job.block_modes */
+} fh_shape_selection;
+
+/* BPTCC::Compress with CompressionSettings::m_ShapeSelectionFn set
+ * (BPTCCompressor.h:123-134, Compressor.cpp:1840-1858) for a BPTC job on host
+ * buffers: selections[k] is the ShapeSelection the caller's function returns
+ * for the k-th block the job processes (raster order from (x_start,
+ * y_start)); CompressClusters, annealing and packing run on the GPU.  Solid
+ * and transparent blocks never reach the selection function; their entries
+ * are ignored.  Every entry must describe at most 31 candidates (a rotation
+ * mode counts 8 or 4, and with quality > 0 a single-subset mode counts once
+ * per shape entry), else FH_ERR_UNSUPPORTED; an index of 64 or more on a
+ * two- or three-partition shape is FH_ERR_INVALID_ARGUMENT (the reference
+ * would read past its partition tables). */
+fh_status fh_compress_with_selection(const fh_job *job, const fh_shape_selection *selections,
+                                     size_t selections_count);
 
 /* The text BlockLogger writes for one block (Compressor.cpp:1373-1390,
  * 1945-1950): a line "<block_index>: BlockStat_<name> -- <value>" per

@@ -147,7 +147,27 @@ namespace BPTCC {
 
 enum ErrorMetric { eErrorMetric_Uniform, eErrorMetric_Nonuniform, kNumErrorMetrics };
 
+// Shape, ShapeSelection and ShapeSelectionFn (BPTCCompressor.h:80-120).
+struct Shape {
+  uint32_t m_NumPartitions;
+  uint32_t m_Index;
+};
+
+struct ShapeSelection {
+  uint32_t m_NumShapesToSearch = 0;
+  Shape m_Shapes[10] = {};
+  uint32_t m_SelectedModes = 0xFF;
+};
+
+typedef ShapeSelection (*ShapeSelectionFn)(uint32_t x, uint32_t y, const uint32_t pixels[16],
+                                           const void *userData);
+
 struct CompressionSettings {
+  // Called on the host for every block that is neither one colour nor fully
+  // transparent, as the reference calls it (Compressor.cpp:1840-1858); the
+  // encode itself runs on the GPU.  Null: BoxSelection.
+  ShapeSelectionFn m_ShapeSelectionFn = nullptr;
+  const void *m_ShapeSelectionUserData = nullptr;
   uint32_t m_BlockModes = 0xFF;
   ErrorMetric m_ErrorMetric = eErrorMetric_Uniform;
   uint32_t m_NumSimulatedAnnealingSteps = 50;
@@ -163,7 +183,42 @@ inline void Compress(const FasTC::CompressionJob &cj,
   j.block_modes = settings.m_BlockModes;
   j.error_metric = (uint32_t)settings.m_ErrorMetric;
   j.seed = settings.m_Seed;
-  detail::check(fh_compress(&j));
+  if (!settings.m_ShapeSelectionFn) {
+    detail::check(fh_compress(&j));
+    return;
+  }
+  // the caller's selection of every block, in the raster loop's order
+  // (Compressor.cpp:1476-1490); solid and transparent blocks keep an empty
+  // entry, as the reference never asks for theirs
+  static_assert(sizeof(ShapeSelection) == sizeof(fh_shape_selection), "ShapeSelection layout");
+  std::vector<fh_shape_selection> sels(detail::job_blocks(cj) + 1);
+  const uint32_t *px = reinterpret_cast<const uint32_t *>(cj.InBuf());
+  const uint32_t end_y = cj.YEnd() < cj.Height() - 4 ? cj.YEnd() : cj.Height() - 4;
+  size_t k = 0;
+  uint32_t sx = cj.XStart();
+  for (uint32_t y = cj.YStart(); y <= end_y && cj.Height() >= 4; y += 4, sx = 0) {
+    const uint32_t end_x = y == cj.YEnd() ? cj.XEnd() : cj.Width();
+    for (uint32_t x = sx; x < end_x; x += 4, k++) {
+      uint32_t blk[16];
+      bool solid = true, transparent = true;
+      for (int i = 0; i < 16; i++) {
+        blk[i] = px[(size_t)(y + i / 4) * cj.Width() + x + i % 4];
+        solid = solid && blk[i] == blk[0];
+        transparent = transparent && (blk[i] >> 24) == 0;
+      }
+      sels[k] = fh_shape_selection();
+      if (solid || transparent) continue;
+      const ShapeSelection g =
+          settings.m_ShapeSelectionFn(x, y, blk, settings.m_ShapeSelectionUserData);
+      sels[k].num_shapes = g.m_NumShapesToSearch;
+      for (int e = 0; e < 10; e++) {
+        sels[k].shapes[e].num_partitions = g.m_Shapes[e].m_NumPartitions;
+        sels[k].shapes[e].index = g.m_Shapes[e].m_Index;
+      }
+      sels[k].selected_modes = g.m_SelectedModes;
+    }
+  }
+  detail::check(fh_compress_with_selection(&j, sels.data(), sels.size()));
 }
 
 // BPTCC::CompressWithStats (BPTCCompressor.h:173-174, Compressor.cpp:1577-1623):

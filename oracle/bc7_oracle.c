@@ -1134,8 +1134,9 @@ static double compress_cluster_alpha(cmode *m, const clus *c, v4 *p1, v4 *p2, ui
 /* Fit tag for the counter RNG: identifies (mode, shape entry, subset,
  * rotation, index mode) within a block. */
 static uint32_t fit_tag(int mode, int shape_entry, int subset, int rot, int im) {
-  return (uint32_t)mode | ((uint32_t)shape_entry << 3) | ((uint32_t)subset << 5) |
-         ((uint32_t)rot << 7) | ((uint32_t)im << 9);
+  /* shape entries 0..4: the two low bits, then bit 10 */
+  return (uint32_t)mode | ((uint32_t)(shape_entry & 3) << 3) | ((uint32_t)subset << 5) |
+         ((uint32_t)rot << 7) | ((uint32_t)im << 9) | ((uint32_t)(shape_entry >> 2) << 10);
 }
 
 static void m_rekey(cmode *m, int shape_entry, int subset) {
@@ -1306,7 +1307,7 @@ static double estimate(const clus *c, int nb) {
 
 typedef struct {
   uint32_t nshapes;
-  int sh_idx[2], sh_parts[2];
+  int sh_idx[5], sh_parts[5];  /* the entries CompressClusters tries (at most 5) */
   uint32_t modes;
 } shape_sel;
 
@@ -1314,8 +1315,8 @@ typedef struct {
 static shape_sel box_selection(const uint32_t px[16]) {
   shape_sel r;
   r.modes = 0xFF;
-  r.sh_idx[0] = r.sh_idx[1] = 0;
-  r.sh_parts[0] = r.sh_parts[1] = 0;
+  memset(r.sh_idx, 0, sizeof(r.sh_idx));
+  memset(r.sh_parts, 0, sizeof(r.sh_parts));
   int opaque = 1;
   for (int i = 0; i < 16; i++) opaque = opaque && (((px[i] >> 24) & 0xFF) >= 250);
   double best[2] = {DBL_MAX, DBL_MAX};
@@ -1734,8 +1735,8 @@ static double est_three_stats(const clus *c, double se[2]) {
 static shape_sel stats_selection(const uint32_t px[16], ora_bc7_log_stats *ls) {
   shape_sel r;
   r.modes = 0xFF;
-  r.sh_idx[0] = r.sh_idx[1] = 0;
-  r.sh_parts[0] = r.sh_parts[1] = 0;
+  memset(r.sh_idx, 0, sizeof(r.sh_idx));
+  memset(r.sh_parts, 0, sizeof(r.sh_parts));
   clus c;
   clus_init(&c, px);
   int opaque = 1;
@@ -1867,6 +1868,54 @@ void ora_bc7_compress_job_logged(const uint8_t *in, uint8_t *out, uint32_t width
         wm = *wm_counter;
       }
       ora_bc7_compress_block_logged(blk, o, s, wm, (uint64_t)(j / 4) * bw + i / 4, &stats[k++]);
+      o += 16;
+    }
+    sx = 0;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* A caller's shape selection (CompressionSettings::m_ShapeSelectionFn,     */
+/* Compressor.cpp:1840-1858).                                               */
+/* ------------------------------------------------------------------------ */
+
+void ora_bc7_compress_job_selected(const uint8_t *in, uint8_t *out, uint32_t width,
+                                   uint32_t height, uint32_t x_start, uint32_t y_start,
+                                   uint32_t x_end, uint32_t y_end, const ora_bc7_settings *s,
+                                   uint32_t *wm_counter, const ora_shape_selection *sels) {
+  const uint32_t *pix = (const uint32_t *)in;
+  const uint32_t bw = width / 4;
+  uint8_t *o = out + ((y_start / 4) * bw + x_start / 4) * 16;
+  const uint32_t end_y = y_end < height - 4 ? y_end : height - 4;
+  uint32_t sx = x_start;
+  size_t k = 0;
+  for (uint32_t j = y_start; j <= end_y; j += 4) {
+    const uint32_t end_x = j == y_end ? x_end : width;
+    for (uint32_t i = sx; i < end_x; i += 4, k++) {
+      uint32_t blk[16];
+      for (int r = 0; r < 4; r++) memcpy(blk + 4 * r, pix + (j + r) * width + i, 16);
+      memset(o, 0, 16);
+      int transparent = 1;
+      for (int q = 0; q < 16; q++)
+        if ((float)((blk[q] >> 24) & 0xFF) > 0.0f) transparent = 0;
+      if (all_one_color(blk)) {
+        *wm_counter = (*wm_counter + 1) % 9;
+        write_solid(blk[0], o, *wm_counter);
+      } else if (transparent) {
+        write_transparent(o);
+      } else {
+        const ora_shape_selection *g = &sels[k];
+        shape_sel sel;
+        memset(&sel, 0, sizeof(sel));
+        sel.nshapes = g->num_shapes;
+        for (int e = 0; e < 5; e++) {
+          sel.sh_parts[e] = (int)g->shapes[e].num_partitions;
+          sel.sh_idx[e] = (int)g->shapes[e].index;
+        }
+        sel.modes = g->selected_modes & s->block_modes;
+        compress_clusters(&sel, blk, s, (uint64_t)(j / 4) * bw + i / 4, o, NULL);
+        q0_guard(&sel, blk, s, (uint64_t)(j / 4) * bw + i / 4, o);
+      }
       o += 16;
     }
     sx = 0;

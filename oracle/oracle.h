@@ -127,6 +127,25 @@ void ora_bc7_compress_job_logged(const uint8_t *in, uint8_t *out, uint32_t width
                                  const ora_bc7_settings *s, uint32_t *wm_counter,
                                  ora_bc7_log_stats *stats);
 
+/* BPTCC::ShapeSelection (BPTCEncoder/include/FasTC/BPTCCompressor.h:85-112);
+ * same layout as fh_shape_selection. */
+typedef struct {
+  uint32_t num_shapes;                                /* m_NumShapesToSearch */
+  struct { uint32_t num_partitions, index; } shapes[10];  /* m_Shapes */
+  uint32_t selected_modes;                            /* m_SelectedModes */
+} ora_shape_selection;
+
+/* BPTCC::Compress with a caller's shape selection function
+ * (CompressionSettings::m_ShapeSelectionFn, Compressor.cpp:1840-1858):
+ * sels[k] is the selection of the k-th block processed (ignored for solid
+ * and transparent blocks, which never reach the selection function); the
+ * selected modes are and-ed with block_modes, then CompressClusters runs over
+ * the first min(5, num_shapes) entries. */
+void ora_bc7_compress_job_selected(const uint8_t *in, uint8_t *out, uint32_t width,
+                                   uint32_t height, uint32_t x_start, uint32_t y_start,
+                                   uint32_t x_end, uint32_t y_end, const ora_bc7_settings *s,
+                                   uint32_t *wm_counter, const ora_shape_selection *sels);
+
 /* BPTCC::Decompress (Decompressor.cpp:345-367), including the reference's
  * treatment of mode 4 with index mode 1 (Decompressor.cpp:263-295 interpolate
  * with the per-mode bit counts, not the swapped ones). */

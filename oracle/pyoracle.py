@@ -60,6 +60,8 @@ def lib():
             ctypes.POINTER(Settings), ctypes.POINTER(ctypes.c_uint32)]
         L.ora_bc7_compress_job_logged.argtypes = [vp, vp] + [ctypes.c_uint32] * 6 + [
             ctypes.POINTER(Settings), ctypes.POINTER(ctypes.c_uint32), vp]
+        L.ora_bc7_compress_job_selected.argtypes = [vp, vp] + [ctypes.c_uint32] * 6 + [
+            ctypes.POINTER(Settings), ctypes.POINTER(ctypes.c_uint32), vp]
         L.ora_bc7_decompress.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32]
         L.ora_bc7_decode_block_spec.argtypes = [vp, vp]
         L.ora_psnr.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32]
@@ -141,6 +143,30 @@ def bc7_compress_logged(img, quality=0, seed=0, rng=RNG_COUNTER, lcg_seed=0, blo
     lib().ora_bc7_compress_job_logged(_p(img), _p(out), w, h, x0, y0, x1, y1, ctypes.byref(s),
                                       ctypes.byref(wm), _p(stats))
     return out, wm.value, stats
+
+
+# ora_shape_selection / fh_shape_selection
+SHAPE_SELECTION_DTYPE = np.dtype([("num_shapes", "<u4"),
+                                  ("shapes", [("num_partitions", "<u4"), ("index", "<u4")],
+                                   (10,)),
+                                  ("selected_modes", "<u4")])
+
+
+def bc7_compress_selected(img, selections, quality=0, seed=0, rng=RNG_COUNTER, lcg_seed=0,
+                          block_modes=0xFF, rect=None, wm_counter=0xFFFFFFFF, q0_guard=False):
+    """BPTCC::Compress with a caller's shape selection per processed block.
+    Returns (bytes, wm)."""
+    img = np.ascontiguousarray(img, dtype=np.uint32)
+    sels = np.ascontiguousarray(selections, dtype=SHAPE_SELECTION_DTYPE)
+    h, w = img.shape
+    out = np.zeros((h // 4) * (w // 4) * 16, dtype=np.uint8)
+    lcg = ctypes.c_uint32(lcg_seed)
+    s = Settings(quality, block_modes, rng, ctypes.pointer(lcg), seed, int(q0_guard))
+    wm = ctypes.c_uint32(wm_counter)
+    x0, y0, x1, y1 = rect if rect is not None else (0, 0, w, h)
+    lib().ora_bc7_compress_job_selected(_p(img), _p(out), w, h, x0, y0, x1, y1, ctypes.byref(s),
+                                        ctypes.byref(wm), _p(sels))
+    return out, wm.value
 
 
 def format_block_stats(rec, block_index):

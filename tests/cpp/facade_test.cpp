@@ -3,6 +3,8 @@
 //   facade_test validate                 -> checks error behaviour (no GPU needed)
 //   facade_test encode in.raw w h outdir -> BPTC(q=0), DXT1, DXT5, ETC1 via the
 //                                           codec entry points and CompressImageData
+//   facade_test select in.raw w h outdir -> BPTCC::Compress (q=0) with a shape
+//                                           selection function
 //   facade_test stats in.raw w h outdir  -> BPTCC::CompressWithStats (q=0) on a
 //                                           sub-job and CompressImageData with a
 //                                           log stream: blocks and log text
@@ -79,6 +81,34 @@ static int encode(const char *in, unsigned w, unsigned h, const std::string &dir
   return 0;
 }
 
+// A selection function: the two-subset shape (x / 4 + y / 4) % 64, all modes.
+static BPTCC::ShapeSelection diagonal(uint32_t x, uint32_t y, const uint32_t *, const void *ud) {
+  BPTCC::ShapeSelection s;
+  s.m_NumShapesToSearch = 1;
+  s.m_Shapes[0].m_NumPartitions = 2;
+  s.m_Shapes[0].m_Index = (x / 4 + y / 4 + *static_cast<const uint32_t *>(ud)) % 64;
+  s.m_SelectedModes = 0xFF;
+  return s;
+}
+
+static int select(const char *in, unsigned w, unsigned h, const std::string &dir) {
+  std::vector<uint8_t> img((size_t)w * h * 4);
+  FILE *f = fopen(in, "rb");
+  if (!f || fread(img.data(), 1, img.size(), f) != img.size()) return 2;
+  fclose(f);
+  std::vector<uint8_t> out((size_t)w * h, 0);
+  const uint32_t offset = 5;
+  BPTCC::CompressionSettings bs;
+  bs.m_NumSimulatedAnnealingSteps = 0;
+  bs.m_ShapeSelectionFn = diagonal;
+  bs.m_ShapeSelectionUserData = &offset;
+  BPTCC::Compress(FasTC::CompressionJob(FasTC::eCompressionFormat_BPTC, img.data(), out.data(),
+                                        w, h), bs);
+  save(dir + "/select.bin", out);
+  printf("select ok\n");
+  return 0;
+}
+
 static int stats(const char *in, unsigned w, unsigned h, const std::string &dir) {
   std::vector<uint8_t> img((size_t)w * h * 4);
   FILE *f = fopen(in, "rb");
@@ -109,8 +139,10 @@ int main(int argc, char **argv) {
   if (argc >= 2 && !strcmp(argv[1], "validate")) return validate();
   if (argc >= 6 && !strcmp(argv[1], "encode"))
     return encode(argv[2], (unsigned)atoi(argv[3]), (unsigned)atoi(argv[4]), argv[5]);
+  if (argc >= 6 && !strcmp(argv[1], "select"))
+    return select(argv[2], (unsigned)atoi(argv[3]), (unsigned)atoi(argv[4]), argv[5]);
   if (argc >= 6 && !strcmp(argv[1], "stats"))
     return stats(argv[2], (unsigned)atoi(argv[3]), (unsigned)atoi(argv[4]), argv[5]);
-  fprintf(stderr, "usage: facade_test validate | encode|stats in.raw w h outdir\n");
+  fprintf(stderr, "usage: facade_test validate | encode|select|stats in.raw w h outdir\n");
   return 2;
 }

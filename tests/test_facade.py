@@ -76,3 +76,25 @@ def test_facade_compress_with_stats(fh, oracle, facade_bin, tmp_path):
         n = int(np.count_nonzero((rst["mode"] != 0) | (rst["path"] != 0)))
         want = "".join(oracle.format_block_stats(rst[k], first + k) for k in range(n))
         assert (tmp_path / f"stats_{name}.log").read_text() == want, name
+
+
+@pytest.mark.gpu
+def test_facade_shape_selection_function(fh, oracle, facade_bin, tmp_path):
+    """CompressionSettings::m_ShapeSelectionFn called per block by the
+    facade (two-subset shape (x / 4 + y / 4 + 5) % 64), against the oracle
+    given the same selections (watermark entries aside)."""
+    img = synth.generate(64, 64, 3)
+    raw = tmp_path / "in.raw"
+    img.tofile(raw)
+    r = subprocess.run([facade_bin, "select", str(raw), "64", "64", str(tmp_path)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    sels = np.zeros(256, oracle.SHAPE_SELECTION_DTYPE)
+    for k in range(256):
+        by, bx = divmod(k, 16)
+        sels[k]["num_shapes"] = 1
+        sels[k]["shapes"][0] = (2, (bx + by + 5) % 64)
+        sels[k]["selected_modes"] = 0xFF
+    ref, _ = oracle.bc7_compress_selected(img, sels)
+    got = np.fromfile(tmp_path / "select.bin", dtype=np.uint8)
+    assert np.array_equal(oracle.bc7_decompress(got, 64, 64), oracle.bc7_decompress(ref, 64, 64))
