@@ -2684,17 +2684,80 @@ __global__ void __launch_bounds__(256) guard_kernel(BlockRange r, const uint4 *q
   if (premul_err(px, dz) < premul_err(px, da)) out[b] = z;
 }
 
-// Final selection of the learned search: every candidate's block is packed
-// and decoded with the reference decoder, and the smallest premul_err wins
-// (ties to the lower slot).  Pass 0 (before annealing) keeps the quality-0
-// winner and its error; the final pass writes its winner unless the
-// quality-0 one decodes strictly better, so no block is worse than at
-// quality 0.  Solid and transparent blocks are written as pack_kernel does.
-__global__ void __launch_bounds__(256) pack_decoded_kernel(BlockRange r, Bc7Scratch scr, Work w,
-                                                           uint4 *q0blk,
-                                                           unsigned long long *q0err,
-                                                           unsigned long long *cerr,
-                                                           int final_pass) {
+// The learned search chooses among a block's candidates by decoded error:
+// each candidate's block is packed and decoded with the reference decoder,
+// and the smallest premul_err wins (ties to the lower slot).  Pass 0 (before
+// annealing) keeps the quality-0 winner and its error; the final pass writes
+// its winner unless the quality-0 one decodes strictly better, so no block is
+// worse than at quality 0.  The errors come from cand_error_kernel, one
+// candidate per thread; pick_kernel then chooses per block.
+
+// Candidate decoded errors into cerr[id].  Pass 0 walks the type queues (a
+// wave packs one mode) and also fills the slot of a mode-6 second shape
+// entry, whose set-up copied the first's record; the final pass walks the
+// annealing list and re-scores only the records annealing rewrote.
+// Grid-stride: the item counts are device counters.
+__global__ void __launch_bounds__(256) cand_error_kernel(BlockRange r, Work w,
+                                                         unsigned long long *cerr,
+                                                         int final_pass) {
+  uint32_t total = 0;
+  if (final_pass) {
+    total = w.counters[18];
+  } else {
+#pragma unroll
+    for (int q = 0; q < kNumTypes; q++) total += w.counters[q];
+  }
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
+    uint32_t id;
+    if (final_pass) {
+      id = w.aid[w.alist[t]];
+    } else {
+      uint32_t base = 0, item = 0;
+      int type = kNumTypes - 1;
+#pragma unroll
+      for (int q = 0; q < kNumTypes; q++) {
+        const uint32_t c = w.counters[q];
+        if (t >= base && t < base + c) {
+          type = q;
+          item = t - base;
+        }
+        base += c;
+      }
+      id = w.queue[(size_t)type * w.qcap + item];
+    }
+    const CandRec rc = w.rec[id];
+    if (final_pass && !(rc.meta & kMetaAnnealed)) continue;
+    const uint32_t rel = id_block(id);
+    Cand k;
+    cand_from_desc(w.cdesc[id], k);
+    BitWriter bw;
+    pack_mode(k.mode, k.shape, rc, bw);
+    const uint4 blk = make_uint4((uint32_t)bw.lo, (uint32_t)(bw.lo >> 32), (uint32_t)bw.hi,
+                                 (uint32_t)(bw.hi >> 32));
+    uint32_t px[16], dec[16];
+    load_block(r, rel, px);
+    bc7dec::decode_block(blk, dec);
+    const unsigned long long e = premul_err(px, dec);
+    cerr[id] = e;
+    if (!final_pass && k.mode == 6) {
+      const uint32_t c = id_slot(id);
+      if (c + 1 < (uint32_t)sel_count(w.sel[rel])) {
+        const uint32_t d1 = w.cdesc[id + 1];
+        if ((d1 & 7) == 6 && ((d1 >> 12) & 7) == 1) cerr[id + 1] = e;
+      }
+    }
+  }
+}
+
+// Per block: the smallest candidate error (first slot on ties), packed.
+// Pass 0 stores the quality-0 winner and its error; the final pass writes
+// the block (solid and transparent blocks as pack_kernel does), keeping the
+// quality-0 winner when it decodes strictly better (q0err, if given).
+__global__ void __launch_bounds__(256) pick_kernel(BlockRange r, Bc7Scratch scr, Work w,
+                                                   uint4 *q0blk, unsigned long long *q0err,
+                                                   const unsigned long long *cerr,
+                                                   int final_pass) {
   const uint32_t rel = blockIdx.x * blockDim.x + threadIdx.x;
   if (rel >= r.num_blocks) return;
   const uint32_t b = r.first_block + rel;
@@ -2714,40 +2777,17 @@ __global__ void __launch_bounds__(256) pack_decoded_kernel(BlockRange r, Bc7Scra
     best = make_uint4((uint32_t)bw.lo, (uint32_t)(bw.lo >> 32), (uint32_t)bw.hi,
                       (uint32_t)(bw.hi >> 32));
   } else {
-    uint32_t px[16];
-    load_block(r, rel, px);
     unsigned long long best_err = ~0ull;
     int best_i = -1;
-    bool best_packed = false;
     const int count = sel_count(sel);
     for (int i = 0; i < count; i++) {
-      const CandRec &rc = w.rec[(size_t)rel * kMaxCands + i];
-      unsigned long long e;
-      uint4 blk = make_uint4(0, 0, 0, 0);
-      bool packed = false;
-      if (final_pass && cerr && !(rc.meta & kMetaAnnealed)) {
-        e = cerr[(size_t)rel * kMaxCands + i];  // record unchanged since pass 0
-      } else {
-        Cand k;
-        cand_from_desc(w.cdesc[(size_t)rel * kMaxCands + i], k);
-        BitWriter bw;
-        pack_mode(k.mode, k.shape, rc, bw);
-        blk = make_uint4((uint32_t)bw.lo, (uint32_t)(bw.lo >> 32), (uint32_t)bw.hi,
-                         (uint32_t)(bw.hi >> 32));
-        packed = true;
-        uint32_t dec[16];
-        bc7dec::decode_block(blk, dec);
-        e = premul_err(px, dec);
-        if (!final_pass && cerr) cerr[(size_t)rel * kMaxCands + i] = e;
-      }
+      const unsigned long long e = cerr[(size_t)rel * kMaxCands + i];
       if (e < best_err) {
         best_err = e;
-        best = blk;
         best_i = i;
-        best_packed = packed;
       }
     }
-    if (best_i >= 0 && !best_packed) {
+    if (best_i >= 0) {
       Cand k;
       cand_from_desc(w.cdesc[(size_t)rel * kMaxCands + best_i], k);
       BitWriter bw;
@@ -3095,6 +3135,8 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   const uint64_t max_waves = ((uint64_t)n * bc7::kMaxCands + 63) / 64;
   if (persistent > max_waves) persistent = (uint32_t)max_waves;
   const dim3 pack_grid((n + 255) / 256);
+  // candidate-error kernel: grid-stride, a few workgroups per CU
+  const uint32_t err_grid = (uint32_t)g_num_cus[dev] * 8u;
   const bool guard = p.quality > 0 && p.q0_guard;
   int t = timing_begin(stream);
   hipLaunchKernelGGL(bc7::reset_kernel, dim3(1), dim3(64), 0, stream, w.counters, 64);
@@ -3119,8 +3161,8 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
       timing_end(t, kTagBc7Select, stream);
     }
     t = timing_begin(stream);
-    hipLaunchKernelGGL(bc7::select_kernel<bc7::kSelBox>, dim3(ngroups), dim3(256), 0, stream, r, p, w,
-                       (const uint32_t *)fallback, (const uint32_t *)fallback_count);
+    hipLaunchKernelGGL(bc7::select_kernel<bc7::kSelBox>, dim3(ngroups), dim3(256), 0, stream,
+                       r, p, w, (const uint32_t *)fallback, (const uint32_t *)fallback_count);
     timing_end(t, kTagBc7Select, stream);
   } else if (p.stats) {
     t = timing_begin(stream);
@@ -3134,8 +3176,8 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
     timing_end(t, kTagBc7Select, stream);
   } else {
     t = timing_begin(stream);
-    hipLaunchKernelGGL(bc7::select_kernel<bc7::kSelBox>, dim3(ngroups), dim3(256), 0, stream, r, p, w,
-                       (const uint32_t *)nullptr, (const uint32_t *)nullptr);
+    hipLaunchKernelGGL(bc7::select_kernel<bc7::kSelBox>, dim3(ngroups), dim3(256), 0, stream,
+                       r, p, w, (const uint32_t *)nullptr, (const uint32_t *)nullptr);
     timing_end(t, kTagBc7Select, stream);
   }
   t = timing_begin(stream);
@@ -3152,8 +3194,10 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   if (learned) {
     if (p.quality > 0) {
       t = timing_begin(stream);
-      hipLaunchKernelGGL(bc7::pack_decoded_kernel, pack_grid, dim3(256), 0, stream, r, s, w, q0,
-                         q0err, cerr, 0);
+      hipLaunchKernelGGL(bc7::cand_error_kernel, dim3(err_grid), dim3(256), 0, stream, r, w,
+                         cerr, 0);
+      hipLaunchKernelGGL(bc7::pick_kernel, pack_grid, dim3(256), 0, stream, r, s, w, q0, q0err,
+                         (const unsigned long long *)cerr, 0);
       timing_end(t, kTagBc7Pack, stream);
       bc7::Work wa = w;
       wa.alist = alist;
@@ -3172,9 +3216,14 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
       timing_end(t, kTagBc7Anneal, stream);
     }
     t = timing_begin(stream);
-    hipLaunchKernelGGL(bc7::pack_decoded_kernel, pack_grid, dim3(256), 0, stream, r, s, w, q0,
+    bc7::Work wf = w;
+    wf.alist = alist;
+    // quality 0: every candidate's error; otherwise the annealed ones
+    hipLaunchKernelGGL(bc7::cand_error_kernel, dim3(err_grid), dim3(256), 0, stream, r, wf, cerr,
+                       p.quality > 0 ? 1 : 0);
+    hipLaunchKernelGGL(bc7::pick_kernel, pack_grid, dim3(256), 0, stream, r, s, w, q0,
                        p.quality > 0 ? q0err : (unsigned long long *)nullptr,
-                       p.quality > 0 ? cerr : (unsigned long long *)nullptr, 1);
+                       (const unsigned long long *)cerr, 1);
     timing_end(t, kTagBc7Pack, stream);
     return hipGetLastError();
   }
