@@ -121,50 +121,53 @@ __device__ bool evaluate_fast(const Opt &o, int cr, int cg, int cb, Solution &be
   bool tvalid = false;
   const int lmax = o.luma_sorted[7], lmin = o.luma_sorted[0];
   for (int it = 7; it >= 0; --it) {
-    int bc[4][3], bi[4];
-#pragma unroll
-    for (int s = 0; s < 4; s++) {
-      const int yd = kInten[it][s];
-      bc[s][0] = clamp255(sr + yd);
-      bc[s][1] = clamp255(sg + yd);
-      bc[s][2] = clamp255(sb + yd);
-      bi[s] = bc[s][0] + bc[s][1] + bc[s][2];
-    }
-    const int mid0 = bi[0] + bi[1], mid1 = bi[1] + bi[2], mid2 = bi[2] + bi[3];
+    // the four candidate colours of intensity row it (no private arrays:
+    // a per-lane index into one would live in scratch memory)
+    const int y0 = kInten[it][0], y1 = kInten[it][1], y2 = kInten[it][2], y3 = kInten[it][3];
+    const int c0r = clamp255(sr + y0), c0g = clamp255(sg + y0), c0b = clamp255(sb + y0);
+    const int c1r = clamp255(sr + y1), c1g = clamp255(sg + y1), c1b = clamp255(sb + y1);
+    const int c2r = clamp255(sr + y2), c2g = clamp255(sg + y2), c2b = clamp255(sb + y2);
+    const int c3r = clamp255(sr + y3), c3g = clamp255(sg + y3), c3b = clamp255(sb + y3);
+    const int bi0 = c0r + c0g + c0b, bi1 = c1r + c1g + c1b, bi2 = c2r + c2g + c2b,
+              bi3 = c3r + c3g + c3b;
+    const int mid0 = bi0 + bi1, mid1 = bi1 + bi2, mid2 = bi2 + bi3;
     uint64_t total = 0;
     uint32_t sel = 0;
     if (lmax * 2 < mid0) {
-      if (bi[0] > lmax && (uint64_t)(bi[0] - lmax) >= terr) continue;
+      if (bi0 > lmax && (uint64_t)(bi0 - lmax) >= terr) continue;
 #pragma unroll
       for (int c = 0; c < 8; c++) {
-        const int dr = chan(o.px[c], 0) - bc[0][0], dg = chan(o.px[c], 1) - bc[0][1],
-                  db = chan(o.px[c], 2) - bc[0][2];
+        const int dr = chan(o.px[c], 0) - c0r, dg = chan(o.px[c], 1) - c0g,
+                  db = chan(o.px[c], 2) - c0b;
         total += (uint64_t)(dr * dr + dg * dg + db * db);
       }
       sel = 0;
     } else if (lmin * 2 >= mid2) {
-      if (lmin > bi[3] && (uint64_t)(lmin - bi[3]) >= terr) continue;
+      if (lmin > bi3 && (uint64_t)(lmin - bi3) >= terr) continue;
 #pragma unroll
       for (int c = 0; c < 8; c++) {
-        const int dr = chan(o.px[c], 0) - bc[3][0], dg = chan(o.px[c], 1) - bc[3][1],
-                  db = chan(o.px[c], 2) - bc[3][2];
+        const int dr = chan(o.px[c], 0) - c3r, dg = chan(o.px[c], 1) - c3g,
+                  db = chan(o.px[c], 2) - c3b;
         total += (uint64_t)(dr * dr + dg * dg + db * db);
       }
       sel = 0xFFFFu;
     } else {
-      int cur = 0;
-      const int mids[3] = {mid0, mid1, mid2};
+      // the reference walks cur up the midpoints as the sorted lumas rise;
+      // the midpoints do not decrease (the row's offsets increase), so the
+      // walk ends at the number of midpoints at or below 2 y
 #pragma unroll
       for (int c = 0; c < 8; c++) {
-        const int y = o.luma_sorted[c];
-        while (cur < 3 && y * 2 >= mids[cur]) cur++;
+        const int y2x = o.luma_sorted[c] * 2;
+        const int cur = (y2x >= mid0) + (y2x >= mid1) + (y2x >= mid2);
         const int pi = o.idx_sorted[c];
         uint32_t p = 0;
 #pragma unroll
         for (int k = 0; k < 8; k++)
           if (k == pi) p = o.px[k];
-        const int dr = chan(p, 0) - bc[cur][0], dg = chan(p, 1) - bc[cur][1],
-                  db = chan(p, 2) - bc[cur][2];
+        const int er = cur == 0 ? c0r : (cur == 1 ? c1r : (cur == 2 ? c2r : c3r));
+        const int eg = cur == 0 ? c0g : (cur == 1 ? c1g : (cur == 2 ? c2g : c3g));
+        const int eb = cur == 0 ? c0b : (cur == 1 ? c1b : (cur == 2 ? c2b : c3b));
+        const int dr = chan(p, 0) - er, dg = chan(p, 1) - eg, db = chan(p, 2) - eb;
         total += (uint64_t)(dr * dr + dg * dg + db * db);
         sel |= (uint32_t)cur << (2 * pi);
       }
@@ -295,6 +298,30 @@ __device__ __forceinline__ void push_sel(uint32_t &s0, uint32_t &s1, uint32_t se
   s1 = (s1 << 1) | (b >> 1);
 }
 
+// One sub-block's fit (the body of pack_etc1_block's subblock loop,
+// rg_etc1.cpp:2321-2352): its 8 pixels in the reference's order, gathered
+// with static indices and selects, then etc1_optimizer.
+__device__ __forceinline__ bool fit_subblock(const uint32_t px[16], int flip, int sb, bool color4,
+                                             bool constrain, int base_r, int base_g, int base_b,
+                                             Solution &res) {
+  uint32_t sub[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    // flipped: rows 2 sb, 2 sb + 1; otherwise columns 2 sb, 2 sb + 1
+    const uint32_t f = sb ? px[8 + k] : px[k];
+    const int q = (k >> 2) + 4 * (k & 3);
+    const uint32_t u = sb ? px[2 + q] : px[q];
+    sub[k] = flip ? f : u;
+  }
+  Opt o;
+  opt_init(o, sub, color4);
+  o.constrain = constrain;
+  o.base_r = base_r;
+  o.base_g = base_g;
+  o.base_b = base_b;
+  return opt_compute(o, res);
+}
+
 __device__ uint2 pack_block(const Tables *t, const uint32_t px[16]) {
   bool solid = true;
 #pragma unroll
@@ -303,54 +330,30 @@ __device__ uint2 pack_block(const Tables *t, const uint32_t px[16]) {
 
   uint64_t best_error = ~0ull;
   int best_flip = 0, best_c4 = 0;
-  Solution best_res[2];
-  best_res[0].valid = best_res[1].valid = false;
-  best_res[0].r = best_res[0].g = best_res[0].b = best_res[0].inten = 0;
-  best_res[1] = best_res[0];
-  best_res[0].sel = best_res[1].sel = 0;
+  Solution best0, best1;
+  best0.valid = best1.valid = false;
+  best0.r = best0.g = best0.b = best0.inten = 0;
+  best0.sel = 0;
+  best0.err = 0;
+  best1 = best0;
   for (int flip = 0; flip < 2; flip++) {
     for (int c4 = 0; c4 < 2; c4++) {
-      uint64_t trial_error = 0;
-      Solution res[2];
-      int sb;
-      for (sb = 0; sb < 2; sb++) {
-        uint32_t sub[8];
-        if (flip) {
-#pragma unroll
-          for (int k = 0; k < 8; k++) sub[k] = px[sb * 8 + k];
-        } else {
-          const int c = sb * 2;
-          sub[0] = px[c + 0];
-          sub[1] = px[c + 4];
-          sub[2] = px[c + 8];
-          sub[3] = px[c + 12];
-          sub[4] = px[c + 1];
-          sub[5] = px[c + 5];
-          sub[6] = px[c + 9];
-          sub[7] = px[c + 13];
-        }
-        Opt o;
-        opt_init(o, sub, c4 != 0);
-        o.constrain = false;
-        o.base_r = o.base_g = o.base_b = 0;
-        if (!c4 && sb) {
-          o.constrain = true;
-          o.base_r = res[0].r;
-          o.base_g = res[0].g;
-          o.base_b = res[0].b;
-        }
-        if (!opt_compute(o, res[sb])) break;
-        trial_error += res[sb].err;
-        if (trial_error >= best_error) break;
-      }
-      if (sb < 2) continue;
+      Solution r0, r1;
+      if (!fit_subblock(px, flip, 0, c4 != 0, false, 0, 0, 0, r0)) continue;
+      uint64_t trial_error = r0.err;
+      if (trial_error >= best_error) continue;
+      // the 555 mode's second colour is a delta of the first: constrained
+      if (!fit_subblock(px, flip, 1, c4 != 0, !c4, r0.r, r0.g, r0.b, r1)) continue;
+      trial_error += r1.err;
+      if (trial_error >= best_error) continue;
       best_error = trial_error;
-      best_res[0] = res[0];
-      best_res[1] = res[1];
+      best0 = r0;
+      best1 = r1;
       best_flip = flip;
       best_c4 = c4;
     }
   }
+  Solution best_res[2] = {best0, best1};
   uint8_t b0, b1, b2;
   if (best_c4) {
     b0 = (uint8_t)(best_res[1].r | (best_res[0].r << 4));
@@ -416,7 +419,8 @@ __global__ void __launch_bounds__(256) encode_kernel(BlockRange r, const Tables 
 
 // Batch form: persistent workgroups take chunks of the group's blocks from
 // a device counter; each block finds its surface through block_surf.
-__global__ void batch_kernel(const SurfDesc *surfs, const uint32_t *block_surf, uint32_t total,
+// Workgroups of at most 512 threads (the autotuner's largest shape).
+__global__ void __launch_bounds__(512) batch_kernel(const SurfDesc *surfs, const uint32_t *block_surf, uint32_t total,
                              uint32_t *counter, const Tables *t) {
   __shared__ uint32_t s_chunk;
   for (;;) {
