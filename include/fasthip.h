@@ -23,7 +23,7 @@
  *   fh_psnr, fh_block_error_device, fh_image_error_device
  *                        Image::ComputePSNR (Base/src/Image.cpp:205-255) and
  *                        its per-block terms, fused with decoding.
- *   fh_compress_with_stats, fh_bc7_stats_format
+ *   fh_compress_with_stats[_device], fh_bc7_stats_format
  *                        BPTCC::CompressWithStats (BPTCCompressor.h:173,
  *                        BPTCEncoder/src/Compressor.cpp:1577-1623) and the
  *                        BlockLogger lines it writes (Compressor.cpp:1373-1390).
