@@ -1112,7 +1112,7 @@ struct AState {
 // Learned search: candidates whose quality-0 block decodes to more than
 // 1 + 1/kAnnealMarginDiv times the block's best quality-0 error are not
 // annealed (annealing lowers a fit's error by a few percent at most).
-constexpr unsigned long long kAnnealMarginDiv = 20;
+constexpr unsigned long long kAnnealMarginDiv = 50;
 
 // Learned search: the list of annealing states that pass the margin, in
 // astate order (the set-up kernels append by mode, so annealing waves stay

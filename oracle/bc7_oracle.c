@@ -1526,7 +1526,7 @@ double ora_bc7_encode_candidate(const uint32_t px[16], int mode, int slot, uint8
  * mode | slot << 3 | shape entry << 9 (at most 16).  Among the quality-0 fits
  * the block that decodes (reference decoder) to the smallest
  * ora_block_premul_err wins, ties to the earlier candidate; with quality > 0
- * the candidates within 1/20 of that error are annealed, the same choice is
+ * the candidates within 1/50 of that error are annealed, the same choice is
  * made over annealed and remaining quality-0 fits, and the quality-0 winner
  * is kept if it decodes strictly better. */
 void ora_bc7_encode_learned(const uint32_t px[16], const int32_t *cands, int n, uint32_t quality,
@@ -1542,9 +1542,9 @@ void ora_bc7_encode_learned(const uint32_t px[16], const int32_t *cands, int n, 
       const int mode = cands[i] & 7, slot = (cands[i] >> 3) & 63, entry = (cands[i] >> 9) & 1;
       uint8_t b[16];
       uint32_t dec[16];
-      /* only candidates whose quality-0 block decodes within 1/20 of the
+      /* only candidates whose quality-0 block decodes within 1/50 of the
        * best are annealed; the others keep their quality-0 fit */
-      if (pass == 1 && !(err0[i] <= best0 + best0 / 20)) {
+      if (pass == 1 && !(err0[i] <= best0 + best0 / 50)) {
         memcpy(b, cand0[i], 16);
       } else {
         encode_candidate(px, mode, slot, entry, pass ? quality : 0, seed, block_id, b);
