@@ -1545,8 +1545,8 @@ __device__ uint32_t box_select_block(const uint32_t px[16], int lane, const Bc7P
 // quantised with endpoint masks mask_a and mask_b (ToPixel without p-bit,
 // RGBAEndpoints.cpp:167-177), NB index entries: ea/eb per subset (exact
 // integers), flat when the subset is one colour.  Each pixel is scored once
-// per mask, against its own subset's endpoints.
-template <int NS, int NB>
+// per mask, against its own subset's endpoints; TWO = false scores mask_a only.
+template <int NS, int NB, bool TWO = true>
 __device__ __forceinline__ void masked_subset_errors(const uint32_t px[16], uint32_t m1,
                                                      uint32_t m2, uint32_t mask_a,
                                                      uint32_t mask_b, bool flat[NS],
@@ -1573,7 +1573,7 @@ __device__ __forceinline__ void masked_subset_errors(const uint32_t px[16], uint
     }
     int best;
     const uint32_t xa = pixel_err(px[i], px[i], ka, NB, (float)(NB - 1), wf, &best);
-    const uint32_t xb = pixel_err(px[i], px[i], kb, NB, (float)(NB - 1), wf, &best);
+    const uint32_t xb = TWO ? pixel_err(px[i], px[i], kb, NB, (float)(NB - 1), wf, &best) : 0u;
 #pragma unroll
     for (int t = 0; t < NS; t++) {
       ea[t] += sub == (uint32_t)t ? xa : 0u;
@@ -1713,7 +1713,7 @@ __device__ uint32_t stats_select_block(const uint32_t px[16], int lane, const Bc
   {
     bool flat[1];
     uint32_t ea[1], eb[1];
-    masked_subset_errors<1, 4>(px, 0u, 0u, 0xFEFEFEFEu, 0xFEFEFEFEu, flat, ea, eb);
+    masked_subset_errors<1, 4, false>(px, 0u, 0u, 0xFEFEFEFEu, 0xFEFEFEFEu, flat, ea, eb);
     est6 = flat[0] ? 0.0 : 0.0001 + (double)ea[0];
   }
   double e1, e3;
