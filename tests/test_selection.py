@@ -158,3 +158,31 @@ def test_box_selection_through_the_hook_equals_compress(fh, oracle):
         b = fh.compress_with_selection(img, sels, quality=q, seed=5,
                                        wm_counter=ctypes.c_uint32(0xFFFFFFFF))
         assert np.array_equal(a, b), q
+
+
+@pytest.mark.gpu
+def test_selections_multi_call_job(fh, oracle):
+    """More than 2^20 blocks: the second call reads its own selections."""
+    w, h = 4104, 4096
+    img = synth.generate(w, h, 12)
+    print("selection multi-call 4104x4096", flush=True)
+    n = (w // 4) * (h // 4)
+    sels = np.zeros(n, fastc_amd.SHAPE_SELECTION_DTYPE)
+    k = np.arange(n)
+    sels["num_shapes"] = 1
+    sels["shapes"][:, 0]["num_partitions"] = 2
+    sels["shapes"][:, 0]["index"] = k % 64
+    sels["selected_modes"] = np.where(k % 3 == 0, 0x82, 0xC2)  # modes 1, 7 (and 6)
+    got = fh.compress_with_selection(img, sels, wm_counter=ctypes.c_uint32(0xFFFFFFFF))
+    bw = w // 4
+    blocks = img.reshape(h // 4, 4, bw, 4).transpose(0, 2, 1, 3).reshape(-1, 16)
+    rng = np.random.default_rng(5)
+    checked = 0
+    for b in np.concatenate([rng.integers(1 << 20, n, 20), rng.integers(0, 1 << 20, 6)]):
+        px = blocks[b]
+        if (px == px[0]).all() or not (px >> 24).any():
+            continue
+        ref, _ = oracle.bc7_compress_selected(px.reshape(4, 4), sels[b:b + 1])
+        assert np.array_equal(got[16 * b:16 * b + 16], ref), b
+        checked += 1
+    assert checked >= 15

@@ -152,3 +152,27 @@ def test_stats_device_entry_point(fh, oracle):
     with pytest.raises(fh.FastHipError):
         fh.compress_with_stats_device(d_img.data_ptr(), enc.data_ptr(), enc.numel(),
                                       rec.data_ptr(), nb - 1, 128, 64, wm_counter=_wm())
+
+
+def test_stats_multi_call_job(fh, oracle):
+    """A job of more than 2^20 blocks runs as consecutive calls: the records
+    of the second call land at their blocks (sampled against the oracle)."""
+    w, h = 4104, 4096  # 1,050,624 blocks
+    img = synth.generate(w, h, 11)
+    print("stats multi-call 4104x4096", flush=True)
+    got, st = fh.compress_with_stats(img, wm_counter=_wm())
+    bw = w // 4
+    blocks = img.reshape(h // 4, 4, bw, 4).transpose(0, 2, 1, 3).reshape(-1, 16)
+    rng = np.random.default_rng(4)
+    picks = np.concatenate([rng.integers((1 << 20) - 64, 1 << 20, 6),
+                            rng.integers(1 << 20, st.size, 18), rng.integers(0, 1 << 20, 8)])
+    checked = 0
+    for k in picks:
+        if st["path"][k] == 0:
+            continue  # a solid block's watermark entry depends on the blocks before it
+        ref, _, rst = oracle.bc7_compress_logged(blocks[k].reshape(4, 4))
+        assert np.array_equal(got[16 * k:16 * k + 16], ref), k
+        for f in st.dtype.names:
+            assert np.array_equal(st[f][k], rst[f][0]), (k, f)
+        checked += 1
+    assert checked >= 20
