@@ -12,9 +12,11 @@
  * whose results are not pinned by a golden vector, a closed form, an
  * invariant or a brute-force check are marked "parity unpinned" in their
  * comment.  Parity unpinned: the cluster fits of CompressCluster (RGB and
- * alpha/rotation variants, with GetPrincipalAxis and ClampEndpointsToGrid)
- * and Pack's multi-subset field order; their float trajectories need the
- * reference BC7 encoder, which cannot be built without its CMake step.
+ * alpha/rotation variants, with GetPrincipalAxis and ClampEndpointsToGrid);
+ * their float trajectories need the reference BC7 encoder, which cannot be
+ * built without its CMake step.  Pack's multi-subset layout is pinned by the
+ * equality of a fit's internal error and its decoded error (modes 0, 1, 3, 6
+ * and 7, outside the reference's two endpoint quirks).
  */
 #ifndef FASTC_ORACLE_H_
 #define FASTC_ORACLE_H_

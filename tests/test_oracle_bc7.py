@@ -433,3 +433,49 @@ def test_single_color_brute_force(oracle):
                     assert a in [_expand(k, bits, pl if pbt else -1) for k in range(1 << bits)]
                     assert b in [_expand(k, bits, ph if pbt else -1) for k in range(1 << bits)]
                     assert abs((((64 - w) * a + w * b + 32) >> 6) - int(col[ch])) == best_d[ch]
+
+
+def _pbits_equal_per_subset(blk, mode):
+    """Whether both endpoints of every subset carry the same p-bit (the p-bits
+    follow the endpoint fields: modes 0, 3, 6, 7 at bits 77, 94, 63, 94)."""
+    start, nsub = {0: (77, 3), 3: (94, 2), 6: (63, 1), 7: (94, 2)}[mode]
+    bits = int.from_bytes(bytes(blk), "little")
+    return all(((bits >> (start + 2 * s)) & 1) == ((bits >> (start + 2 * s + 1)) & 1)
+               for s in range(nsub))
+
+
+def test_internal_error_equals_decoded_error(oracle):
+    """Under the uniform metric a fit's internal error (its QuantizedError sums,
+    Compressor.cpp:538-630) equals the squared error of the packed block as the
+    BC7 specification decodes it, wherever the reference's two endpoint quirks
+    do not apply:
+      * Pack swaps the endpoints of a subset whose anchor index has its top bit
+        set but not their p-bits (Compressor.cpp:1149-1170), so with unique
+        p-bits (modes 0, 3, 6, 7) the equality holds when both p-bits agree;
+      * for modes without p-bits GetPBitCombo returns {0, 0}, not null
+        (CompressionMode.h:243-250), so QuantizedError quantises the endpoints
+        as if they carried a zero p-bit (RGBAEndpoints.cpp:207-209) while Pack
+        writes them without one: mode 2 is left out.
+    Mode 1 (shared p-bits) always satisfies it.  A field of Pack's multi-subset
+    layout in the wrong place would decode to other colours and break it."""
+    from fastc_amd import synth
+    rng = np.random.default_rng(21)
+    img = synth.generate(64, 64, 9) | np.uint32(0xFF000000)
+    blocks = [img[y:y + 4, x:x + 4].reshape(16).copy() for y in range(0, 64, 4)
+              for x in range(0, 64, 4)]
+    blocks = [b for b in blocks if not (b == b[0]).all()][:60]
+    for _ in range(20):
+        blocks.append((rng.integers(0, 2**24, 16) | 0xFF000000).astype(np.uint32))
+    checked = {m: 0 for m in (0, 1, 3, 6, 7)}
+    for px in blocks:
+        for mode in checked:
+            for slot in ((0, 5, 13) if mode != 6 else (0,)):
+                out, err = oracle.bc7_encode_candidate(px, mode, slot)
+                if mode != 1 and not _pbits_equal_per_subset(out, mode):
+                    continue
+                dec = oracle.bc7_decode_block_spec(out)
+                sse = sum(int(((int(px[i]) >> (8 * k)) & 255) - ((int(dec[i]) >> (8 * k)) & 255)) ** 2
+                          for i in range(16) for k in range(4))
+                assert err == sse, (mode, slot, err, sse)
+                checked[mode] += 1
+    assert checked[1] == 240 and min(checked.values()) >= 20, checked
