@@ -432,6 +432,8 @@ fh_status fh_compress_with_stats_device(const fh_job *j, fh_bc7_block_stats *sta
   if (s != FH_OK) return s;
   if (count && (!stats || stats_count < count))
     return fail(FH_ERR_BUFFER_TOO_SMALL, "statistics buffer too small");
+  if (!aligned(stats, alignof(fh_bc7_block_stats)))
+    return fail(FH_ERR_INVALID_ARGUMENT, "statistics records must be 8-byte aligned");
   const int dev = current_device(&s);
   if (dev < 0) return s;
   DeviceState *st = device_state(dev);

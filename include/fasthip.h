@@ -160,8 +160,8 @@ typedef struct fh_bc7_block_stats {
  * (Core/src/TexComp.cpp:87-108). */
 fh_status fh_compress_with_stats(const fh_job *job, fh_bc7_block_stats *stats,
                                  size_t stats_count);
-/* The same on device buffers (job in/out and stats in device memory),
- * asynchronous on hip_stream, as fh_compress_device. */
+/* The same on device buffers (job in/out and stats in device memory, the
+ * records 8-byte aligned), asynchronous on hip_stream, as fh_compress_device. */
 fh_status fh_compress_with_stats_device(const fh_job *job, fh_bc7_block_stats *stats,
                                         size_t stats_count, void *hip_stream);
 

@@ -250,3 +250,15 @@ def test_stats_entry_point_validation():
     j.out_size = 4 * 16
     assert L.fh_compress_with_stats(ctypes.byref(j), st.ctypes.data, 3) == 5
     assert L.fh_compress_with_stats(ctypes.byref(j), None, 0) == 5
+
+
+def test_stats_device_entry_point_checks():
+    """The device entry point rejects a short or misaligned record buffer
+    before any device work."""
+    img = np.zeros((8, 8), np.uint32)
+    out = np.zeros(64, np.uint8)
+    j = fastc_amd._job(3, img.ctypes.data, out.ctypes.data, out.nbytes, 8, 8, None, 0, 0, 0xFF,
+                       None, True, None, 4, 0.6)
+    L = fastc_amd.lib()
+    assert L.fh_compress_with_stats_device(ctypes.byref(j), ctypes.c_void_p(4096), 3, None) == 5
+    assert L.fh_compress_with_stats_device(ctypes.byref(j), ctypes.c_void_p(4100), 4, None) == 1
