@@ -1121,15 +1121,16 @@ constexpr unsigned long long kAnnealMarginDiv = 50;
 // scan, then each chunk writes its passing states from its offset.
 constexpr uint32_t kListChunk = 4096;
 
+// Flags of the states cb + 256 k + t (k = 0..15) of thread t: bit k set when
+// the state passes the margin.  Adjacent lanes read adjacent states.
 __device__ __forceinline__ uint32_t list_flags(const uint32_t *aid, uint32_t total,
                                                const unsigned long long *cerr,
                                                const unsigned long long *q0best, uint32_t cb,
                                                int t) {
-  // thread t checks states cb + 16 t .. cb + 16 t + 15
   uint32_t flags = 0;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
-    const uint32_t i = cb + 16u * (uint32_t)t + (uint32_t)k;
+    const uint32_t i = cb + 256u * (uint32_t)k + (uint32_t)t;
     if (i < total) {
       const uint32_t id = aid[i];
       const unsigned long long best = q0best[id_block(id)];
@@ -1199,17 +1200,32 @@ __global__ void __launch_bounds__(256) list_write_kernel(const uint32_t *aid, co
                                                          const unsigned long long *q0best,
                                                          const uint32_t *chunk_offset,
                                                          uint32_t *alist) {
-  __shared__ uint32_t s_wave[4];
   const uint32_t total = counters[16];
   const uint32_t cb = blockIdx.x * kListChunk;
   if (cb >= total) return;  // whole workgroup
   const uint32_t flags = list_flags(aid, total, cerr, q0best, cb, threadIdx.x);
-  uint32_t excl;
-  wg_exclusive_scan((uint32_t)__popc(flags), &excl, s_wave);
-  uint32_t pos = chunk_offset[blockIdx.x] + excl;
+  // round k places the passing states cb + 256 k .. cb + 256 k + 255 in
+  // order: rank within the wave by ballot, then the waves before it
+  __shared__ uint32_t s_round[2][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t pos = chunk_offset[blockIdx.x];
 #pragma unroll
-  for (int k = 0; k < 16; k++)
-    if ((flags >> k) & 1u) alist[pos++] = cb + 16u * threadIdx.x + (uint32_t)k;
+  for (int k = 0; k < 16; k++) {
+    const bool f = (flags >> k) & 1u;
+    const uint64_t m = __ballot(f);
+    if (lane == 0) s_round[k & 1][wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      before += q < wave ? s_round[k & 1][q] : 0u;
+      all += s_round[k & 1][q];
+    }
+    if (f)
+      alist[pos + before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] =
+          cb + 256u * (uint32_t)k + (uint32_t)threadIdx.x;
+    pos += all;
+  }
 }
 
 struct Work {
