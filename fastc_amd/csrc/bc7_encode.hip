@@ -2712,10 +2712,11 @@ __global__ void __launch_bounds__(256) guard_kernel(BlockRange r, const uint4 *q
 // wave packs one mode) and also fills the slot of a mode-6 second shape
 // entry, whose set-up copied the first's record; the final pass walks the
 // annealing list and re-scores only the records annealing rewrote.
-// Grid-stride: the item counts are device counters.
-__global__ void __launch_bounds__(256) cand_error_kernel(BlockRange r, Work w,
-                                                         unsigned long long *cerr,
-                                                         int final_pass) {
+// Grid-stride: the item counts are device counters.  Four waves per SIMD
+// (128 VGPRs, no spills) hide the dependent loads of each item better than
+// the three its natural allocation allows.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+cand_error_kernel(BlockRange r, Work w, unsigned long long *cerr, int final_pass) {
   uint32_t total = 0;
   if (final_pass) {
     total = w.counters[18];
