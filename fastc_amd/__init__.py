@@ -151,6 +151,8 @@ def lib():
                                    ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         L.fh_compress_with_stats.argtypes = [ctypes.POINTER(FhJob), vp, ctypes.c_size_t]
         L.fh_compress_with_selection.argtypes = [ctypes.POINTER(FhJob), vp, ctypes.c_size_t]
+        L.fh_compress_with_selection_device.argtypes = [ctypes.POINTER(FhJob), vp,
+                                                        ctypes.c_size_t, vp]
         L.fh_compress_with_stats_device.argtypes = [ctypes.POINTER(FhJob), vp, ctypes.c_size_t,
                                                     vp]
         L.fh_bc7_stats_format.argtypes = [vp, ctypes.c_uint64, vp, ctypes.c_size_t,
@@ -158,7 +160,8 @@ def lib():
         for name in ("fh_decompress_device", "fh_block_error_device", "fh_image_error_device",
                      "fh_psnr", "fh_mip_level_device", "fh_pad4_device", "fh_ktx_write",
                      "fh_compress_with_stats", "fh_compress_with_stats_device",
-                     "fh_bc7_stats_format", "fh_compress_with_selection"):
+                     "fh_bc7_stats_format", "fh_compress_with_selection",
+                     "fh_compress_with_selection_device"):
             getattr(L, name).restype = ctypes.c_int
         for name in ("fh_batch_create", "fh_batch_destroy", "fh_batch_run", "fh_batch_autotune",
                      "fh_batch_blocks", "fh_batch_set_tune"):
@@ -182,7 +185,8 @@ EXPORTED_SYMBOLS = ["fh_compress", "fh_compress_device", "fh_decompress",
                     "fh_block_error_device", "fh_image_error_device", "fh_psnr_from_sse",
                     "fh_psnr", "fh_mip_level_device", "fh_pad4_device", "fh_ktx_write",
                     "fh_compress_with_stats", "fh_compress_with_stats_device",
-                    "fh_bc7_stats_format", "fh_compress_with_selection"]
+                    "fh_bc7_stats_format", "fh_compress_with_selection",
+                    "fh_compress_with_selection_device"]
 
 # fh_shape_selection (include/fasthip.h): BPTCC::ShapeSelection
 SHAPE_SELECTION_DTYPE = np.dtype([("num_shapes", "<u4"),
@@ -302,6 +306,18 @@ def compress_with_selection(img, selections, quality=0, seed=0, block_modes=0xFF
              quality, seed, block_modes, wm_counter, q0_guard, None, 4, 0.6)
     _check(lib().fh_compress_with_selection(ctypes.byref(j), sels.ctypes.data, sels.size))
     return out
+
+
+def compress_with_selection_device(in_ptr, out_ptr, out_size, sel_ptr, sel_count, width, height,
+                                   quality=0, seed=0, block_modes=0xFF, stream=0,
+                                   wm_counter=None, q0_guard=True):
+    """fh_compress_with_selection_device: device pointers for the image, the
+    blocks and `sel_count` SHAPE_SELECTION_DTYPE records; asynchronous on
+    `stream`."""
+    j = _job(eCompressionFormat_BPTC, in_ptr, out_ptr, out_size, width, height, None, quality,
+             seed, block_modes, wm_counter, q0_guard, None, 4, 0.6)
+    _check(lib().fh_compress_with_selection_device(ctypes.byref(j), ctypes.c_void_p(sel_ptr),
+                                                   sel_count, ctypes.c_void_p(stream)))
 
 
 def stats_format(stats, block_index):

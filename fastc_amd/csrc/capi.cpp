@@ -452,6 +452,29 @@ fh_status fh_compress_with_selection(const fh_job *j, const fh_shape_selection *
   return compress_host(j, false, nullptr, 0, selections, selections_count);
 }
 
+fh_status fh_compress_with_selection_device(const fh_job *j, const fh_shape_selection *selections,
+                                            size_t selections_count, void *hip_stream) {
+  fh_status s = stats_job_ok(j);
+  if (s != FH_OK) return s;
+  if (!selections) return fail(FH_ERR_INVALID_ARGUMENT, "null shape selections");
+  if (!aligned(selections, alignof(fh_shape_selection)))
+    return fail(FH_ERR_INVALID_ARGUMENT, "shape selections must be 4-byte aligned");
+  s = validate(j, true);
+  if (s != FH_OK) return s;
+  uint32_t first = 0, count = 0;
+  s = job_range(j, &first, &count);
+  if (s != FH_OK) return s;
+  if (selections_count < count) return fail(FH_ERR_BUFFER_TOO_SMALL, "too few shape selections");
+  const int dev = current_device(&s);
+  if (dev < 0) return s;
+  DeviceState *st = device_state(dev);
+  std::lock_guard<std::mutex> g(st->mu);
+  hipError_t e = ensure_init(st);
+  if (e != hipSuccess) return fail_hip(e, "device initialisation");
+  return run_device(st, j, j->in, j->out, reinterpret_cast<hipStream_t>(hip_stream), first,
+                    count, nullptr, selections);
+}
+
 fh_status fh_bc7_stats_format(const fh_bc7_block_stats *st, uint64_t block_index, char *out,
                               size_t out_capacity, size_t *out_size) {
   if (!st || !out_size) return fail(FH_ERR_INVALID_ARGUMENT, "null statistics or size");

@@ -27,7 +27,7 @@
  *                        BPTCC::CompressWithStats (BPTCCompressor.h:173,
  *                        BPTCEncoder/src/Compressor.cpp:1577-1623) and the
  *                        BlockLogger lines it writes (Compressor.cpp:1373-1390).
- *   fh_compress_with_selection
+ *   fh_compress_with_selection[_device]
  *                        BPTCC::Compress with a caller's m_ShapeSelectionFn
  *                        (BPTCCompressor.h:123-134, Compressor.cpp:1840-1858).
  *   fh_compressed_size   CompressedImage::GetCompressedSize
@@ -190,6 +190,14 @@ typedef struct fh_shape_selection {
  * would read past its partition tables). */
 fh_status fh_compress_with_selection(const fh_job *job, const fh_shape_selection *selections,
                                      size_t selections_count);
+/* The same on device buffers (job in/out and the selections in device
+ * memory, 4-byte aligned), asynchronous on hip_stream.  The selections are
+ * not checked on the host here: a selection beyond 31 candidates keeps its
+ * first 31 (in CompressClusters' order), and a shape index is taken modulo
+ * 64, so no selection can make the kernels read outside their tables. */
+fh_status fh_compress_with_selection_device(const fh_job *job,
+                                            const fh_shape_selection *selections,
+                                            size_t selections_count, void *hip_stream);
 
 /* The text BlockLogger writes for one block (Compressor.cpp:1373-1390,
  * 1945-1950): a line "<block_index>: BlockStat_<name> -- <value>" per

@@ -186,3 +186,23 @@ def test_selections_multi_call_job(fh, oracle):
         assert np.array_equal(got[16 * b:16 * b + 16], ref), b
         checked += 1
     assert checked >= 15
+
+
+@pytest.mark.gpu
+def test_selection_device_entry_point(fh, oracle):
+    """fh_compress_with_selection_device on torch buffers equals the host
+    entry point."""
+    import torch
+    img = synth.generate(64, 64, 7)
+    print("selection device entry point", flush=True)
+    sels = random_selections((64 // 4) ** 2, 9, 3)
+    want = fh.compress_with_selection(img, sels, quality=3, seed=1,
+                                      wm_counter=ctypes.c_uint32(0xFFFFFFFF))
+    d_img = torch.from_numpy(img.view(np.int32)).cuda()
+    d_sel = torch.from_numpy(sels.view(np.uint8).copy()).cuda()
+    enc = torch.zeros(64 * 64, dtype=torch.uint8, device="cuda")
+    fh.compress_with_selection_device(d_img.data_ptr(), enc.data_ptr(), enc.numel(),
+                                      d_sel.data_ptr(), sels.size, 64, 64, quality=3, seed=1,
+                                      wm_counter=ctypes.c_uint32(0xFFFFFFFF))
+    torch.cuda.synchronize()
+    assert np.array_equal(enc.cpu().numpy(), want)
