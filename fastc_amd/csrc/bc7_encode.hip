@@ -3035,8 +3035,8 @@ hipError_t bc7_upload_tables(int device) {
 
 void launch_anneal(uint32_t grid, const BlockRange &r, const Bc7Params &p, const bc7::Work &w,
                    hipStream_t stream) {
-  if (w.alist)  // learned search
-    hipLaunchKernelGGL((bc7::anneal_kernel<1, 1>), dim3(grid), dim3(64), 0, stream, r, p, w);
+  if (w.alist)  // learned search: four steps between refills
+    hipLaunchKernelGGL((bc7::anneal_kernel<4, 1>), dim3(grid), dim3(64), 0, stream, r, p, w);
   else if (p.anneal_unroll >= 4)
     hipLaunchKernelGGL((bc7::anneal_kernel<4, 3>), dim3(grid), dim3(64), 0, stream, r, p, w);
   else if (p.anneal_unroll == 2)
