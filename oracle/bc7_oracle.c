@@ -478,8 +478,9 @@ int ora_power_method(const float m[16], float eig[4], float *eigval) {
 /* RGBACluster::GetPrincipalAxis with eigOne = eigTwo = NULL
  * (RGBAEndpoints.cpp:327-428).  Reproduces the read of default-valued
  * (-1,-1,-1,-1) entries past the unique-point count in the collinearity
- * test (RGBAEndpoints.cpp:364-371).  Parity unpinned (float trajectory); the
- * power method inside it is pinned on its own. */
+ * test (RGBAEndpoints.cpp:364-371).  Pinned by the float64 eigenvector of
+ * the covariance and by known answers of the collinear shortcut with and
+ * without that padding (tests/test_oracle_fit.py). */
 static void principal_axis(const clus *c, v4 *axis) {
   v4 to[16];
   for (uint32_t i = 0; i < c->n; i++) to[i] = v4_sub(clus_pt(c, i), c->avg);
@@ -536,6 +537,7 @@ typedef struct {
   uint32_t *lcg;
   uint32_t key, ctr;
   uint64_t seed, block_id;
+  ora_fit_trace *trace; /* optional: intermediate values of the RGB fit */
 } cmode;
 
 typedef struct {
@@ -630,8 +632,8 @@ static void clamp_endpoints(v4 *p1, v4 *p2) {
   }
 }
 
-/* ClampEndpointsToGrid (Compressor.cpp:212-250).  Parity unpinned (float
- * trajectory; only checked through whole-block GPU/oracle agreement). */
+/* ClampEndpointsToGrid (Compressor.cpp:212-250).  Pinned by a float64 brute
+ * force over the p-bit combinations (tests/test_oracle_fit.py). */
 static void clamp_to_grid(const cmode *m, v4 *p1, v4 *p2, uint8_t *best_pbc) {
   const int npbc = m_npbc(m);
   const int has_pb = npbc > 1;
@@ -867,14 +869,25 @@ static double optimize_endpoints(cmode *m, const clus *c, v4 *p1, v4 *p2, uint8_
   return best;
 }
 
-/* CompressCluster, RGB variant (Compressor.cpp:921-1094).  Parity unpinned:
- * the k-means and least-squares float trajectories can only be pinned by
- * running the reference, whose BC7 encoder does not build without CMake. */
+static void trace_v4(float out[4], v4 v) {
+  for (int k = 0; k < 4; k++) out[k] = v.v[k];
+}
+
+/* CompressCluster, RGB variant (Compressor.cpp:921-1094).  Pinned at each
+ * stage through m->trace (tests/test_oracle_fit.py): the axis, the k-means
+ * fixed point, the least-squares solve, the grid clamp, and exact fits of
+ * two-colour subsets. */
 static double compress_cluster_rgb(cmode *m, const clus *c, v4 *p1, v4 *p2, uint8_t *best_idx,
                                    uint8_t *best_pbc) {
+  ora_fit_trace *tr = m->trace;
+  if (tr) memset(tr, 0, sizeof(*tr));
   if (v4_eq(c->mx, c->mn)) {
     double e = single_color(m, clus_pt(c, 0), p1, p2, best_pbc);
     for (uint32_t i = 0; i < c->n; i++) best_idx[i] = 1;
+    if (tr) {
+      tr->path = ORA_FIT_SAME_POINT;
+      trace_v4(tr->single, clus_pt(c, 0));
+    }
     return (double)c->n * e;
   }
   const uint32_t nb = 1u << m_bits_idx(m, -1);
@@ -889,6 +902,12 @@ static double compress_cluster_rgb(cmode *m, const clus *c, v4 *p1, v4 *p2, uint
   *p1 = v4_add(c->avg, v4_mul(axis, mindp));
   *p2 = v4_add(c->avg, v4_mul(axis, maxdp));
   clamp_endpoints(p1, p2);
+  if (tr) {
+    trace_v4(tr->axis, axis);
+    trace_v4(tr->start1, *p1);
+    trace_v4(tr->start2, *p2);
+    tr->nbuckets = nb;
+  }
 
   v4 pts[16];
   uint32_t npts[16];
@@ -927,6 +946,14 @@ static double compress_cluster_rgb(cmode *m, const clus *c, v4 *p1, v4 *p2, uint
     for (uint32_t i = 0; i < nb; i++)
       if (!v4_eq(pts[i], newp[i])) fixed = 0;
     for (uint32_t i = 0; i < nb; i++) pts[i] = newp[i];
+    if (tr) tr->iterations++;
+  }
+  if (tr) {
+    for (uint32_t i = 0; i < nb; i++) {
+      trace_v4(tr->centroid[i], pts[i]);
+      tr->count[i] = npts[i];
+    }
+    for (uint32_t i = 0; i < c->n; i++) tr->bucket[i] = (uint8_t)bidx[i];
   }
   int nfilled = 0, last = -1;
   for (uint32_t i = 0; i < nb; i++)
@@ -937,6 +964,10 @@ static double compress_cluster_rgb(cmode *m, const clus *c, v4 *p1, v4 *p2, uint
   if (nfilled == 1) {
     double e = single_color(m, pts[last], p1, p2, best_pbc);
     for (uint32_t i = 0; i < c->n; i++) best_idx[i] = 1;
+    if (tr) {
+      tr->path = ORA_FIT_ONE_BUCKET;
+      trace_v4(tr->single, pts[last]);
+    }
     return (double)c->n * e;
   }
   float asq = 0.0f, bsq = 0.0f, ab = 0.0f;
@@ -959,7 +990,17 @@ static double compress_cluster_rgb(cmode *m, const clus *c, v4 *p1, v4 *p2, uint
   const float f = 1.0f / (asq * bsq - ab * ab);
   *p1 = v4_mul(v4_sub(v4_mul(ax, bsq), v4_mul(bx, ab)), f);
   *p2 = v4_mul(v4_sub(v4_mul(bx, asq), v4_mul(ax, ab)), f);
+  if (tr) {
+    tr->path = ORA_FIT_LEAST_SQUARES;
+    trace_v4(tr->lsq1, *p1);
+    trace_v4(tr->lsq2, *p2);
+  }
   clamp_to_grid(m, p1, p2, best_pbc);
+  if (tr) {
+    trace_v4(tr->grid1, *p1);
+    trace_v4(tr->grid2, *p2);
+    tr->pbit_combo = *best_pbc;
+  }
   return optimize_endpoints(m, c, p1, p2, best_idx, best_pbc);
 }
 
@@ -969,8 +1010,9 @@ static float clamp_alpha(float x) {
   return t < 255.0f ? t : 255.0f;
 }
 
-/* CompressCluster, alpha/rotation variant (Compressor.cpp:632-919).  Parity
- * unpinned, including the stale-statistics quirk (see the RGB variant). */
+/* CompressCluster, alpha/rotation variant (Compressor.cpp:632-919).  Pinned
+ * by known answers of the stale-statistics quirk, of the equal-alpha mode-4
+ * table branch and of the alpha k-means (tests/test_oracle_fit.py). */
 static double compress_cluster_alpha(cmode *m, const clus *c, v4 *p1, v4 *p2, uint8_t *best_idx,
                                      uint8_t *alpha_idx) {
   if (v4_eq(c->mx, c->mn)) {
@@ -1003,6 +1045,8 @@ static double compress_cluster_alpha(cmode *m, const clus *c, v4 *p1, v4 *p2, ui
   uint8_t dummy_pbc = 0;
   v4 rp1, rp2;
   double rgb_err = compress_cluster_rgb(m, &rc, &rp1, &rp2, best_idx, &dummy_pbc);
+  ora_fit_trace *tr = m->trace;
+  if (tr) tr->rgb_error = rgb_err;
 
   float a1 = amin, a2 = amax;
   double alpha_err = DBL_MAX;
@@ -1105,6 +1149,18 @@ static double compress_cluster_alpha(cmode *m, const clus *c, v4 *p1, v4 *p2, ui
     const uint8_t amask = (uint8_t)(mask_seed >> (m->a->alpha_prec - 1));
     const uint8_t a1b = ora_quantize_channel((uint8_t)a1, amask, -1);
     const uint8_t a2b = ora_quantize_channel((uint8_t)a2, amask, -1);
+    if (tr) {
+      tr->alpha_path = ORA_FIT_LEAST_SQUARES;
+      for (uint32_t i = 0; i < nb; i++) {
+        tr->alpha_centroid[i] = vals[i];
+        tr->alpha_count[i] = (uint32_t)npts[i];
+      }
+      for (uint32_t i = 0; i < 16; i++) tr->alpha_bucket[i] = (uint8_t)buckets[i];
+      tr->alpha_lsq[0] = a1;
+      tr->alpha_lsq[1] = a2;
+      tr->alpha_q[0] = a1b;
+      tr->alpha_q[1] = a2b;
+    }
     alpha_err = 0.0;
     for (uint32_t i = 0; i < 16; i++) {
       uint8_t val = (uint8_t)avals[i];
@@ -1128,7 +1184,78 @@ static double compress_cluster_alpha(cmode *m, const clus *c, v4 *p1, v4 *p2, ui
     p1->v[i] = i == 3 ? a1 : rp1.v[i];
     p2->v[i] = i == 3 ? a2 : rp2.v[i];
   }
+  if (tr) tr->alpha_error = alpha_err;
   return rgb_err + alpha_err;
+}
+
+/* Test hooks into the cluster fits.  A cluster is the subset `subset` of
+ * partition shape `shape` of the mode's subset count (SetShapeIndex +
+ * SetPartition, RGBAEndpoints.h:205-226). */
+static void fit_setup(cmode *m, clus *c, const uint32_t px[16], int mode, int shape, int subset,
+                      int rot, int idx_mode) {
+  memset(m, 0, sizeof(*m));
+  m->a = &kModes[mode & 7];
+  m->metric = v4_splat(1.0f);
+  m->rot = rot;
+  m->idx_mode = idx_mode;
+  clus_init(c, px);
+  c->shape = (uint32_t)(m->a->subsets > 1 ? shape : 0);
+  c->nparts = (uint32_t)m->a->subsets;
+  clus_set_partition(c, (uint32_t)subset);
+}
+
+void ora_principal_axis(const uint32_t px[16], int nparts, int shape, int subset, float axis[4]) {
+  clus c;
+  clus_init(&c, px);
+  c.shape = (uint32_t)(nparts > 1 ? shape : 0);
+  c.nparts = (uint32_t)nparts;
+  clus_set_partition(&c, (uint32_t)subset);
+  v4 a;
+  principal_axis(&c, &a);
+  trace_v4(axis, a);
+}
+
+void ora_clamp_to_grid(int mode, float p1[4], float p2[4], int *pbc) {
+  cmode m;
+  memset(&m, 0, sizeof(m));
+  m.a = &kModes[mode & 7];
+  v4 a = v4_set(p1[0], p1[1], p1[2], p1[3]), b = v4_set(p2[0], p2[1], p2[2], p2[3]);
+  uint8_t bp = 0xFF;
+  clamp_to_grid(&m, &a, &b, &bp);
+  trace_v4(p1, a);
+  trace_v4(p2, b);
+  *pbc = bp;
+}
+
+double ora_fit_rgb(const uint32_t px[16], int mode, int shape, int subset, int idx_mode,
+                   ora_fit_trace *tr, float p1[4], float p2[4], uint8_t idx[16], int *pbc) {
+  cmode m;
+  clus c;
+  fit_setup(&m, &c, px, mode, shape, subset, 0, idx_mode);
+  m.trace = tr;
+  v4 a, b;
+  uint8_t bp = 0;
+  memset(idx, 0, 16);
+  const double e = compress_cluster_rgb(&m, &c, &a, &b, idx, &bp);
+  trace_v4(p1, a);
+  trace_v4(p2, b);
+  *pbc = bp;
+  return e;
+}
+
+double ora_fit_alpha(const uint32_t px[16], int mode, int rot, int idx_mode, ora_fit_trace *tr,
+                     float p1[4], float p2[4], uint8_t idx[16], uint8_t aidx[16]) {
+  cmode m;
+  clus c;
+  fit_setup(&m, &c, px, mode, 0, 0, rot, idx_mode);
+  m.trace = tr;
+  v4 a, b;
+  memset(idx, 0, 16);
+  memset(aidx, 0, 16);
+  const double e = compress_cluster_alpha(&m, &c, &a, &b, idx, aidx);
+  trace_v4(p1, a);
+  trace_v4(p2, b);
+  return e;
 }
 
 /* Fit tag for the counter RNG: identifies (mode, shape entry, subset,
@@ -1210,10 +1337,12 @@ static void bw_write(bitw *w, uint32_t val, int n) {
   }
 }
 
-/* CompressionMode::Pack (Compressor.cpp:1096-1298).  Parity unpinned for the
- * multi-subset field order (covered only by decoding what it writes).  The p-bits are written
- * from the combo as chosen, also when the anchor fix-up swapped the two
- * endpoints (Compressor.cpp:1149-1170 vs 1234-1241). */
+/* CompressionMode::Pack (Compressor.cpp:1096-1298).  Pinned for every mode by
+ * an independent BC7-specification reader in the tests (fields, anchors,
+ * rotation) and by the internal error of fits equalling their decoded
+ * error.  The p-bits are written from the combo as chosen, also when the
+ * anchor fix-up swapped the two endpoints (Compressor.cpp:1149-1170 vs
+ * 1234-1241). */
 static void mode_pack(const cmode *m, params *pr, uint8_t out[16]) {
   bitw w = {out, 0};
   const int mode = m->a->mode;
@@ -1516,6 +1645,32 @@ static double encode_candidate(const uint32_t px[16], int mode, int slot, int en
   pm.metric = v4_splat(1.0f);
   mode_pack(&pm, &pr, out);
   return e;
+}
+
+void ora_bc7_pack(int mode, int shape, int rot, int idx_mode, const float ep[3][2][4],
+                  const uint8_t pbc[3], const uint8_t idx[16], const uint8_t aidx[16],
+                  uint8_t out[16]) {
+  cmode m;
+  memset(&m, 0, sizeof(m));
+  m.a = &kModes[mode & 7];
+  m.metric = v4_splat(1.0f);
+  params pr;
+  memset(&pr, 0, sizeof(pr));
+  const int nsub = m.a->subsets;
+  pr.shape = nsub > 1 ? shape : 0;
+  pr.rot = m.a->has_rot ? rot : 0;
+  pr.idx_mode = m.a->has_idx_mode ? idx_mode : 0;
+  for (int s = 0; s < nsub; s++) {
+    pr.p1[s] = v4_set(ep[s][0][0], ep[s][0][1], ep[s][0][2], ep[s][0][3]);
+    pr.p2[s] = v4_set(ep[s][1][0], ep[s][1][1], ep[s][1][2], ep[s][1][3]);
+    pr.pbc[s] = pbc[s];
+  }
+  for (int i = 0; i < 16; i++) {
+    pr.idx[subset_for(i, pr.shape, nsub)][i] = idx[i];
+    pr.aidx[i] = aidx[i];
+  }
+  memset(out, 0, 16);
+  mode_pack(&m, &pr, out);
 }
 
 double ora_bc7_encode_candidate(const uint32_t px[16], int mode, int slot, uint8_t out[16]) {

@@ -10,13 +10,16 @@
  *
  * Every exported function cites the reference lines it follows.  Functions
  * whose results are not pinned by a golden vector, a closed form, an
- * invariant or a brute-force check are marked "parity unpinned" in their
- * comment.  Parity unpinned: the cluster fits of CompressCluster (RGB and
- * alpha/rotation variants, with GetPrincipalAxis and ClampEndpointsToGrid);
- * their float trajectories need the reference BC7 encoder, which cannot be
- * built without its CMake step.  Pack's multi-subset layout is pinned by the
- * equality of a fit's internal error and its decoded error (modes 0, 1, 3, 6
- * and 7, outside the reference's two endpoint quirks).
+ * invariant or a brute-force check would be marked "parity unpinned" in their
+ * comment; none is.  The reference BC7 encoder cannot be built without its
+ * CMake step, so its cluster fits are pinned stage by stage instead
+ * (tests/test_oracle_fit.py): GetPrincipalAxis against the float64
+ * eigenvector and hand-derived known answers of its collinear shortcut,
+ * k-means at its fixed point, least squares against the float64 normal
+ * equations, ClampEndpointsToGrid by float64 brute force, whole fits by
+ * exact two-colour subsets, the rotation fit by known answers of its
+ * stale-statistics and equal-alpha branches, BoxSelection by a rescan of all
+ * shapes, and Pack by an independent reader of the BC7 specification.
  */
 #ifndef FASTC_ORACLE_H_
 #define FASTC_ORACLE_H_
@@ -191,6 +194,51 @@ void ora_table_partitions(uint16_t shape2[64], uint16_t shape3[128], uint8_t anc
 /* CompressionMode::CompressSingleColor (Compressor.cpp:252-353) for one
  * mode and index mode (rotation 0): endpoints p1/p2 (8-bit values as
  * floats), p-bit combination and the per-pixel error. */
+/* Intermediate values of one RGB cluster fit (CompressCluster,
+ * Compressor.cpp:921-1094), for the tests. */
+enum { ORA_FIT_SAME_POINT = 1, ORA_FIT_ONE_BUCKET = 2, ORA_FIT_LEAST_SQUARES = 3 };
+typedef struct {
+  int path;               /* ORA_FIT_* */
+  float single[4];        /* the colour given to CompressSingleColor (paths 1, 2) */
+  float axis[4];          /* GetPrincipalAxis */
+  float start1[4], start2[4];  /* avg + {min,max} dp * axis, clamped */
+  uint32_t nbuckets;
+  uint32_t iterations;    /* k-means passes until nothing moved */
+  float centroid[16][4];  /* k-means centroids at the fixed point */
+  uint32_t count[16];     /* points per bucket */
+  uint8_t bucket[16];     /* bucket of each point, cluster order */
+  float lsq1[4], lsq2[4]; /* least-squares endpoints (path 3) */
+  float grid1[4], grid2[4];  /* after ClampEndpointsToGrid */
+  int pbit_combo;
+  /* rotation fits only (modes 4 and 5) */
+  double rgb_error;        /* the rotated RGB fit's error */
+  double alpha_error;      /* the alpha term */
+  int alpha_path;          /* ORA_FIT_LEAST_SQUARES when the alpha values differ */
+  float alpha_centroid[8]; /* 1-D k-means at its fixed point */
+  uint32_t alpha_count[8];
+  uint8_t alpha_bucket[16];
+  float alpha_lsq[2];      /* least squares, clamped to [0, 255] */
+  uint8_t alpha_q[2];      /* quantised after truncation to uint8 */
+} ora_fit_trace;
+
+/* GetPrincipalAxis of subset `subset` of shape `shape` (nparts subsets). */
+void ora_principal_axis(const uint32_t px[16], int nparts, int shape, int subset, float axis[4]);
+/* ClampEndpointsToGrid of `mode` (in place; combo index in *pbc). */
+void ora_clamp_to_grid(int mode, float p1[4], float p2[4], int *pbc);
+/* CompressCluster (RGB) of one subset at quality 0; trace optional.
+ * idx receives the cluster-order indices. */
+double ora_fit_rgb(const uint32_t px[16], int mode, int shape, int subset, int idx_mode,
+                   ora_fit_trace *tr, float p1[4], float p2[4], uint8_t idx[16], int *pbc);
+/* CompressCluster (alpha/rotation, modes 4 and 5) at quality 0; the trace
+ * records the fit of the rotated RGB cluster. */
+double ora_fit_alpha(const uint32_t px[16], int mode, int rot, int idx_mode, ora_fit_trace *tr,
+                     float p1[4], float p2[4], uint8_t idx[16], uint8_t aidx[16]);
+/* CompressionMode::Pack of given parameters: ep[subset][endpoint][channel]
+ * (unquantised floats, quantised by Pack), p-bit combos per subset, the
+ * colour (idx) and alpha (aidx) index of every pixel. */
+void ora_bc7_pack(int mode, int shape, int rot, int idx_mode, const float ep[3][2][4],
+                  const uint8_t pbc[3], const uint8_t idx[16], const uint8_t aidx[16],
+                  uint8_t out[16]);
 double ora_single_color(int mode, int idx_mode, const float p[4], float p1[4], float p2[4],
                         int *pbc);
 uint32_t ora_counter_rng(uint32_t key, uint32_t n);

@@ -32,6 +32,25 @@ class BlockStats(ctypes.Structure):
                 ("q0_kept", ctypes.c_int)]
 
 
+class FitTrace(ctypes.Structure):
+    """ora_fit_trace: intermediate values of one RGB cluster fit."""
+    _fields_ = [("path", ctypes.c_int), ("single", ctypes.c_float * 4),
+                ("axis", ctypes.c_float * 4), ("start1", ctypes.c_float * 4),
+                ("start2", ctypes.c_float * 4), ("nbuckets", ctypes.c_uint32),
+                ("iterations", ctypes.c_uint32), ("centroid", (ctypes.c_float * 4) * 16),
+                ("count", ctypes.c_uint32 * 16), ("bucket", ctypes.c_uint8 * 16),
+                ("lsq1", ctypes.c_float * 4), ("lsq2", ctypes.c_float * 4),
+                ("grid1", ctypes.c_float * 4), ("grid2", ctypes.c_float * 4),
+                ("pbit_combo", ctypes.c_int), ("rgb_error", ctypes.c_double),
+                ("alpha_error", ctypes.c_double), ("alpha_path", ctypes.c_int),
+                ("alpha_centroid", ctypes.c_float * 8), ("alpha_count", ctypes.c_uint32 * 8),
+                ("alpha_bucket", ctypes.c_uint8 * 16), ("alpha_lsq", ctypes.c_float * 2),
+                ("alpha_q", ctypes.c_uint8 * 2)]
+
+
+FIT_SAME_POINT, FIT_ONE_BUCKET, FIT_LEAST_SQUARES = 1, 2, 3
+
+
 # ora_bc7_log_stats / fh_bc7_block_stats
 LOG_STATS_DTYPE = np.dtype([("path", "<i4"), ("mode", "<i4"), ("num_two_shape", "<u4"),
                             ("num_three_shape", "<u4"), ("mode_estimate", "<f8", (8,)),
@@ -102,6 +121,16 @@ def lib():
         L.ora_etc1_decompress.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, vp]
         L.ora_block_errors.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp]
         L.ora_block_errors.restype = ctypes.c_uint64
+        L.ora_principal_axis.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+        L.ora_clamp_to_grid.argtypes = [ctypes.c_int, vp, vp, ctypes.POINTER(ctypes.c_int)]
+        L.ora_fit_rgb.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(FitTrace), vp, vp, vp,
+                                  ctypes.POINTER(ctypes.c_int)]
+        L.ora_fit_rgb.restype = ctypes.c_double
+        L.ora_fit_alpha.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(FitTrace), vp, vp, vp, vp]
+        L.ora_fit_alpha.restype = ctypes.c_double
+        L.ora_bc7_pack.argtypes = [ctypes.c_int] * 4 + [vp] * 5
         _lib = L
     return _lib
 
@@ -356,3 +385,57 @@ def adam_step(w, m, v, g, t, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
     """In place on float32 arrays w, m, v."""
     g = np.ascontiguousarray(g, dtype=np.float32)
     _mlp_lib().ora_adam_step(_p(w), _p(m), _p(v), _p(g), w.size, t, lr, beta1, beta2, eps)
+
+
+def principal_axis(px, nparts=1, shape=0, subset=0):
+    px = np.ascontiguousarray(px, dtype=np.uint32)
+    axis = np.zeros(4, np.float32)
+    lib().ora_principal_axis(_p(px), nparts, shape, subset, _p(axis))
+    return axis
+
+
+def clamp_to_grid(mode, p1, p2):
+    """ClampEndpointsToGrid: returns (p1, p2, p-bit combo index)."""
+    a = np.array(p1, dtype=np.float32)
+    b = np.array(p2, dtype=np.float32)
+    pbc = ctypes.c_int(0)
+    lib().ora_clamp_to_grid(mode, _p(a), _p(b), ctypes.byref(pbc))
+    return a, b, pbc.value
+
+
+def fit_rgb(px, mode, shape=0, subset=0, idx_mode=0):
+    """One RGB cluster fit at quality 0: (error, p1, p2, indices, combo, trace)."""
+    px = np.ascontiguousarray(px, dtype=np.uint32)
+    tr = FitTrace()
+    p1 = np.zeros(4, np.float32)
+    p2 = np.zeros(4, np.float32)
+    idx = np.zeros(16, np.uint8)
+    pbc = ctypes.c_int(0)
+    e = lib().ora_fit_rgb(_p(px), mode, shape, subset, idx_mode, ctypes.byref(tr), _p(p1),
+                          _p(p2), _p(idx), ctypes.byref(pbc))
+    return e, p1, p2, idx, pbc.value, tr
+
+
+def fit_alpha(px, mode, rot, idx_mode=0):
+    """One rotation-mode fit at quality 0: (error, p1, p2, indices, alpha
+    indices, trace of the rotated RGB fit)."""
+    px = np.ascontiguousarray(px, dtype=np.uint32)
+    tr = FitTrace()
+    p1 = np.zeros(4, np.float32)
+    p2 = np.zeros(4, np.float32)
+    idx = np.zeros(16, np.uint8)
+    aidx = np.zeros(16, np.uint8)
+    e = lib().ora_fit_alpha(_p(px), mode, rot, idx_mode, ctypes.byref(tr), _p(p1), _p(p2),
+                            _p(idx), _p(aidx))
+    return e, p1, p2, idx, aidx, tr
+
+
+def bc7_pack(mode, shape, rot, idx_mode, ep, pbc, idx, aidx):
+    """CompressionMode::Pack of explicit parameters (ep: [3][2][4] floats)."""
+    ep = np.ascontiguousarray(ep, dtype=np.float32).reshape(3, 2, 4)
+    pbc = np.ascontiguousarray(pbc, dtype=np.uint8).reshape(3)
+    idx = np.ascontiguousarray(idx, dtype=np.uint8).reshape(16)
+    aidx = np.ascontiguousarray(aidx, dtype=np.uint8).reshape(16)
+    out = np.zeros(16, np.uint8)
+    lib().ora_bc7_pack(mode, shape, rot, idx_mode, _p(ep), _p(pbc), _p(idx), _p(aidx), _p(out))
+    return out
