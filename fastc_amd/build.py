@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfasthip.so")
 SOURCES = ["capi.cpp", "tables.cpp", "timing.cpp", "ranker.cpp", "batch.cpp", "bc7_encode.hip",
-           "dxt_encode.hip", "etc1_encode.hip", "mlp.hip", "decode.hip", "mips.hip", "ktx.cpp"]
+           "dxt_encode.hip", "etc1_encode.hip", "mlp.hip", "decode.hip", "mips.hip", "ktx.cpp",
+           "static_mem.hip"]
 ARCH = os.environ.get("FASTHIP_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -35,13 +35,23 @@ fh_status fail_hip(hipError_t e, const char *what) {
 
 // Per-device state: a growable scratch pool, the process-wide watermark
 // counter (the reference's gWMVal, initially 0xFFFFFFFF) and the BC7
-// single-colour table.
+// single-colour table.  The counter and the table live in the code object
+// (static_mem.hip); `mem` holds their device addresses.
+//
+// Calls on different streams share the pool and the counter.  Each call
+// that uses them records `last_use` on its stream after its last launch, and
+// the next call's stream waits for it first, so calls on one device run
+// their encodes in call order whatever streams they use.
 struct DeviceState {
   std::mutex mu;
+  int dev = 0;
+  StaticDeviceMemory mem;
   void *scratch = nullptr;
   size_t scratch_size = 0;
-  uint32_t *counter = nullptr;  // device word
-  uint32_t *sctab = nullptr;
+  uint32_t *counter = nullptr;  // device words: mem.wm
+  uint32_t *sctab = nullptr;    // mem.sctab
+  hipEvent_t last_use = nullptr;
+  bool last_use_recorded = false;
   uint8_t *in_buf = nullptr;
   size_t in_size = 0;
   uint8_t *out_buf = nullptr;
@@ -60,27 +70,53 @@ std::vector<DeviceState *> g_devices;
 DeviceState *device_state(int dev) {
   std::lock_guard<std::mutex> g(g_devices_mu);
   if ((int)g_devices.size() <= dev) g_devices.resize(dev + 1, nullptr);
-  if (!g_devices[dev]) g_devices[dev] = new DeviceState();
+  if (!g_devices[dev]) {
+    g_devices[dev] = new DeviceState();
+    g_devices[dev]->dev = dev;
+  }
   return g_devices[dev];
 }
 
+// One-time set-up per device (caller holds st->mu, the device current):
+// look up the code-object memory and fill the single-colour table.  Nothing
+// here allocates device memory.
 hipError_t ensure_init(DeviceState *st) {
   if (st->init) return hipSuccess;
-  hipError_t e = hipMalloc(&st->counter, 2 * sizeof(uint32_t));
+  hipError_t e = static_device_memory(st->dev, &st->mem);
   if (e != hipSuccess) return e;
-  const uint32_t c0[2] = {0xFFFFFFFFu, 0};
-  e = hipMemcpy(st->counter, c0, sizeof(c0), hipMemcpyHostToDevice);
+  static uint32_t tab[kSingleColorEntries];
+  static std::once_flag built;
+  std::call_once(built, [] { build_bc7_single_color_table(tab); });
+  e = hipMemcpy(st->mem.sctab, tab, sizeof(tab), hipMemcpyHostToDevice);
   if (e != hipSuccess) return e;
-  std::vector<uint32_t> tab(kSingleColorEntries);
-  build_bc7_single_color_table(tab.data());
-  e = hipMalloc(&st->sctab, tab.size() * sizeof(uint32_t));
-  if (e != hipSuccess) return e;
-  e = hipMemcpy(st->sctab, tab.data(), tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
-  if (e != hipSuccess) return e;
-  e = hipMalloc(&st->sse, 2 * sizeof(unsigned long long));
-  if (e != hipSuccess) return e;
+  st->counter = st->mem.wm;
+  st->sctab = st->mem.sctab;
   st->init = true;
   return hipSuccess;
+}
+
+bool capturing(hipStream_t stream) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+// Orders this call's use of the shared pool and counter after the previous
+// call's (see DeviceState).  Not while the stream is being captured: a graph
+// is ordered by its caller.
+hipError_t order_after_last_use(DeviceState *st, hipStream_t stream) {
+  if (!st->last_use_recorded || capturing(stream)) return hipSuccess;
+  return hipStreamWaitEvent(stream, st->last_use, 0);
+}
+
+hipError_t record_use(DeviceState *st, hipStream_t stream) {
+  if (capturing(stream)) return hipSuccess;
+  if (!st->last_use) {
+    hipError_t e = hipEventCreateWithFlags(&st->last_use, hipEventDisableTiming);
+    if (e != hipSuccess) return e;
+  }
+  hipError_t e = hipEventRecord(st->last_use, stream);
+  if (e == hipSuccess) st->last_use_recorded = true;
+  return e;
 }
 
 bool aligned(const void *p, size_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
@@ -118,6 +154,13 @@ uint32_t block_bytes(uint32_t format) {
     default:
       return 0;
   }
+}
+
+// Decoder formats: the four codecs, and BPTC | FH_DECODE_SPEC (the BC7
+// specification's decode); -1 for anything else.
+int decode_format(uint32_t format) {
+  if (format == (FH_FORMAT_BPTC | FH_DECODE_SPEC)) return kFormatBptcSpec;
+  return block_bytes(format) ? (int)format : -1;
 }
 
 // The block range a (sub-)job covers in raster order, following the loop of
@@ -168,6 +211,10 @@ fh_status validate(const fh_job *j, bool device) {
     return fail(FH_ERR_BUFFER_TOO_SMALL, "Not enough space for compressed data!");
   if (j->format == FH_FORMAT_BPTC && j->error_metric != 0)
     return fail(FH_ERR_UNSUPPORTED, "only the uniform error metric is implemented");
+  // a mask without any of modes 0-7 leaves no candidate for any searched
+  // block (the reference asserts, Compressor.cpp:1857)
+  if (j->format == FH_FORMAT_BPTC && j->block_modes && !(j->block_modes & 0xFFu))
+    return fail(FH_ERR_INVALID_ARGUMENT, "block_modes selects no BC7 mode");
   if ((j->flags & FH_FLAG_CALLER_WATERMARK) && !j->wm_counter)
     return fail(FH_ERR_INVALID_ARGUMENT, "FH_FLAG_CALLER_WATERMARK needs wm_counter");
   if ((j->flags & FH_FLAG_LEARNED) && j->format == FH_FORMAT_BPTC) {
@@ -178,6 +225,15 @@ fh_status validate(const fh_job *j, bool device) {
       return fail(FH_ERR_INVALID_ARGUMENT, "ranker belongs to another device");
   }
   return FH_OK;
+}
+
+// Bytes of device scratch one BC7 call of up to `blocks` blocks takes: the
+// watermark scan's prefixes, then the work area.
+size_t bc7_call_bytes(uint32_t blocks, bool learned, size_t *work_off) {
+  const uint32_t nchunks = (blocks + 255) / 256;
+  const size_t prefix_bytes = ((size_t)blocks + 255) & ~(size_t)255;
+  *work_off = (prefix_bytes + (size_t)nchunks * 4 + 255) & ~(size_t)255;
+  return *work_off + bc7_work_bytes(blocks, learned) + 256;
 }
 
 // Runs the encoder on device buffers.  Caller holds st->mu.
@@ -198,11 +254,17 @@ fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8
     case FH_FORMAT_BPTC: {
       const uint32_t chunk = kBc7MaxBlocksPerCall;
       const uint32_t per = count < chunk ? count : chunk;
-      const uint32_t nchunks = (per + 255) / 256;
       const size_t prefix_bytes = ((size_t)per + 255) & ~(size_t)255;
-      const size_t work_off = (prefix_bytes + (size_t)nchunks * 4 + 255) & ~(size_t)255;
       const bool learned = (j->flags & FH_FLAG_LEARNED) != 0;
-      const size_t need = work_off + bc7_work_bytes(per, learned) + 256;
+      size_t work_off = 0;
+      const size_t need = bc7_call_bytes(per, learned, &work_off);
+      e = order_after_last_use(st, stream);
+      if (e != hipSuccess) return fail_hip(e, "stream ordering");
+      if (st->scratch_size < need && st->last_use_recorded) {
+        // the pool is about to be replaced: the previous call must be done with it
+        e = hipEventSynchronize(st->last_use);
+        if (e != hipSuccess) return fail_hip(e, "previous encode");
+      }
       e = grow(&st->scratch, &st->scratch_size, need);
       if (e != hipSuccess) return fail_hip(e, "scratch allocation");
       uint8_t *sbase = reinterpret_cast<uint8_t *>(st->scratch);
@@ -244,6 +306,8 @@ fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8
         e = bc7_encode(rc, p, s, work, stream);
         if (e != hipSuccess) return fail_hip(e, "BC7 encode launch");
       }
+      e = record_use(st, stream);
+      if (e != hipSuccess) return fail_hip(e, "stream ordering");
       if (j->flags & FH_FLAG_CALLER_WATERMARK) {
         e = hipMemcpyAsync(&caller_c, s.counter, 4, hipMemcpyDeviceToHost, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -306,7 +370,30 @@ int selection_candidates(const fh_shape_selection &g, uint32_t block_modes, uint
   return n;
 }
 
-fh_status check_selections(const fh_shape_selection *sels, uint32_t count, const fh_job *j) {
+// Whether block `b` (raster index) of a host image takes the selection
+// function's path: neither one colour nor fully transparent
+// (CompressBC7Block, Compressor.cpp:1819-1860).
+bool searched_block(const fh_job *j, uint32_t b) {
+  const uint32_t bw = j->width / 4;
+  const uint32_t *img = reinterpret_cast<const uint32_t *>(j->in);
+  const uint32_t x = (b % bw) * 4, y = (b / bw) * 4;
+  const uint32_t p0 = img[(size_t)y * j->width + x];
+  bool same = true, transparent = true;
+  for (uint32_t r = 0; r < 4; r++)
+    for (uint32_t c = 0; c < 4; c++) {
+      const uint32_t p = img[(size_t)(y + r) * j->width + x + c];
+      same = same && p == p0;
+      transparent = transparent && (p >> 24) == 0;
+    }
+  return !same && !transparent;
+}
+
+// Caller selections: shape indices inside the tables and at most 31
+// candidates per block; with a host image (first = the job's first block),
+// every searched block must get at least one candidate, where the reference
+// asserts (Compressor.cpp:1857).
+fh_status check_selections(const fh_shape_selection *sels, uint32_t count, const fh_job *j,
+                           bool host_image = false, uint32_t first = 0) {
   const uint32_t modes = j->block_modes ? (j->block_modes & 0xFFu) : 0xFFu;
   for (uint32_t k = 0; k < count; k++) {
     const fh_shape_selection &g = sels[k];
@@ -316,8 +403,11 @@ fh_status check_selections(const fh_shape_selection *sels, uint32_t count, const
       if ((parts == 2 || parts == 3) && g.shapes[e].index >= 64)
         return fail(FH_ERR_INVALID_ARGUMENT, "shape index beyond the partition tables");
     }
-    if (selection_candidates(g, modes, j->quality) > 31)
-      return fail(FH_ERR_UNSUPPORTED, "a shape selection with more than 31 candidates");
+    const int n = selection_candidates(g, modes, j->quality);
+    if (n > 31) return fail(FH_ERR_UNSUPPORTED, "a shape selection with more than 31 candidates");
+    if (n == 0 && host_image && searched_block(j, first + k))
+      return fail(FH_ERR_INVALID_ARGUMENT,
+                  "a shape selection that leaves a searched block no mode to try");
   }
   return FH_OK;
 }
@@ -347,7 +437,7 @@ fh_status compress_host(const fh_job *j, bool with_stats, fh_bc7_block_stats *st
     return fail(FH_ERR_BUFFER_TOO_SMALL, "statistics buffer too small");
   if (sels) {
     if (sels_count < count) return fail(FH_ERR_BUFFER_TOO_SMALL, "too few shape selections");
-    s = check_selections(sels, count, j);
+    s = check_selections(sels, count, j, true, first);
     if (s != FH_OK) return s;
   }
   const int dev = current_device(&s);
@@ -535,8 +625,9 @@ fh_status fh_decompress(uint32_t format, const uint8_t *in, size_t in_size, uint
                         uint32_t width, uint32_t height) {
   if (!in || !out || width == 0 || height == 0)
     return fail(FH_ERR_INVALID_ARGUMENT, "null buffer or empty image");
-  const uint32_t bb = block_bytes(format);
-  if (!bb) return fail(FH_ERR_UNKNOWN_FORMAT, "Unknown compression format");
+  const int df = decode_format(format);
+  if (df < 0) return fail(FH_ERR_UNKNOWN_FORMAT, "Unknown compression format");
+  const uint32_t bb = block_bytes(format & 0xFFu);
   if (!decode_dims_ok(width, height)) return fail(FH_ERR_INVALID_ARGUMENT, "image too large");
   const size_t need = (size_t)((width + 3) / 4) * ((height + 3) / 4) * bb;
   if (in_size < need) return fail(FH_ERR_BUFFER_TOO_SMALL, "compressed buffer too small");
@@ -554,8 +645,8 @@ fh_status fh_decompress(uint32_t format, const uint8_t *in, size_t in_size, uint
   if (e != hipSuccess) return fail_hip(e, "device buffer allocation");
   e = hipMemcpy(st->in_buf, in, need, hipMemcpyHostToDevice);
   if (e != hipSuccess) return fail_hip(e, "input upload");
-  e = texture_decode(format, st->in_buf, reinterpret_cast<uint32_t *>(st->out_buf), width, height,
-                     nullptr);
+  e = texture_decode((uint32_t)df, st->in_buf, reinterpret_cast<uint32_t *>(st->out_buf), width,
+                     height, nullptr);
   if (e != hipSuccess) return fail_hip(e, "decode launch");
   e = hipMemcpy(out, st->out_buf, out_bytes, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return fail_hip(e, "output download");
@@ -565,11 +656,12 @@ fh_status fh_decompress(uint32_t format, const uint8_t *in, size_t in_size, uint
 fh_status fh_decompress_device(uint32_t format, const uint8_t *in, uint8_t *out, uint32_t width,
                                uint32_t height, void *hip_stream) {
   if (!in || !out) return fail(FH_ERR_INVALID_ARGUMENT, "null buffer");
-  if (!block_bytes(format)) return fail(FH_ERR_UNKNOWN_FORMAT, "Unknown compression format");
+  const int df = decode_format(format);
+  if (df < 0) return fail(FH_ERR_UNKNOWN_FORMAT, "Unknown compression format");
   if (!decode_dims_ok(width, height)) return fail(FH_ERR_INVALID_ARGUMENT, "image too large");
-  if (!aligned(in, block_bytes(format)) || !aligned(out, 4))
+  if (!aligned(in, block_bytes(format & 0xFFu)) || !aligned(out, 4))
     return fail(FH_ERR_INVALID_ARGUMENT, "misaligned buffer (blocks: block size, pixels: 4 B)");
-  hipError_t e = texture_decode(format, in, reinterpret_cast<uint32_t *>(out), width, height,
+  hipError_t e = texture_decode((uint32_t)df, in, reinterpret_cast<uint32_t *>(out), width, height,
                                 reinterpret_cast<hipStream_t>(hip_stream));
   if (e != hipSuccess) return fail_hip(e, "decode launch");
   return FH_OK;
@@ -580,12 +672,13 @@ fh_status fh_block_error_device(uint32_t format, const uint32_t *original, const
                                 unsigned long long *block_errors, unsigned long long *sse,
                                 void *hip_stream) {
   if (!original || !encoded || !sse) return fail(FH_ERR_INVALID_ARGUMENT, "null buffer");
-  if (!block_bytes(format)) return fail(FH_ERR_UNKNOWN_FORMAT, "Unknown compression format");
+  const int df = decode_format(format);
+  if (df < 0) return fail(FH_ERR_UNKNOWN_FORMAT, "Unknown compression format");
   if (!decode_dims_ok(width, height)) return fail(FH_ERR_INVALID_ARGUMENT, "image too large");
-  if (!aligned(encoded, block_bytes(format)) || !aligned(original, 4) || !aligned(sse, 8) ||
-      !aligned(block_errors, 8))
+  if (!aligned(encoded, block_bytes(format & 0xFFu)) || !aligned(original, 4) ||
+      !aligned(sse, 8) || !aligned(block_errors, 8))
     return fail(FH_ERR_INVALID_ARGUMENT, "misaligned buffer");
-  hipError_t e = block_error((int)format, original, encoded, width, height, block_errors, sse,
+  hipError_t e = block_error(df, original, encoded, width, height, block_errors, sse,
                              reinterpret_cast<hipStream_t>(hip_stream));
   if (e != hipSuccess) return fail_hip(e, "error launch");
   return FH_OK;
@@ -649,6 +742,7 @@ fh_status fh_psnr(const uint8_t *a, const uint8_t *b, uint32_t width, uint32_t h
   const size_t bytes = (size_t)width * height * 4;
   e = grow(reinterpret_cast<void **>(&st->in_buf), &st->in_size, bytes);
   if (e == hipSuccess) e = grow(reinterpret_cast<void **>(&st->out_buf), &st->out_size, bytes);
+  if (e == hipSuccess && !st->sse) e = hipMalloc(&st->sse, 2 * sizeof(unsigned long long));
   if (e != hipSuccess) return fail_hip(e, "device buffer allocation");
   e = hipMemcpy(st->in_buf, a, bytes, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(st->out_buf, b, bytes, hipMemcpyHostToDevice);

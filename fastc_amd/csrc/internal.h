@@ -67,6 +67,21 @@ constexpr uint32_t kBc7MlpChunk = 1u << 18;
 // Largest block count one bc7_encode call takes; callers split bigger jobs.
 constexpr uint32_t kBc7MaxBlocksPerCall = 1u << 20;
 
+// Entries of the BC7 single-colour endpoint table (tables.cpp).
+constexpr int kSingleColorEntries = 5 * 5 * 4 * 256;
+
+// Devices the per-device tables cover.
+constexpr int kMaxDevices = 64;
+
+// Device addresses of the memory kept in the code object (static_mem.hip):
+// the watermark counter words and the BC7 single-colour table.  Looked up
+// once per device with that device current.
+struct StaticDeviceMemory {
+  uint32_t *wm = nullptr;
+  uint32_t *sctab = nullptr;
+};
+hipError_t static_device_memory(int dev, StaticDeviceMemory *out);
+
 hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch &s, void *work,
                       hipStream_t stream);
 
@@ -133,6 +148,8 @@ hipError_t bc7_scatter(const SurfDesc *surfs, const uint32_t *block_surf, uint32
 // per-block premultiplied errors (block_err, optional) and their 128-bit
 // total in sse[0] (low) and sse[1] (high), both overwritten.
 constexpr int kFormatRgba8 = -1;
+// BPTC decoded by the BC7 specification (FH_FORMAT_BPTC | FH_DECODE_SPEC).
+constexpr int kFormatBptcSpec = 4;
 
 // Mip levels and padding (mips.hip): dst = max(1, w/2) x max(1, h/2) box-filtered
 // texels; pad4 copies w x h into ceil4(w) x ceil4(h) with transparent black.
@@ -202,7 +219,7 @@ void build_bc7_mode5_table(uint8_t out[256][2]);
 void build_bc7_dxt6_table(uint8_t out[256][2][3]);
 void build_bc7_single_color_table(uint32_t *out);  // kSingleColorEntries entries
 
-constexpr int kSingleColorEntries = 5 * 5 * 4 * 256;
+
 
 // Sets the message fh_last_error returns on this thread (capi.cpp).
 void set_error(const char *msg);

@@ -78,3 +78,28 @@ def test_no_device_is_reported_not_emulated():
     assert _status(img) == 7
     with pytest.raises(fastc_amd.FastHipError):
         fastc_amd.compress(img)
+
+
+def test_modes_that_leave_no_candidate_are_rejected():
+    """The reference asserts when a searched block has no mode to try
+    (Compressor.cpp:1857): a mode mask without any of modes 0-7, or a caller
+    selection that allows nothing on a block that is neither one colour nor
+    transparent, is an invalid argument (checked before any device work)."""
+    L = fastc_amd.lib()
+    img = np.arange(64, dtype=np.uint32).reshape(8, 8) | np.uint32(0xFF000000)
+    out = np.zeros(64, np.uint8)
+    j = fastc_amd._job(3, img.ctypes.data, out.ctypes.data, 64, 8, 8, None, 0, 0, 0x100, None)
+    assert L.fh_compress(ctypes.byref(j)) == 1
+    assert b"no BC7 mode" in L.fh_last_error()
+    sels = np.zeros(4, dtype=fastc_amd.SHAPE_SELECTION_DTYPE)
+    sels["num_shapes"] = 1
+    sels["shapes"]["num_partitions"][:, 0] = 2
+    sels["selected_modes"] = 1 << 1
+    sels["selected_modes"][2] = 1 << 0  # mode 0 needs three subsets: nothing to try
+    j = fastc_amd._job(3, img.ctypes.data, out.ctypes.data, 64, 8, 8, None, 0, 0, 0xFF, None)
+    assert L.fh_compress_with_selection(ctypes.byref(j), sels.ctypes.data, 4) == 1
+    assert b"no mode" in L.fh_last_error()
+    # the same empty selection on a one-colour block is never consulted
+    img[4:8, 0:4] = 0xFF123456
+    if not fastc_amd.device_available():
+        assert L.fh_compress_with_selection(ctypes.byref(j), sels.ctypes.data, 4) == 7

@@ -72,7 +72,10 @@ def random_selections(n, seed, quality, block_modes=0xFF):
             for e in range(10):
                 s["shapes"][e] = (rng.choice([1, 2, 3]), rng.integers(0, 64))
             s["selected_modes"] = rng.integers(1, 256)
-            if count_candidates(s, block_modes, quality) <= 31:
+            # at least one candidate: the reference asserts on a searched
+            # block with none (Compressor.cpp:1857) and the host entry point
+            # rejects such a selection
+            if 1 <= count_candidates(s, block_modes, quality) <= 31:
                 break
         sels[k] = s
     return sels
