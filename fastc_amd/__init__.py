@@ -21,6 +21,9 @@ eCompressionFormat_DXT1 = 0
 eCompressionFormat_DXT5 = 1
 eCompressionFormat_ETC1 = 2
 eCompressionFormat_BPTC = 3
+# or'ed into a BPTC format for decoding: the BC7 specification's decode
+# (FH_DECODE_SPEC) instead of the reference decoder's
+DECODE_SPEC = 0x100
 
 BLOCK_BYTES = {0: 8, 1: 16, 2: 8, 3: 16}
 

@@ -3,8 +3,10 @@
 //
 // Like the reference, mode 4 blocks with index mode 1 interpolate with the
 // per-mode index widths rather than the swapped ones (Decompressor.cpp:263-295),
-// so decoded pixels match the reference decoder byte for byte.  Blocks with
-// the reserved mode (no mode bit in the first byte) decode to zero.
+// so decoded pixels match the reference decoder byte for byte.  With SPEC the
+// widths swap with the indices, as the BC7 specification (and hardware)
+// decodes such blocks.  Blocks with the reserved mode (no mode bit in the
+// first byte) decode to zero.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -82,6 +84,7 @@ __device__ __forceinline__ uint32_t lerp_chan(uint32_t e0, uint32_t e1, uint32_t
 // pixels in raster order.  Every field position is computed from the mode's
 // attributes, so lanes holding blocks of different modes run one path and
 // nothing is indexed dynamically in private memory.
+template <bool SPEC = false>
 __device__ inline void decode_block(const uint4 raw, uint32_t px[16]) {
   const uint64_t lo = (uint64_t)raw.x | ((uint64_t)raw.y << 32);
   const uint64_t hi = (uint64_t)raw.z | ((uint64_t)raw.w << 32);
@@ -164,9 +167,11 @@ __device__ inline void decode_block(const uint4 raw, uint32_t px[16]) {
     const uint32_t sub = ((m1 >> i) & 1u) ? 1u + ((m2 >> i) & 1u) : 0u;
     const uint32_t e0 = sub == 0 ? ep[0][0] : (sub == 1 ? ep[1][0] : ep[2][0]);
     const uint32_t e1 = sub == 0 ? ep[0][1] : (sub == 1 ? ep[1][1] : ep[2][1]);
-    const uint32_t wc = weight1(ci, ib);
-    const bool cvalid = ci < (1u << ib);
-    const uint32_t na = aib ? aib : ib;
+    // index widths: the reference keeps the per-mode widths after the swap
+    const uint32_t cb = SPEC && im ? aib : ib;
+    const uint32_t na = SPEC && im ? ib : (aib ? aib : ib);
+    const uint32_t wc = weight1(ci, cb);
+    const bool cvalid = ci < (1u << cb);
     const uint32_t wa = weight1(ai, na);
     const bool avalid = ai < (1u << na);
     uint32_t pixel = 0;

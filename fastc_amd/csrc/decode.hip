@@ -152,6 +152,8 @@ __device__ __forceinline__ void decode_one(const uint8_t *in, uint32_t b, uint32
   } else if constexpr (FMT == FH_FORMAT_ETC1) {
     const uint2 v = reinterpret_cast<const uint2 *>(in)[b];
     etc1_block(v.x, v.y, px);
+  } else if constexpr (FMT == kFormatBptcSpec) {
+    bc7dec::decode_block<true>(reinterpret_cast<const uint4 *>(in)[b], px);
   } else {
     bc7dec::decode_block(reinterpret_cast<const uint4 *>(in)[b], px);
   }
@@ -319,6 +321,7 @@ hipError_t texture_decode(uint32_t format, const uint8_t *in, uint32_t *out, uin
     case FH_FORMAT_DXT5: e = dec::launch_decode<FH_FORMAT_DXT5>(in, out, width, height, stream); break;
     case FH_FORMAT_ETC1: e = dec::launch_decode<FH_FORMAT_ETC1>(in, out, width, height, stream); break;
     case FH_FORMAT_BPTC: e = dec::launch_decode<FH_FORMAT_BPTC>(in, out, width, height, stream); break;
+    case kFormatBptcSpec: e = dec::launch_decode<kFormatBptcSpec>(in, out, width, height, stream); break;
   }
   timing_end(t, kTagDecode, stream);
   return e;
@@ -346,6 +349,9 @@ hipError_t block_error(int format, const uint32_t *original, const void *encoded
       break;
     case FH_FORMAT_BPTC:
       e = dec::launch_error<FH_FORMAT_BPTC>(original, encoded, width, height, block_err, sse, stream);
+      break;
+    case kFormatBptcSpec:
+      e = dec::launch_error<kFormatBptcSpec>(original, encoded, width, height, block_err, sse, stream);
       break;
   }
   timing_end(t, kTagBlockError, stream);

@@ -75,6 +75,14 @@ enum {
   FH_FORMAT_BPTC = 3
 };
 
+/* Or'ed into the format of fh_decompress, fh_decompress_device and
+ * fh_block_error_device with FH_FORMAT_BPTC: decode by the BC7 specification
+ * instead of the reference decoder.  No counterpart in the reference: its
+ * decoder (Decompressor.cpp:263-295) keeps the per-mode index widths for
+ * mode 4 blocks with index mode 1, which the specification (and hardware)
+ * swaps with the indices; every other block decodes the same. */
+enum { FH_DECODE_SPEC = 0x100 };
+
 /* The BC7 candidate ranker (see the ranker section below). */
 typedef struct fh_ranker fh_ranker;
 

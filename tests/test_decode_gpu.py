@@ -150,3 +150,45 @@ def test_full_size_bc7_error_consistency(fh, oracle):
     assert int(s[0]) == int(b1.cpu().numpy().view(np.uint64).sum(dtype=np.uint64))
     decoded = dec.cpu().numpy().view(np.uint32).reshape(n, n)
     assert math.isclose(fh.psnr_from_sse(s, n, n), oracle.psnr(img, decoded), rel_tol=1e-12)
+
+
+def test_bc7_spec_decode(fh, oracle):
+    """FH_DECODE_SPEC: BPTC decoded by the BC7 specification equals the
+    oracle's specification decoder (pinned by an independent reader in
+    tests/test_oracle_fit.py) on random blocks of every mode, index-mode-1
+    mode-4 blocks included; the fused error of the same decode matches."""
+    rng = np.random.default_rng(71)
+    n = 4096
+    raw = rng.integers(0, 256, (n, 16), dtype=np.uint8)
+    for k in range(n):  # one mode per block, round robin; every 4th mode-4 block index mode 1
+        mode = k % 8
+        v = int.from_bytes(raw[k].tobytes(), "little")
+        v = (v >> (mode + 1) << (mode + 1)) | (1 << mode)
+        if mode == 4 and k % 3 == 0:
+            v |= 1 << 7
+        raw[k] = np.frombuffer(v.to_bytes(16, "little"), np.uint8)
+    w, h = 512, 128
+    data = raw.reshape(-1)
+    got = fh.decompress(data, w, h, fmt=3 | fh.DECODE_SPEC)
+    want = np.zeros((h, w), np.uint32)
+    for k in range(n):
+        by, bx = divmod(k, w // 4)
+        want[4 * by:4 * by + 4, 4 * bx:4 * bx + 4] = \
+            oracle.bc7_decode_block_spec(raw[k]).reshape(4, 4)
+    assert np.array_equal(got, want)
+    ref = fh.decompress(data, w, h, fmt=3)
+    assert not np.array_equal(ref, got)  # the mode-4 index-mode-1 blocks differ
+    img = synth.generate(w, h, 5)
+    d_img = torch.from_numpy(img.view(np.int32)).cuda()
+    d_enc = torch.from_numpy(data.copy()).cuda()
+    sse = torch.zeros(2, dtype=torch.int64, device="cuda")
+    blk = torch.zeros(n, dtype=torch.int64, device="cuda")
+    fh.block_error_device(d_img.data_ptr(), d_enc.data_ptr(), w, h, sse.data_ptr(),
+                          blk.data_ptr(), fmt=3 | fh.DECODE_SPEC)
+    torch.cuda.synchronize()
+    be = blk.cpu().numpy().view(np.uint64)
+    for k in range(0, n, 97):
+        by, bx = divmod(k, w // 4)
+        src = img[4 * by:4 * by + 4, 4 * bx:4 * bx + 4].reshape(16)
+        assert be[k] == oracle.block_premul_err(src, want[4 * by:4 * by + 4,
+                                                          4 * bx:4 * bx + 4].reshape(16))
