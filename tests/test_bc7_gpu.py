@@ -182,14 +182,25 @@ def test_bc7_multi_call_job_q0(fh, oracle):
 
 
 def test_bc7_q50_psnr_vs_reference_statistics(fh, oracle):
-    """PSNR at q=50 is within 0.05 dB of the oracle run with the reference's
-    own LCG annealing stream (mean of three seeds), or better."""
+    """PSNR at q=50 without the quality guard (the reference algorithm) is
+    within 0.05 dB of the oracle run with the reference's own LCG annealing
+    stream, or better: mean of five GPU seeds against the mean of five
+    time-seed-like LCG seeds, both without the guard (it is tested on its
+    own).  The reference seeds its LCG with time(NULL) (Compressor.cpp:358),
+    so only the distribution can be compared."""
+    from concurrent.futures import ThreadPoolExecutor
     img = synth.generate(256, 256, 1)
-    got = fh.compress(img, quality=50, seed=11, wm_counter=_wm())
-    pg = oracle.psnr(img, fh.decompress(got, 256, 256))
-    refs = []
-    for s in (1, 2, 3):
+    gpu = []
+    for s in (11, 12, 13, 14, 15):
+        got = fh.compress(img, quality=50, seed=s, wm_counter=_wm(), q0_guard=False)
+        gpu.append(oracle.psnr(img, fh.decompress(got, 256, 256)))
+
+    def ref_run(s):
         ref, _ = oracle.bc7_compress(img, quality=50, rng=oracle.RNG_LCG, lcg_seed=s)
-        refs.append(oracle.psnr(img, oracle.bc7_decompress(ref, 256, 256)))
-    print(f"gpu {pg:.4f} ref {np.mean(refs):.4f} +- {np.std(refs):.4f}", flush=True)
-    assert pg >= np.mean(refs) - 0.05
+        return oracle.psnr(img, oracle.bc7_decompress(ref, 256, 256))
+
+    with ThreadPoolExecutor(5) as ex:
+        refs = list(ex.map(ref_run, (1, 2, 3, 1764000000, 1764000001)))
+    print(f"gpu {np.mean(gpu):.4f} +- {np.std(gpu):.4f}, ref {np.mean(refs):.4f} +- "
+          f"{np.std(refs):.4f}", flush=True)
+    assert np.mean(gpu) >= np.mean(refs) - 0.05

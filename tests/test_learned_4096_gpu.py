@@ -1,0 +1,89 @@
+"""The learned search at its benchmark workload: the 4096x4096 seed-3 image,
+quality 50, the committed trained ranker, top-4 candidates, 0.6 confidence
+threshold (the configuration the headline figure is quoted on).
+
+The image is 2^20 blocks, so the encode ranks them in four chunks of 2^18
+(the fused inference kernel's chunk).  Blocks sampled from every chunk, and
+the 64 blocks on each side of every chunk boundary, are encoded by the
+oracle from the same pruning decision (top softmax probability against the
+threshold on the ranker's logits, top-4 candidates or BoxSelection's) and
+must match bit for bit; no block of the image may decode worse than its own
+quality-0 encoding."""
+import ctypes
+import os
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+from fastc_amd import synth
+from tests.test_bc7_learned_gpu import _box_candidates, _logits
+
+pytestmark = pytest.mark.gpu
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "fastc_amd", "data", "bc7_ranker.npy")
+N = 4096
+CHUNK = 1 << 18
+
+
+def _premul_errors(img_blocks, dec_blocks):
+    a = img_blocks.view(np.uint8).reshape(-1, 16, 4).astype(np.int64)
+    b = dec_blocks.view(np.uint8).reshape(-1, 16, 4).astype(np.int64)
+    d = a[:, :, :3] * a[:, :, 3:4] - b[:, :, :3] * b[:, :, 3:4]
+    return (d * d).sum(axis=(1, 2))
+
+
+def test_learned_4096_matches_oracle_across_chunks(fh, oracle):
+    img = synth.generate(N, N, 3)
+    blocks = synth.blocks(img)
+    rk = fh.Ranker()
+    rk.set_params(np.load(WEIGHTS))
+    print("learned 4096x4096 q=50 top-4 threshold 0.6", flush=True)
+    got = fh.compress(img, quality=50, seed=3, wm_counter=ctypes.c_uint32(0xFFFFFFFF),
+                      ranker=rk, topk=4, threshold=0.6).reshape(-1, 16)
+    q0 = fh.compress(img, quality=0, seed=3, wm_counter=ctypes.c_uint32(0xFFFFFFFF),
+                     ranker=rk, topk=4, threshold=0.6)
+    # no block worse than its own quality-0 encoding (reference decoder)
+    e50 = _premul_errors(blocks, synth.blocks(fh.decompress(got.reshape(-1), N, N)))
+    e0 = _premul_errors(blocks, synth.blocks(fh.decompress(q0, N, N)))
+    assert (e50 <= e0).all(), int((e50 > e0).sum())
+
+    rng = np.random.default_rng(5)
+    picks = [rng.integers(c * CHUNK, (c + 1) * CHUNK, 120) for c in range(4)]
+    picks += [np.arange(b - 64, b + 64) for b in (CHUNK, 2 * CHUNK, 3 * CHUNK)]
+    picks = np.unique(np.concatenate(picks))
+    z = _logits(fh, rk, blocks[picks])
+    valid = oracle.mlp_valid_mask()
+    zz = np.where(valid, z, -np.inf)
+    pmax = 1.0 / np.exp(zz - zz.max(1, keepdims=True)).sum(1)
+    solid = (blocks == blocks[:, :1]).all(axis=1)
+    nsolid_before = np.concatenate([[0], np.cumsum(solid)[:-1]])
+
+    def want(i):
+        b = int(picks[i])
+        px = blocks[b]
+        if solid[b] or not ((px >> 24) != 0).any():
+            ref, _ = oracle.bc7_compress_block(px, quality=0, block_id=b,
+                                               wm_index=int(nsolid_before[b] % 9))
+            return b, ref, "plain"
+        if abs(pmax[i] - 0.6) < 1e-4:
+            return b, None, "tie"  # float32 on the GPU could decide either way
+        if pmax[i] >= 0.6:
+            order = sorted(np.nonzero(valid)[0], key=lambda j: (-z[i, j], j))[:4]
+            cands = [(int(j) >> 6, int(j) & 63, 0) for j in order]
+            kind = "confident"
+        else:
+            cands = _box_candidates(oracle, px, 0xFF, 50)
+            kind = "fallback"
+        return b, oracle.bc7_encode_learned(px, cands, quality=50, seed=3, block_id=b), kind
+
+    with ThreadPoolExecutor(16) as ex:
+        results = list(ex.map(want, range(len(picks))))
+    kinds = {}
+    for b, ref, kind in results:
+        kinds[kind] = kinds.get(kind, 0) + 1
+        if ref is not None:
+            assert np.array_equal(got[b], ref), (b, kind)
+    print(f"checked {len(picks)} blocks: {kinds}", flush=True)
+    assert kinds.get("confident", 0) >= 20 and kinds.get("fallback", 0) >= 200, kinds
