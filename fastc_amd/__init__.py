@@ -106,6 +106,8 @@ def lib():
         L.fh_compress.restype = ctypes.c_int
         L.fh_compress_device.argtypes = [ctypes.POINTER(FhJob), ctypes.c_void_p]
         L.fh_compress_device.restype = ctypes.c_int
+        L.fh_compress_device_noalloc.argtypes = [ctypes.POINTER(FhJob), ctypes.c_void_p]
+        L.fh_compress_device_noalloc.restype = ctypes.c_int
         L.fh_decompress.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t,
                                     ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
         L.fh_decompress.restype = ctypes.c_int

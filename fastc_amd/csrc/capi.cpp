@@ -35,8 +35,9 @@ fh_status fail_hip(hipError_t e, const char *what) {
 
 // Per-device state: a growable scratch pool, the process-wide watermark
 // counter (the reference's gWMVal, initially 0xFFFFFFFF) and the BC7
-// single-colour table.  The counter and the table live in the code object
-// (static_mem.hip); `mem` holds their device addresses.
+// single-colour table.  The counter, the table and the no-allocation work
+// area live in the code object (static_mem.hip); `mem` holds their device
+// addresses.
 //
 // Calls on different streams share the pool and the counter.  Each call
 // that uses them records `last_use` on its stream after its last launch, and
@@ -236,11 +237,14 @@ size_t bc7_call_bytes(uint32_t blocks, bool learned, size_t *work_off) {
   return *work_off + bc7_work_bytes(blocks, learned) + 256;
 }
 
-// Runs the encoder on device buffers.  Caller holds st->mu.
+// Runs the encoder on device buffers.  Caller holds st->mu.  With
+// fixed_work, BC7 works in that area (fixed_size bytes, large enough for the
+// job) and nothing is allocated; otherwise in the device's growable pool.
 fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8_t *dout,
                      hipStream_t stream, uint32_t first, uint32_t count,
                      fh_bc7_block_stats *dstats = nullptr,
-                     const fh_shape_selection *dsels = nullptr) {
+                     const fh_shape_selection *dsels = nullptr,
+                     uint8_t *fixed_work = nullptr, size_t fixed_size = 0) {
   BlockRange r;
   r.img = reinterpret_cast<const uint32_t *>(din);
   r.out = dout;
@@ -258,16 +262,21 @@ fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8
       const bool learned = (j->flags & FH_FLAG_LEARNED) != 0;
       size_t work_off = 0;
       const size_t need = bc7_call_bytes(per, learned, &work_off);
-      e = order_after_last_use(st, stream);
-      if (e != hipSuccess) return fail_hip(e, "stream ordering");
-      if (st->scratch_size < need && st->last_use_recorded) {
-        // the pool is about to be replaced: the previous call must be done with it
-        e = hipEventSynchronize(st->last_use);
-        if (e != hipSuccess) return fail_hip(e, "previous encode");
+      uint8_t *sbase = fixed_work;
+      if (fixed_work) {
+        if (need > fixed_size) return fail(FH_ERR_INVALID_ARGUMENT, "job too large for the work area");
+      } else {
+        e = order_after_last_use(st, stream);
+        if (e != hipSuccess) return fail_hip(e, "stream ordering");
+        if (st->scratch_size < need && st->last_use_recorded) {
+          // the pool is about to be replaced: the previous call must be done with it
+          e = hipEventSynchronize(st->last_use);
+          if (e != hipSuccess) return fail_hip(e, "previous encode");
+        }
+        e = grow(&st->scratch, &st->scratch_size, need);
+        if (e != hipSuccess) return fail_hip(e, "scratch allocation");
+        sbase = reinterpret_cast<uint8_t *>(st->scratch);
       }
-      e = grow(&st->scratch, &st->scratch_size, need);
-      if (e != hipSuccess) return fail_hip(e, "scratch allocation");
-      uint8_t *sbase = reinterpret_cast<uint8_t *>(st->scratch);
       Bc7Scratch s;
       s.local_prefix = sbase;
       s.chunk_prefix = reinterpret_cast<uint32_t *>(sbase + prefix_bytes);
@@ -306,8 +315,10 @@ fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8
         e = bc7_encode(rc, p, s, work, stream);
         if (e != hipSuccess) return fail_hip(e, "BC7 encode launch");
       }
-      e = record_use(st, stream);
-      if (e != hipSuccess) return fail_hip(e, "stream ordering");
+      if (!fixed_work) {
+        e = record_use(st, stream);
+        if (e != hipSuccess) return fail_hip(e, "stream ordering");
+      }
       if (j->flags & FH_FLAG_CALLER_WATERMARK) {
         e = hipMemcpyAsync(&caller_c, s.counter, 4, hipMemcpyDeviceToHost, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -619,6 +630,37 @@ fh_status fh_compress_device(const fh_job *j, void *hip_stream) {
   if (e != hipSuccess) return fail_hip(e, "device initialisation");
   return run_device(st, j, j->in, j->out, reinterpret_cast<hipStream_t>(hip_stream), first,
                     count);
+}
+
+fh_status fh_compress_device_noalloc(const fh_job *j, void *hip_stream) {
+  fh_status s = validate(j, true);
+  if (s != FH_OK) return s;
+  if (j->format != FH_FORMAT_BPTC)
+    return fail(FH_ERR_UNSUPPORTED,
+                "fh_compress_device_noalloc encodes BPTC only (DXT/ETC1 use no work area: "
+                "call fh_compress_device)");
+  if (j->flags & (FH_FLAG_LEARNED | FH_FLAG_CALLER_WATERMARK))
+    return fail(FH_ERR_UNSUPPORTED,
+                "fh_compress_device_noalloc takes neither FH_FLAG_LEARNED nor "
+                "FH_FLAG_CALLER_WATERMARK");
+  uint32_t first = 0, count = 0;
+  s = job_range(j, &first, &count);
+  if (s != FH_OK) return s;
+  if (count > kNoAllocMaxBlocks)
+    return fail(FH_ERR_INVALID_ARGUMENT,
+                "job larger than the no-allocation work area (at most 16384 blocks, "
+                "e.g. 512x512 texels; split it into sub-jobs)");
+  const int dev = current_device(&s);
+  if (dev < 0) return s;
+  DeviceState *st = device_state(dev);
+  std::lock_guard<std::mutex> g(st->mu);
+  hipError_t e = ensure_init(st);
+  if (e != hipSuccess) return fail_hip(e, "device initialisation");
+  size_t work_off = 0;
+  if (bc7_call_bytes(kNoAllocMaxBlocks, false, &work_off) > st->mem.scratch_bytes)
+    return fail(FH_ERR_HIP, "no-allocation work area smaller than its block limit needs");
+  return run_device(st, j, j->in, j->out, reinterpret_cast<hipStream_t>(hip_stream), first,
+                    count, nullptr, nullptr, st->mem.scratch, st->mem.scratch_bytes);
 }
 
 fh_status fh_decompress(uint32_t format, const uint8_t *in, size_t in_size, uint8_t *out,

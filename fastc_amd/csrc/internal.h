@@ -73,12 +73,19 @@ constexpr int kSingleColorEntries = 5 * 5 * 4 * 256;
 // Devices the per-device tables cover.
 constexpr int kMaxDevices = 64;
 
+// fh_compress_device_noalloc: jobs of at most this many blocks (a 512x512
+// image) run in a fixed per-device work area of kNoAllocScratchBytes.
+constexpr uint32_t kNoAllocMaxBlocks = 1u << 14;
+constexpr size_t kNoAllocScratchBytes = (size_t)72 << 20;
+
 // Device addresses of the memory kept in the code object (static_mem.hip):
-// the watermark counter words and the BC7 single-colour table.  Looked up
-// once per device with that device current.
+// the watermark counter words, the BC7 single-colour table and the no-alloc
+// work area.  Looked up once per device with that device current.
 struct StaticDeviceMemory {
   uint32_t *wm = nullptr;
   uint32_t *sctab = nullptr;
+  uint8_t *scratch = nullptr;
+  size_t scratch_bytes = 0;
 };
 hipError_t static_device_memory(int dev, StaticDeviceMemory *out);
 

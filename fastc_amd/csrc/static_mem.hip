@@ -1,6 +1,6 @@
 // Device memory the library keeps in the code object instead of allocating:
-// the process-wide BC7 watermark counter and the BC7 single-colour endpoint
-// table.
+// the process-wide BC7 watermark counter, the BC7 single-colour endpoint
+// table, and the fixed scratch area of fh_compress_device_noalloc.
 //
 // In host code the names below are host-side stand-ins, not device
 // addresses.  static_device_memory() looks each one up with
@@ -20,6 +20,8 @@ namespace {
 // initially 0xFFFFFFFF; wm_words[1]: staging word for a caller's counter.
 __device__ __align__(256) uint32_t wm_words[2] = {0xFFFFFFFFu, 0u};
 __device__ __align__(256) uint32_t sc_table[kSingleColorEntries];
+// fh_compress_device_noalloc's work area (kNoAllocScratchBytes per device)
+__device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
 
 std::mutex g_mu;
 StaticDeviceMemory g_mem[kMaxDevices];
@@ -36,6 +38,9 @@ hipError_t static_device_memory(int dev, StaticDeviceMemory *out) {
     if (e != hipSuccess) return e;
     e = hipGetSymbolAddress(reinterpret_cast<void **>(&m.sctab), HIP_SYMBOL(sc_table));
     if (e != hipSuccess) return e;
+    e = hipGetSymbolAddress(reinterpret_cast<void **>(&m.scratch), HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    m.scratch_bytes = kNoAllocScratchBytes;
     g_mem[dev] = m;
     g_done[dev] = true;
   }

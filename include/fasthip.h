@@ -132,6 +132,19 @@ typedef struct fh_job {
 fh_status fh_compress(const fh_job *job);
 fh_status fh_compress_device(const fh_job *job, void *hip_stream);
 
+/* No counterpart in the reference.  fh_compress_device for BPTC jobs that
+ * allocates nothing, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The encode works in a fixed area kept in the library's code
+ * object, 72 MiB per device, held until the process ends; jobs of at most
+ * 16384 blocks (e.g. 512x512 texels; split larger images into sub-jobs).
+ * Same output as fh_compress_device, the process-wide watermark counter
+ * included.  Not with FH_FLAG_LEARNED or FH_FLAG_CALLER_WATERMARK
+ * (FH_ERR_UNSUPPORTED), BPTC only.  Every call on a device shares that
+ * device's area: calls that overlap on one device (other threads, other
+ * streams not ordered by the caller) race. */
+fh_status fh_compress_device_noalloc(const fh_job *job, void *hip_stream);
+
 /* One block's statistics, in the vocabulary of the log CompressWithStats
  * writes (EBlockStats, BPTCEncoder/src/Compressor.cpp:77-131; the statistics
  * version of CompressBC7Block, :1954-2183). */

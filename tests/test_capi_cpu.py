@@ -80,6 +80,24 @@ def test_no_device_is_reported_not_emulated():
         fastc_amd.compress(img)
 
 
+def test_noalloc_rejects_what_needs_allocation():
+    """fh_compress_device_noalloc validates before touching a device: jobs
+    over its 16384-block work area, other formats and the learned search are
+    refused with a message."""
+    L = fastc_amd.lib()
+    img = np.zeros((4, 4), np.uint32)
+    out = np.zeros(16, np.uint8)
+    j = fastc_amd._job(3, img.ctypes.data, out.ctypes.data, 1 << 30, 1024, 1024, None, 0, 0,
+                       0xFF, None)
+    assert L.fh_compress_device_noalloc(ctypes.byref(j), None) == 1
+    assert b"16384" in L.fh_last_error()
+    j = fastc_amd._job(0, img.ctypes.data, out.ctypes.data, 16, 4, 4, None, 0, 0, 0xFF, None)
+    assert L.fh_compress_device_noalloc(ctypes.byref(j), None) == 6
+    j = fastc_amd._job(3, img.ctypes.data, out.ctypes.data, 16, 4, 4, None, 0, 0, 0xFF,
+                       ctypes.c_uint32(0))
+    assert L.fh_compress_device_noalloc(ctypes.byref(j), None) == 6
+
+
 def test_modes_that_leave_no_candidate_are_rejected():
     """The reference asserts when a searched block has no mode to try
     (Compressor.cpp:1857): a mode mask without any of modes 0-7, or a caller
