@@ -58,6 +58,10 @@ def parse():
     ap.add_argument("--cpu-tile", type=int, default=512,
                     help="side of the image tile the CPU baseline encodes")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-other-formats", action="store_true",
+                    help="leave the DXT1/DXT5/ETC1 ring figures out of the BC7 line")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="leave the 512-texture batch figures out of the BC7 line")
     ap.add_argument("--no-learned", action="store_true",
                     help="time only the full search")
     ap.add_argument("--ranker", default=DEFAULT_RANKER, help="trained ranker weights (.npy)")
@@ -87,54 +91,137 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_per_launch": n, "launches": 3}))
 
 
-def cpu_baseline(img, quality, tile, seed):
-    """Time the CPU oracle on the top-left tile x tile region, split into
-    row bands over the host cores (the oracle releases the GIL)."""
+def host_info():
+    """Cores this process may use and the CPU model (from /proc/cpuinfo)."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": cores, "cpu_model": model}
+
+
+def _bands(rows, parts):
+    return [(i * rows // parts, (i + 1) * rows // parts) for i in range(parts)]
+
+
+def cpu_baseline(img, quality, tile, seed, runs=5):
+    """The CPU oracle (the reference BC7 algorithm with the reference's LCG
+    annealing stream, built -O3 like the reference's Release) on the top-left
+    tile x tile region, split into row bands over every host core (the oracle
+    releases the GIL).  `runs` runs with different LCG seeds, since the
+    reference seeds its stream with time(NULL): throughput from the mean
+    wall time, PSNR as mean and spread, under both decoders."""
     from oracle import pyoracle
     pyoracle.lib()
-    cores = max(1, min(16, os.cpu_count() or 1))
+    host = host_info()
+    cores = host["nproc"]
     sub = np.ascontiguousarray(img[:tile, :tile])
-    rows = tile // 4
-    bands = [(i * rows // cores, (i + 1) * rows // cores) for i in range(cores)]
-    out = np.zeros((tile // 4) * (tile // 4) * 16, dtype=np.uint8)
-    results = [None] * cores
+    bands = _bands(tile // 4, cores)
+    walls, ps, ps_spec = [], [], []
+    out = None
+    for run in range(runs):
+        out = np.zeros((tile // 4) * (tile // 4) * 16, dtype=np.uint8)
+
+        def work(k):
+            r0, r1 = bands[k]
+            if r1 > r0:
+                pyoracle.bc7_compress(sub, quality=quality, rng=pyoracle.RNG_LCG,
+                                      lcg_seed=1000003 * (run + 1) + k,
+                                      rect=(0, 4 * r0, tile, 4 * r1), out=out)
+
+        t0 = time.perf_counter()
+        th = [threading.Thread(target=work, args=(k,)) for k in range(cores)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        walls.append(time.perf_counter() - t0)
+        ps.append(psnr(sub, pyoracle.bc7_decompress(out, tile, tile)))
+        ps_spec.append(psnr(sub, pyoracle.bc7_decompress_spec(out, tile, tile)))
+        print(f"[bench] CPU BC7 run {run + 1}/{runs}: {walls[-1]:.1f} s", file=sys.stderr,
+              flush=True)
+    texels = tile * tile
+    return {
+        "value": texels / float(np.mean(walls)) / 1e6,
+        "unit": "Mtexels/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"BC7 q={quality} on the top-left {tile}x{tile} tile of the {img.shape[1]}x"
+                  f"{img.shape[0]} seed-{seed} image ({texels} texels), {runs} runs with "
+                  f"different reference LCG seeds, {cores} threads (row bands), mean wall "
+                  f"{np.mean(walls):.2f} s",
+        "host": host,
+        "wall_s": walls,
+        "psnr_db": float(np.mean(ps)), "psnr_db_sd": float(np.std(ps)),
+        "psnr_spec_db": float(np.mean(ps_spec)), "psnr_spec_db_sd": float(np.std(ps_spec)),
+    }
+
+
+def cpu_reference_simple(fmt, img, runs=3):
+    """The reference's own stb_dxt (DXT1/DXT5) or rg_etc1 (ETC1) encoder,
+    compiled from the FasTC sources (oracle/_ref), on one image split into row
+    bands over every host core; median of `runs` runs."""
+    from oracle import pyoracle
+    L = pyoracle.ref_lib()
+    if L is None:
+        return None
+    host = host_info()
+    cores = host["nproc"]
+    h, w = img.shape
+    bb = 8 if fmt in (0, 2) else 16
+    out = np.zeros((h // 4) * (w // 4) * bb, np.uint8)
+    if fmt != 2:
+        # stb_dxt builds its tables on first use behind an unsynchronised
+        # flag: one call before the threads start
+        L.ref_dxt_compress(np.zeros(16, np.uint32).ctypes.data, 4, 4, 1, out.ctypes.data)
+    bands = _bands(h // 4, cores)
 
     def work(k):
         r0, r1 = bands[k]
         if r1 <= r0:
             return
-        # reference statistics: the LCG annealing stream, one state per thread
-        o, _ = pyoracle.bc7_compress(sub, quality=quality, rng=pyoracle.RNG_LCG,
-                                     lcg_seed=12345 + k, rect=(0, 4 * r0, tile, 4 * r1),
-                                     out=out)
-        results[k] = o
+        src = img[4 * r0:4 * r1].ctypes.data
+        dst = out[r0 * (w // 4) * bb:].ctypes.data
+        if fmt == 2:
+            L.ref_etc1_compress(src, w, 4 * (r1 - r0), dst)
+        else:
+            L.ref_dxt_compress(src, w, 4 * (r1 - r0), int(fmt == 1), dst)
 
-    t0 = time.perf_counter()
-    th = [threading.Thread(target=work, args=(k,)) for k in range(cores)]
-    for t in th:
-        t.start()
-    for t in th:
-        t.join()
-    dt = time.perf_counter() - t0
-    texels = tile * tile
-    dec = pyoracle.bc7_decompress(out, tile, tile)
-    return {
-        "value": texels / dt / 1e6,
-        "unit": "Mtexels/s",
-        "cores": cores,
-        "kind": "port",
-        "sample": f"BC7 q={quality} on the top-left {tile}x{tile} tile of the {img.shape[1]}x"
-                  f"{img.shape[0]} seed-{seed} image ({texels} texels, {dt:.1f} s wall, "
-                  f"{cores} threads, reference LCG annealing stream)",
-        "psnr_db": psnr(sub, dec),
-    }, out
+    walls = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        th = [threading.Thread(target=work, args=(k,)) for k in range(cores)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        walls.append(time.perf_counter() - t0)
+    wall = float(np.median(walls))
+    texels = h * w
+    label = {0: "DXT1 (stb_dxt)", 1: "DXT5 (stb_dxt)", 2: "ETC1 (rg_etc1, low quality)"}[fmt]
+    return {"value": texels / wall / 1e6, "unit": "Mtexels/s", "cores": cores,
+            "kind": "reference",
+            "sample": f"{label} built from the reference sources, one {w}x{h} seed-2 image "
+                      f"with hard edges, {cores} threads (row bands), median of {runs} runs "
+                      f"({wall * 1e3:.1f} ms)",
+            "host": host, "gbs": (texels * 4 + texels // 16 * bb) / wall / 1e9}
 
 
-def run_batch(args):
+def batch_leg(args, per_surface_steps=None, batch_steps=None):
     """Texture-batch workload: 512 textures of power-of-two sides 64..2048
     (log-uniform per axis), 40% RGB-only, full mip chains, 60% BPTC q=50 /
     25% DXT5 / 15% ETC1, seed 4.  The batch (one graph launch, autotuned
-    launch shapes) against one library call per surface on the same stream."""
+    launch shapes) against one library call per surface on the same stream.
+    Returns the JSON object of the workload."""
     import torch
     import fastc_amd
     from fastc_amd import batch_workload as bw
@@ -143,6 +230,8 @@ def run_batch(args):
     t0 = time.perf_counter()
     tex = bw.make_batch(count=args.textures, seed=4)
     texels = bw.texel_count(tex)
+    texels_by_format = {f: bw.texel_count([t for t in tex if t["format"] == f])
+                        for f in (1, 2, 3)}
     print(f"[bench] batch of {len(tex)} textures, {texels} texels "
           f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     surfs, keep = [], []
@@ -178,7 +267,7 @@ def run_batch(args):
             fastc_amd.compress_device(pin, pout, fastc_amd.compressed_size(fmt, w, h), w, h,
                                       fmt=fmt, quality=q, seed=args.seed, stream=sh)
 
-    ms_single = timed(per_surface, args.steps, "per-surface")
+    ms_single = timed(per_surface, per_surface_steps or args.steps, "per-surface")
     # autotune on the first textures' surfaces, then the whole batch
     nsub = sum(len(t["levels"]) for t in tex[:max(1, len(tex) // 8)])
     sub = fastc_amd.Batch(surfs[:nsub], quality=q, seed=args.seed)
@@ -186,7 +275,8 @@ def run_batch(args):
     del sub
     batch = fastc_amd.Batch(surfs, quality=q, seed=args.seed)
     batch.set_tune(tune)
-    ms_batch = timed(lambda: batch.run(use_graph=True, stream=sh), args.steps, "batch graph")
+    ms_batch = timed(lambda: batch.run(use_graph=True, stream=sh), batch_steps or args.steps,
+                     "batch graph")
     blocks = {name: batch.blocks(f) for name, f in (("dxt5", 1), ("etc1", 2), ("bptc", 3))}
     line = {
         "metric": "texture batch Mtexels/s (512 textures with mips, BPTC q=50 / DXT5 / ETC1)",
@@ -205,8 +295,14 @@ def run_batch(args):
                      "frac": algo_bytes / (ms_batch / 1e3) / 1e9 / HBM_PEAK_GBS,
                      "traffic": None, "kernel": "whole batch",
                      "algorithmic_bytes": algo_bytes},
+        "formats_texels": {name: int(texels_by_format[f]) for name, f in
+                           (("bptc", 3), ("dxt5", 1), ("etc1", 2))},
     }
-    print(json.dumps(line), flush=True)
+    return line
+
+
+def run_batch(args):
+    print(json.dumps(batch_leg(args)), flush=True)
 
 
 def run_decode(args):
@@ -344,11 +440,12 @@ def run_stats(args):
     print(json.dumps(line), flush=True)
 
 
-def run_simple(args, fmt):
+def simple_leg(args, fmt, with_cpu=True):
     """DXT1 / DXT5 / ETC1 on a ring of 16 distinct 2048x2048 images (seeds
     2..17, 5% hard edges): one batch graph of 16 surfaces, so the working set
     (256 MiB of input) exceeds the last-level cache.  HBM-bound: reported as
-    GB/s of algorithmic traffic against 8 TB/s."""
+    GB/s of algorithmic traffic against 8 TB/s, next to the reference's own
+    encoder on the host cores.  Returns the JSON object of the workload."""
     import torch
     import fastc_amd
     from fastc_amd import synth
@@ -363,6 +460,8 @@ def run_simple(args, fmt):
         d_out = torch.zeros(side * side // 16 * bb, dtype=torch.uint8, device="cuda")
         keep.append((d_in, d_out))
         surfs.append((fmt, d_in.data_ptr(), d_out.data_ptr(), side, side))
+        if i == 0:
+            first = img
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
     batch = fastc_amd.Batch(surfs)
@@ -397,7 +496,15 @@ def run_simple(args, fmt):
                      "traffic": None, "kernel": f"{name} batch kernel",
                      "algorithmic_bytes": algo},
     }
-    print(json.dumps(line), flush=True)
+    if with_cpu:
+        cb = cpu_reference_simple(fmt, first)
+        if cb is not None:
+            line["cpu_baseline"] = cb
+    return line
+
+
+def run_simple(args, fmt):
+    print(json.dumps(simple_leg(args, fmt)), flush=True)
 
 
 def main():
@@ -405,10 +512,11 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build targets one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
     if args.workload != "bc7":
-        if args.gpus != 1:
-            print(json.dumps({"error": "this build targets one GPU; --gpus must be 1"}))
-            sys.exit(2)
         if args.workload == "batch":
             run_batch(args)
         elif args.workload == "decode":
@@ -418,10 +526,6 @@ def main():
         else:
             run_simple(args, {"dxt1": 0, "dxt5": 1, "etc1": 2}[args.workload])
         return
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build targets one GPU; --gpus must be 1",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
     import torch
     import fastc_amd
     from fastc_amd import synth
@@ -448,11 +552,22 @@ def main():
         ranker = fastc_amd.Ranker()
         ranker.set_params(np.load(args.ranker))
 
-    def run(rk, label):
+    def image_error(fmt):
+        """Per-block premultiplied errors and PSNR of the encoded surface,
+        decoded on the device (fmt: the reference decoder, or with
+        DECODE_SPEC the BC7 specification's)."""
+        fastc_amd.block_error_device(d_in.data_ptr(), d_out.data_ptr(), W, H, sse.data_ptr(),
+                                     blk_err.data_ptr(), fmt=fmt, stream=sh)
+        torch.cuda.synchronize()
+        s = sse.cpu().numpy().view(np.uint64)
+        be = blk_err.cpu().numpy().view(np.uint64).reshape(H // 4, W // 4)
+        return fastc_amd.psnr_from_sse(s, W, H), be
+
+    def run(rk, label, threshold):
         def step():
             fastc_amd.compress_device(d_in.data_ptr(), d_out.data_ptr(), out_bytes, W, H,
                                       quality=args.quality, seed=args.seed, stream=sh,
-                                      ranker=rk, topk=args.topk, threshold=args.threshold)
+                                      ranker=rk, topk=args.topk, threshold=threshold)
 
         for i in range(args.warmup):
             step()
@@ -481,19 +596,19 @@ def main():
                 kernels[name] = {"avg_ms": tot / n, "launches": n,
                                  "ms_per_step": tot / args.steps}
         fastc_amd.set_kernel_timing(False)
-        # PSNR on the device: the encoded surface is decoded block by block
-        # and compared with the input (exact integer errors, one per block)
-        fastc_amd.block_error_device(d_in.data_ptr(), d_out.data_ptr(), W, H, sse.data_ptr(),
-                                     blk_err.data_ptr(), fmt=3, stream=sh)
-        torch.cuda.synchronize()
-        s = sse.cpu().numpy().view(np.uint64)
-        be = blk_err.cpu().numpy().view(np.uint64).reshape(H // 4, W // 4)
+        p_ref, be = image_error(fastc_amd.eCompressionFormat_BPTC)
+        p_spec, be_spec = image_error(fastc_amd.eCompressionFormat_BPTC | fastc_amd.DECODE_SPEC)
         return {"ms_per_step": ms, "value": texels / (ms / 1e3) / 1e6, "kernels": kernels,
-                "psnr_db": fastc_amd.psnr_from_sse(s, W, H), "wall_s": wall,
-                "block_errors": be}
+                "psnr_db": p_ref, "psnr_spec_db": p_spec, "wall_s": wall,
+                "block_errors": be, "block_errors_spec": be_spec}
 
-    full = run(None, "full search")
-    learned = run(ranker, "learned") if ranker is not None else None
+    full = run(None, "full search", args.threshold)
+    learned = no_net = None
+    if ranker is not None:
+        # every block below the threshold: BoxSelection's candidates with the
+        # learned search's decoded-error choice and annealing shortlist
+        no_net = run(ranker, "no network", 2.0)
+        learned = run(ranker, "learned", args.threshold)
     headline = full
     if learned is not None and learned["psnr_db"] >= full["psnr_db"] - 0.1:
         headline = learned
@@ -508,14 +623,21 @@ def main():
     achieved = algo_bytes / (dom_ms / 1e3) / 1e9
     traffic = None
     valu = None
-    prof = os.path.join(ROOT, "profiles", "bc7_q50_4096_counters.json")
+    prof_name = os.path.join("profiles", "bc7_q50_4096_counters.json")
+    prof = os.path.join(ROOT, prof_name)
     if os.path.exists(prof):
         with open(prof) as f:
             pc = json.load(f)
         if pc.get("kernel") == dominant and pc.get("search") == headline_search(headline, full):
             traffic = pc.get("hbm_bytes_per_launch")
-            valu = pc.get("valu")
+            valu = dict(pc.get("valu") or {})
+            valu["source"] = f"{prof_name}: rocprofv3 --pmc passes of this command (not this run)"
     search = headline_search(headline, full)
+
+    def leg(r):
+        return {"value": r["value"], "ms_per_step": r["ms_per_step"], "psnr_db": r["psnr_db"],
+                "psnr_spec_db": r["psnr_spec_db"]}
+
     line = {
         "metric": "BC7 Mtexels/s (4096\u00b2, q=50); learned-pruned vs full GPU search at "
                   "\u0394PSNR\u22640.1dB",
@@ -534,29 +656,63 @@ def main():
                    "search": search, "texels": texels, "blocks": nblocks},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": f"{prof_name} (FETCH_SIZE/WRITE_SIZE passes, copy-"
+                                       "calibrated; not this run)" if traffic else None,
                      "kernel": dominant, "kernel_avg_ms": dom_ms,
                      "algorithmic_bytes": algo_bytes},
         "kernels": kernels,
         "psnr_db": headline["psnr_db"],
-        "full_search": {"value": full["value"], "ms_per_step": full["ms_per_step"],
-                        "psnr_db": full["psnr_db"], "kernels": full["kernels"]},
+        "psnr_spec_db": headline["psnr_spec_db"],
+        "psnr_note": "psnr_db: blocks decoded by the reference decoder (BPTCC::Decompress); "
+                     "psnr_spec_db: by the BC7 specification, which differs for mode-4 "
+                     "index-mode-1 blocks",
+        "full_search": dict(leg(full), kernels=full["kernels"]),
         "wall_s": headline["wall_s"],
     }
     if learned is not None:
-        line["learned"] = {"value": learned["value"], "ms_per_step": learned["ms_per_step"],
-                           "psnr_db": learned["psnr_db"], "topk": args.topk,
-                           "threshold": args.threshold,
-                           "speedup_vs_full": full["ms_per_step"] / learned["ms_per_step"],
-                           "delta_psnr_db": learned["psnr_db"] - full["psnr_db"],
-                           "weights": os.path.relpath(args.ranker, ROOT)}
+        line["no_network"] = dict(leg(no_net), search="BoxSelection's candidates for every "
+                                  "block, decoded-error choice, annealing within 1/50 of the "
+                                  "best (threshold above 1: no block confident)",
+                                  speedup_vs_full=full["ms_per_step"] / no_net["ms_per_step"])
+        line["learned"] = dict(leg(learned), topk=args.topk, threshold=args.threshold,
+                               speedup_vs_full=full["ms_per_step"] / learned["ms_per_step"],
+                               speedup_vs_no_network=no_net["ms_per_step"] /
+                               learned["ms_per_step"],
+                               delta_psnr_db=learned["psnr_db"] - full["psnr_db"],
+                               delta_psnr_spec_db=learned["psnr_spec_db"] - full["psnr_spec_db"],
+                               weights=os.path.relpath(args.ranker, ROOT))
     if valu is not None:
         line["valu"] = valu
     if not args.no_cpu_baseline:
-        cb, cpu_out = cpu_baseline(img, args.quality, args.cpu_tile, args.seed)
+        cb = cpu_baseline(img, args.quality, args.cpu_tile, args.seed)
         t = args.cpu_tile
-        tile_sse = int(headline["block_errors"][:t // 4, :t // 4].sum(dtype=np.uint64))
-        cb["gpu_psnr_same_tile_db"] = fastc_amd.psnr_from_sse((tile_sse, 0), t, t)
+        for key, errs in (("gpu_psnr_same_tile_db", headline["block_errors"]),
+                          ("gpu_psnr_spec_same_tile_db", headline["block_errors_spec"])):
+            tile_sse = int(errs[:t // 4, :t // 4].sum(dtype=np.uint64))
+            cb[key] = fastc_amd.psnr_from_sse((tile_sse, 0), t, t)
         line["cpu_baseline"] = cb
+    if not args.no_other_formats:
+        formats = {}
+        for name, fmt in (("dxt1", 0), ("dxt5", 1), ("etc1", 2)):
+            formats[name] = simple_leg(args, fmt, with_cpu=not args.no_cpu_baseline)
+            print(f"[bench] {name} ring done", file=sys.stderr, flush=True)
+        line["formats"] = formats
+    if not args.no_batch:
+        b = batch_leg(args, per_surface_steps=1, batch_steps=3)
+        if "cpu_baseline" in line and "formats" in line and "cpu_baseline" in \
+                line["formats"]["dxt5"]:
+            # CPU time of the batch's texels at the per-format CPU rates above
+            rates = {"bptc": line["cpu_baseline"]["value"],
+                     "dxt5": line["formats"]["dxt5"]["cpu_baseline"]["value"],
+                     "etc1": line["formats"]["etc1"]["cpu_baseline"]["value"]}
+            secs = sum(b["formats_texels"][k] / (rates[k] * 1e6) for k in rates)
+            b["cpu_baseline"] = {
+                "value": sum(b["formats_texels"].values()) / secs / 1e6, "unit": "Mtexels/s",
+                "cores": line["cpu_baseline"]["cores"], "kind": "port",
+                "sample": "estimated: the batch's texels of each format at the CPU rates "
+                          "measured above (BC7 q=50 oracle port, stb_dxt DXT5 and rg_etc1 "
+                          "ETC1 reference builds), all host cores"}
+        line["texture_batch"] = b
     print(json.dumps(line), flush=True)
 
 

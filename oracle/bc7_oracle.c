@@ -2208,17 +2208,26 @@ static void decode_block(const uint8_t blk[16], uint32_t px[16], int spec) {
 
 void ora_bc7_decode_block_spec(const uint8_t blk[16], uint32_t px[16]) { decode_block(blk, px, 1); }
 
-void ora_bc7_decompress(const uint8_t *in, uint8_t *out, uint32_t width, uint32_t height) {
+static void decompress_image(const uint8_t *in, uint8_t *out, uint32_t width, uint32_t height,
+                             int spec) {
   uint32_t *o = (uint32_t *)out;
   for (uint32_t j = 0; j < height; j += 4)
     for (uint32_t i = 0; i < width; i += 4) {
       uint32_t px[16];
-      decode_block(in, px, 0);
+      decode_block(in, px, spec);
       const uint32_t w = width - i < 4 ? width - i : 4;
       const uint32_t h = height - j < 4 ? height - j : 4;
       for (uint32_t jj = 0; jj < h; jj++) memcpy(o + (j + jj) * width + i, px + 4 * jj, w * 4);
       in += 16;
     }
+}
+
+void ora_bc7_decompress(const uint8_t *in, uint8_t *out, uint32_t width, uint32_t height) {
+  decompress_image(in, out, width, height, 0);
+}
+
+void ora_bc7_decompress_spec(const uint8_t *in, uint8_t *out, uint32_t width, uint32_t height) {
+  decompress_image(in, out, width, height, 1);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -154,6 +154,8 @@ void ora_bc7_compress_job_selected(const uint8_t *in, uint8_t *out, uint32_t wid
 /* BPTCC::Decompress (Decompressor.cpp:345-367), including the reference's
  * treatment of mode 4 with index mode 1 (Decompressor.cpp:263-295 interpolate
  * with the per-mode bit counts, not the swapped ones). */
+/* The whole image by the BC7 specification's decode (ora_bc7_decode_block_spec). */
+void ora_bc7_decompress_spec(const uint8_t *in, uint8_t *out, uint32_t width, uint32_t height);
 void ora_bc7_decompress(const uint8_t *in, uint8_t *out, uint32_t width,
                         uint32_t height);
 

@@ -82,6 +82,7 @@ def lib():
         L.ora_bc7_compress_job_selected.argtypes = [vp, vp] + [ctypes.c_uint32] * 6 + [
             ctypes.POINTER(Settings), ctypes.POINTER(ctypes.c_uint32), vp]
         L.ora_bc7_decompress.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32]
+        L.ora_bc7_decompress_spec.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32]
         L.ora_bc7_decode_block_spec.argtypes = [vp, vp]
         L.ora_psnr.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32]
         L.ora_psnr.restype = ctypes.c_double
@@ -235,6 +236,31 @@ def bc7_decompress(data, width, height):
     out = np.zeros((height, width), dtype=np.uint32)
     lib().ora_bc7_decompress(_p(data), _p(out), width, height)
     return out
+
+
+def bc7_decompress_spec(data, width, height):
+    """The image decoded by the BC7 specification (mode 4 index mode 1 with
+    swapped index widths, unlike the reference decoder)."""
+    data = np.ascontiguousarray(data, dtype=np.uint8)
+    out = np.zeros((height, width), dtype=np.uint32)
+    lib().ora_bc7_decompress_spec(_p(data), _p(out), width, height)
+    return out
+
+
+_ref = None
+
+
+def ref_lib():
+    """oracle/_ref/libfastc_ref.so: the reference's own stb_dxt and rg_etc1
+    built from the FasTC tree (None when it has not been built)."""
+    global _ref
+    if _ref is None and os.path.exists(REF_LIB_PATH):
+        L = ctypes.CDLL(REF_LIB_PATH)
+        vp = ctypes.c_void_p
+        L.ref_dxt_compress.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, vp]
+        L.ref_etc1_compress.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, vp]
+        _ref = L
+    return _ref
 
 
 def dxt_decompress(data, width, height, dxt5):
