@@ -42,7 +42,7 @@ def psnr(a, b):
 
 def headline_search(headline, full):
     return "full (every mode and selected partition, annealed)" if headline is full else \
-        "learned-pruned (ranker top-k, BoxSelection below the threshold)"
+        "learned-pruned (ranker top-k, the full search below the threshold)"
 TEXEL_IN_BYTES = 4
 BLOCK_OUT_BYTES = 16
 
@@ -603,12 +603,24 @@ def main():
                 "block_errors": be, "block_errors_spec": be_spec}
 
     full = run(None, "full search", args.threshold)
-    learned = no_net = None
+    learned = None
+    confident = None
     if ranker is not None:
-        # every block below the threshold: BoxSelection's candidates with the
-        # learned search's decoded-error choice and annealing shortlist
-        no_net = run(ranker, "no network", 2.0)
         learned = run(ranker, "learned", args.threshold)
+        # the share of searched blocks whose top softmax probability reaches
+        # the threshold (the rest run the full search)
+        blocks = img.reshape(H // 4, 4, W // 4, 4).transpose(0, 2, 1, 3).reshape(-1, 16)
+        searched = ~(blocks == blocks[:, :1]).all(1) & ((blocks >> 24) != 0).any(1)
+        sb = np.ascontiguousarray(blocks[searched])
+        d_b = torch.from_numpy(sb.view(np.int32)).to(dev)
+        z = torch.empty((sb.shape[0], 512), dtype=torch.float32, device=dev)
+        ranker.forward(d_b.data_ptr(), sb.shape[0], z.data_ptr(), sh)
+        valid = torch.zeros(512, dtype=torch.bool, device=dev)
+        for m, k in enumerate([16, 64, 64, 64, 8, 4, 1, 64]):
+            valid[64 * m:64 * m + k] = True
+        z = torch.where(valid, z, torch.full_like(z, -float("inf")))
+        confident = float((torch.softmax(z, 1).max(1).values >= args.threshold).float().mean())
+        del z, d_b
     headline = full
     if learned is not None and learned["psnr_db"] >= full["psnr_db"] - 0.1:
         headline = learned
@@ -670,14 +682,10 @@ def main():
         "wall_s": headline["wall_s"],
     }
     if learned is not None:
-        line["no_network"] = dict(leg(no_net), search="BoxSelection's candidates for every "
-                                  "block, decoded-error choice, annealing within 1/50 of the "
-                                  "best (threshold above 1: no block confident)",
-                                  speedup_vs_full=full["ms_per_step"] / no_net["ms_per_step"])
         line["learned"] = dict(leg(learned), topk=args.topk, threshold=args.threshold,
+                               confident_share=confident,
+                               fallback="the full search (below the threshold)",
                                speedup_vs_full=full["ms_per_step"] / learned["ms_per_step"],
-                               speedup_vs_no_network=no_net["ms_per_step"] /
-                               learned["ms_per_step"],
                                delta_psnr_db=learned["psnr_db"] - full["psnr_db"],
                                delta_psnr_spec_db=learned["psnr_spec_db"] - full["psnr_spec_db"],
                                weights=os.path.relpath(args.ranker, ROOT))

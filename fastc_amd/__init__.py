@@ -124,6 +124,9 @@ def lib():
         L.fh_bc7_exhaustive_labels.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
                                                ctypes.c_void_p, ctypes.c_void_p]
         L.fh_bc7_exhaustive_labels.restype = ctypes.c_int
+        L.fh_bc7_candidate_errors.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                              ctypes.c_void_p]
+        L.fh_bc7_candidate_errors.restype = ctypes.c_int
         vp = ctypes.c_void_p
         L.fh_ranker_create.argtypes = [ctypes.POINTER(vp)]
         L.fh_ranker_destroy.argtypes = [vp]
@@ -191,7 +194,8 @@ EXPORTED_SYMBOLS = ["fh_compress", "fh_compress_device", "fh_decompress",
                     "fh_psnr", "fh_mip_level_device", "fh_pad4_device", "fh_ktx_write",
                     "fh_compress_with_stats", "fh_compress_with_stats_device",
                     "fh_bc7_stats_format", "fh_compress_with_selection",
-                    "fh_compress_with_selection_device"]
+                    "fh_compress_with_selection_device", "fh_compress_device_noalloc",
+                    "fh_bc7_candidate_errors"]
 
 # fh_shape_selection (include/fasthip.h): BPTCC::ShapeSelection
 SHAPE_SELECTION_DTYPE = np.dtype([("num_shapes", "<u4"),
@@ -359,6 +363,13 @@ def kernel_time(kernel):
     n = ctypes.c_uint32(0)
     _check(lib().fh_kernel_time(kernel.encode(), ctypes.byref(ms), ctypes.byref(n)))
     return ms.value, n.value
+
+
+def candidate_errors_device(blocks_ptr, n, errors_ptr, stream=0):
+    """Every candidate's decoded error for the ranker's labels
+    (fh_bc7_candidate_errors): blocks n x 16 uint32, errors n x 512 float32."""
+    _check(lib().fh_bc7_candidate_errors(ctypes.c_void_p(blocks_ptr), n,
+                                         ctypes.c_void_p(errors_ptr), ctypes.c_void_p(stream)))
 
 
 def exhaustive_labels_device(blocks_ptr, n, labels_ptr, errors_ptr=0, stream=0):

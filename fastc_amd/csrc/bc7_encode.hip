@@ -1031,11 +1031,16 @@ __host__ __device__ __forceinline__ uint32_t id_slot(uint32_t id) {
 __constant__ int kTypeOfMode[8] = {0, 2, 1, 3, 6, 7, 5, 4};
 
 // Block class and candidate count, packed: cls 2 | count 5 << 2 (at most 31).
+// kSelFull marks a block of the learned search that runs the full search
+// (the ranker was not confident): every candidate annealed, the reference's
+// choice by internal error, the quality guard.
 __device__ __forceinline__ uint32_t sel_pack(int cls, int count) {
   return (uint32_t)cls | ((uint32_t)count << 2);
 }
+constexpr uint32_t kSelFull = 1u << 7;
 __device__ __forceinline__ int sel_cls(uint32_t s) { return s & 3; }
 __device__ __forceinline__ int sel_count(uint32_t s) { return (s >> 2) & 31; }
+__device__ __forceinline__ bool sel_full(uint32_t s) { return (s & kSelFull) != 0; }
 
 struct Cand {
   int mode, entry, rot, im, shape, nsub;
@@ -1124,6 +1129,7 @@ constexpr uint32_t kListChunk = 4096;
 // Flags of the states cb + 256 k + t (k = 0..15) of thread t: bit k set when
 // the state passes the margin.  Adjacent lanes read adjacent states.
 __device__ __forceinline__ uint32_t list_flags(const uint32_t *aid, uint32_t total,
+                                               const uint32_t *sel,
                                                const unsigned long long *cerr,
                                                const unsigned long long *q0best, uint32_t cb,
                                                int t) {
@@ -1133,8 +1139,10 @@ __device__ __forceinline__ uint32_t list_flags(const uint32_t *aid, uint32_t tot
     const uint32_t i = cb + 256u * (uint32_t)k + (uint32_t)t;
     if (i < total) {
       const uint32_t id = aid[i];
-      const unsigned long long best = q0best[id_block(id)];
-      if (cerr[id] <= best + best / kAnnealMarginDiv) flags |= 1u << k;
+      const uint32_t rel = id_block(id);
+      const unsigned long long best = q0best[rel];
+      // a full-search block anneals every candidate
+      if (sel_full(sel[rel]) || cerr[id] <= best + best / kAnnealMarginDiv) flags |= 1u << k;
     }
   }
   return flags;
@@ -1163,6 +1171,7 @@ __device__ __forceinline__ uint32_t wg_exclusive_scan(uint32_t v, uint32_t *excl
 }
 
 __global__ void __launch_bounds__(256) list_count_kernel(const uint32_t *aid, const uint32_t *counters,
+                                                         const uint32_t *sel,
                                                          const unsigned long long *cerr,
                                                          const unsigned long long *q0best,
                                                          uint32_t *chunk_count) {
@@ -1170,7 +1179,7 @@ __global__ void __launch_bounds__(256) list_count_kernel(const uint32_t *aid, co
   const uint32_t total = counters[16];
   const uint32_t cb = blockIdx.x * kListChunk;
   if (cb >= total) return;  // whole workgroup
-  const uint32_t flags = list_flags(aid, total, cerr, q0best, cb, threadIdx.x);
+  const uint32_t flags = list_flags(aid, total, sel, cerr, q0best, cb, threadIdx.x);
   uint32_t excl;
   const uint32_t all = wg_exclusive_scan((uint32_t)__popc(flags), &excl, s_wave);
   if (threadIdx.x == 0) chunk_count[blockIdx.x] = all;
@@ -1196,6 +1205,7 @@ __global__ void __launch_bounds__(256) list_scan_kernel(uint32_t *counters, uint
 }
 
 __global__ void __launch_bounds__(256) list_write_kernel(const uint32_t *aid, const uint32_t *counters,
+                                                         const uint32_t *sel,
                                                          const unsigned long long *cerr,
                                                          const unsigned long long *q0best,
                                                          const uint32_t *chunk_offset,
@@ -1203,7 +1213,7 @@ __global__ void __launch_bounds__(256) list_write_kernel(const uint32_t *aid, co
   const uint32_t total = counters[16];
   const uint32_t cb = blockIdx.x * kListChunk;
   if (cb >= total) return;  // whole workgroup
-  const uint32_t flags = list_flags(aid, total, cerr, q0best, cb, threadIdx.x);
+  const uint32_t flags = list_flags(aid, total, sel, cerr, q0best, cb, threadIdx.x);
   // round k places the passing states cb + 256 k .. cb + 256 k + 255 in
   // order: rank within the wave by ballot, then the waves before it
   __shared__ uint32_t s_round[2][4];
@@ -1874,9 +1884,12 @@ __global__ void __launch_bounds__(256) select_kernel(BlockRange r, Bc7Params prm
 #pragma unroll
     for (int i = 0; i < 16; i++) px[i] = s_px[blk][i];
     uint16_t cd[kSlots];
-    const uint32_t sel = SEL == kSelStats ? stats_select_block(px, lane, prm, cd, prm.stats + rel)
-                       : SEL == kSelGiven ? given_select_block(px, prm, prm.selections[rel], cd)
-                                          : box_select_block(px, lane, prm, cd);
+    uint32_t sel = SEL == kSelStats ? stats_select_block(px, lane, prm, cd, prm.stats + rel)
+                 : SEL == kSelGiven ? given_select_block(px, prm, prm.selections[rel], cd)
+                                    : box_select_block(px, lane, prm, cd);
+    // the learned search's low-confidence blocks (the fallback list) run the
+    // full search
+    if (list && sel_cls(sel) == 2) sel |= kSelFull;
     const int count = sel_count(sel);
     if (lane == 0) {
       s_sel[blk] = sel;
@@ -2771,10 +2784,15 @@ cand_error_kernel(BlockRange r, Work w, unsigned long long *cerr, int final_pass
 // Pass 0 stores the quality-0 winner and its error; the final pass writes
 // the block (solid and transparent blocks as pack_kernel does), keeping the
 // quality-0 winner when it decodes strictly better (q0err, if given).
+//
+// A full-search block (kSelFull) chooses as the full search does: the first
+// strict minimum of the internal errors (CompressClusters, Compressor.cpp:
+// 1752-1817), and with the guard on the final pass keeps the quality-0 block
+// when it decodes strictly better (guard_kernel's rule).
 __global__ void __launch_bounds__(256) pick_kernel(BlockRange r, Bc7Scratch scr, Work w,
                                                    uint4 *q0blk, unsigned long long *q0err,
                                                    const unsigned long long *cerr,
-                                                   int final_pass) {
+                                                   int final_pass, int q0_guard) {
   const uint32_t rel = blockIdx.x * blockDim.x + threadIdx.x;
   if (rel >= r.num_blocks) return;
   const uint32_t b = r.first_block + rel;
@@ -2797,11 +2815,23 @@ __global__ void __launch_bounds__(256) pick_kernel(BlockRange r, Bc7Scratch scr,
     unsigned long long best_err = ~0ull;
     int best_i = -1;
     const int count = sel_count(sel);
-    for (int i = 0; i < count; i++) {
-      const unsigned long long e = cerr[(size_t)rel * kMaxCands + i];
-      if (e < best_err) {
-        best_err = e;
-        best_i = i;
+    if (sel_full(sel)) {
+      uint32_t best_int = 0xFFFFFFFFu;
+      for (int i = 0; i < count; i++) {
+        const uint32_t e = w.rec[(size_t)rel * kMaxCands + i].err;
+        if (e < best_int) {
+          best_int = e;
+          best_i = i;
+        }
+      }
+      if (best_i >= 0) best_err = cerr[(size_t)rel * kMaxCands + best_i];
+    } else {
+      for (int i = 0; i < count; i++) {
+        const unsigned long long e = cerr[(size_t)rel * kMaxCands + i];
+        if (e < best_err) {
+          best_err = e;
+          best_i = i;
+        }
       }
     }
     if (best_i >= 0) {
@@ -2817,7 +2847,7 @@ __global__ void __launch_bounds__(256) pick_kernel(BlockRange r, Bc7Scratch scr,
       q0err[rel] = best_err;
       return;
     }
-    if (q0err && q0err[rel] < best_err) best = q0blk[rel];
+    if (q0err && (q0_guard || !sel_full(sel)) && q0err[rel] < best_err) best = q0blk[rel];
   }
   reinterpret_cast<uint4 *>(r.out)[b] = best;
 }
@@ -2850,9 +2880,12 @@ __global__ void __launch_bounds__(256) label_fill_kernel(unsigned long long *bes
   if (i < n) best[i] = ~0ull;
 }
 
+// ERRORS: every candidate's decoded error to errs (n x 512 floats, logit
+// order) instead of the per-block minimum.
+template <bool ERRORS>
 __global__ void __launch_bounds__(64) label_kernel(BlockRange r, int mode, uint32_t nslots,
                                                    const uint32_t *sctab,
-                                                   unsigned long long *best) {
+                                                   unsigned long long *best, float *errs) {
   __shared__ uint32_t s_px[64 * kPxStride];
   const uint64_t items = (uint64_t)r.num_blocks * nslots;
   for (uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; item < items;
@@ -2879,9 +2912,13 @@ __global__ void __launch_bounds__(64) label_kernel(BlockRange r, int mode, uint3
     uint32_t dec[16];
     bc7dec::decode_block(blk, dec);
     const unsigned long long err = premul_err(px, dec);  // < 2^38
-    const unsigned long long key =
-        (err << 16) | (unsigned long long)(kModePos[mode] * 64 + slot);
-    atomicMin(&best[rel], key);
+    if constexpr (ERRORS) {
+      errs[(size_t)rel * 512 + mode * 64 + slot] = (float)err;
+    } else {
+      const unsigned long long key =
+          (err << 16) | (unsigned long long)(kModePos[mode] * 64 + slot);
+      atomicMin(&best[rel], key);
+    }
   }
 }
 
@@ -3214,7 +3251,7 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
       hipLaunchKernelGGL(bc7::cand_error_kernel, dim3(err_grid), dim3(256), 0, stream, r, w,
                          cerr, 0);
       hipLaunchKernelGGL(bc7::pick_kernel, pack_grid, dim3(256), 0, stream, r, s, w, q0, q0err,
-                         (const unsigned long long *)cerr, 0);
+                         (const unsigned long long *)cerr, 0, (int)p.q0_guard);
       timing_end(t, kTagBc7Pack, stream);
       bc7::Work wa = w;
       wa.alist = alist;
@@ -3222,13 +3259,14 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
       // ordered compaction of the annealing list (grid covers the largest list)
       const uint32_t nlist = (uint32_t)((nc + bc7::kListChunk - 1) / bc7::kListChunk);
       hipLaunchKernelGGL(bc7::list_count_kernel, dim3(nlist), dim3(256), 0, stream,
-                         (const uint32_t *)w.aid, (const uint32_t *)w.counters, cerr, q0err,
-                         chunk_count);
+                         (const uint32_t *)w.aid, (const uint32_t *)w.counters,
+                         (const uint32_t *)w.sel, cerr, q0err, chunk_count);
       hipLaunchKernelGGL(bc7::list_scan_kernel, dim3(1), dim3(256), 0, stream, w.counters,
                          chunk_count);
       hipLaunchKernelGGL(bc7::list_write_kernel, dim3(nlist), dim3(256), 0, stream,
-                         (const uint32_t *)w.aid, (const uint32_t *)w.counters, cerr, q0err,
-                         (const uint32_t *)chunk_count, alist);
+                         (const uint32_t *)w.aid, (const uint32_t *)w.counters,
+                         (const uint32_t *)w.sel, cerr, q0err, (const uint32_t *)chunk_count,
+                         alist);
       launch_anneal(persistent, r, p, wa, stream);
       timing_end(t, kTagBc7Anneal, stream);
     }
@@ -3240,7 +3278,7 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
                        p.quality > 0 ? 1 : 0);
     hipLaunchKernelGGL(bc7::pick_kernel, pack_grid, dim3(256), 0, stream, r, s, w, q0,
                        p.quality > 0 ? q0err : (unsigned long long *)nullptr,
-                       (const unsigned long long *)cerr, 1);
+                       (const unsigned long long *)cerr, 1, (int)p.q0_guard);
     timing_end(t, kTagBc7Pack, stream);
     return hipGetLastError();
   }
@@ -3289,12 +3327,40 @@ hipError_t bc7_exhaustive_labels(const BlockRange &r, unsigned long long *best,
     uint64_t groups = (items + 63) / 64;
     const uint64_t cap = (uint64_t)g_num_cus[dev] * 64;
     if (groups > cap) groups = cap;
-    hipLaunchKernelGGL(bc7::label_kernel, dim3((uint32_t)groups), dim3(64), 0, stream, r, mode,
-                       kSlots[mode], sct, best);
+    hipLaunchKernelGGL(bc7::label_kernel<false>, dim3((uint32_t)groups), dim3(64), 0, stream, r,
+                       mode, kSlots[mode], sct, best, (float *)nullptr);
   }
   timing_end(t, kTagBc7Labels, stream);
   hipLaunchKernelGGL(bc7::label_finish_kernel, dim3((n + 255) / 256), dim3(256), 0, stream,
                      (const unsigned long long *)best, n, labels, errors);
+  return hipGetLastError();
+}
+
+hipError_t bc7_candidate_errors(const BlockRange &r, float *errs, hipStream_t stream) {
+  const uint32_t n = r.num_blocks;
+  if (n == 0) return hipSuccess;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  e = bc7_upload_tables(dev);
+  if (e != hipSuccess) return e;
+  const uint32_t *sct = bc7_single_color_table(dev);
+  if (!sct) return hipErrorOutOfMemory;
+  // logits that are no candidate stay +inf
+  const uint64_t words = (uint64_t)n * 512;
+  hipLaunchKernelGGL(bc7::fill_kernel, dim3((uint32_t)((words + 255) / 256)), dim3(256), 0, stream,
+                     reinterpret_cast<uint32_t *>(errs), (uint32_t)words, 0x7F800000u);
+  static const uint32_t kSlots[8] = {16, 64, 64, 64, 8, 4, 1, 64};
+  const int t = timing_begin(stream);
+  for (int mode = 0; mode < 8; mode++) {
+    const uint64_t items = (uint64_t)n * kSlots[mode];
+    uint64_t groups = (items + 63) / 64;
+    const uint64_t cap = (uint64_t)g_num_cus[dev] * 64;
+    if (groups > cap) groups = cap;
+    hipLaunchKernelGGL(bc7::label_kernel<true>, dim3((uint32_t)groups), dim3(64), 0, stream, r,
+                       mode, kSlots[mode], sct, (unsigned long long *)nullptr, errs);
+  }
+  timing_end(t, kTagBc7Labels, stream);
   return hipGetLastError();
 }
 

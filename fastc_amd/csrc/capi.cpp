@@ -818,6 +818,11 @@ fh_status fh_bc7_exhaustive_labels(const uint32_t *blocks, uint32_t n, int32_t *
   std::lock_guard<std::mutex> g(st->mu);
   hipError_t e = ensure_init(st);
   if (e != hipSuccess) return fail_hip(e, "device initialisation");
+  const hipStream_t stream = reinterpret_cast<hipStream_t>(hip_stream);
+  e = order_after_last_use(st, stream);
+  if (e == hipSuccess && st->scratch_size < (size_t)n * 8 + 256 && st->last_use_recorded)
+    e = hipEventSynchronize(st->last_use);
+  if (e != hipSuccess) return fail_hip(e, "previous encode");
   e = grow(&st->scratch, &st->scratch_size, (size_t)n * 8 + 256);
   if (e != hipSuccess) return fail_hip(e, "scratch allocation");
   // n blocks of 16 pixels, seen as a 4 x 4n image one block wide
@@ -829,8 +834,35 @@ fh_status fh_bc7_exhaustive_labels(const uint32_t *blocks, uint32_t n, int32_t *
   r.first_block = 0;
   r.num_blocks = n;
   e = bc7_exhaustive_labels(r, reinterpret_cast<unsigned long long *>(st->scratch), labels,
-                            errors, reinterpret_cast<hipStream_t>(hip_stream));
+                            errors, stream);
   if (e != hipSuccess) return fail_hip(e, "label search launch");
+  e = record_use(st, stream);
+  if (e != hipSuccess) return fail_hip(e, "stream ordering");
+  return FH_OK;
+}
+
+fh_status fh_bc7_candidate_errors(const uint32_t *blocks, uint32_t n, float *errors,
+                                  void *hip_stream) {
+  if (!blocks || !errors) return fail(FH_ERR_INVALID_ARGUMENT, "null buffer");
+  if (n == 0) return FH_OK;
+  if ((uint64_t)n * 512 >= (1ull << 32))
+    return fail(FH_ERR_INVALID_ARGUMENT, "at most 2^23 - 1 blocks per call");
+  fh_status s;
+  const int dev = current_device(&s);
+  if (dev < 0) return s;
+  DeviceState *st = device_state(dev);
+  std::lock_guard<std::mutex> g(st->mu);
+  hipError_t e = ensure_init(st);
+  if (e != hipSuccess) return fail_hip(e, "device initialisation");
+  BlockRange r;
+  r.img = blocks;
+  r.out = nullptr;
+  r.width = 4;
+  r.blocks_wide = 1;
+  r.first_block = 0;
+  r.num_blocks = n;
+  e = bc7_candidate_errors(r, errors, reinterpret_cast<hipStream_t>(hip_stream));
+  if (e != hipSuccess) return fail_hip(e, "candidate error launch");
   return FH_OK;
 }
 

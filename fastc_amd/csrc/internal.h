@@ -98,6 +98,9 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
 // term times 255^2).  best: n words of scratch.
 hipError_t bc7_exhaustive_labels(const BlockRange &r, unsigned long long *best, int32_t *labels,
                                  unsigned long long *errors, hipStream_t stream);
+// Every candidate's decoded error (errs: n x 512 floats in logit order,
+// +inf where a logit is no candidate).
+hipError_t bc7_candidate_errors(const BlockRange &r, float *errs, hipStream_t stream);
 
 hipError_t dxt_encode(const BlockRange &r, int dxt5, hipStream_t stream);
 hipError_t etc1_encode(const BlockRange &r, hipStream_t stream);

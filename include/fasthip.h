@@ -315,6 +315,13 @@ int fh_device_available(void);
  * Asynchronous on hip_stream. */
 fh_status fh_bc7_exhaustive_labels(const uint32_t *blocks, uint32_t n, int32_t *labels,
                                    unsigned long long *errors, void *hip_stream);
+/* The same search's per-candidate errors (no reference counterpart):
+ * errors = n x 512 floats on the device, errors[i * 512 + mode * 64 + slot]
+ * the decoded premultiplied error of that candidate (float32), +inf for the
+ * 227 logits that are no candidate; n < 2^23.  For label definitions other
+ * than the arg-min (sets of near-best candidates).  Asynchronous. */
+fh_status fh_bc7_candidate_errors(const uint32_t *blocks, uint32_t n, float *errors,
+                                  void *hip_stream);
 
 /* ---- BC7 candidate ranker (no reference counterpart) ----------------------
  * A 64 -> 256 -> 256 -> 512 MLP (ReLU, bf16 weights on the matrix cores,

@@ -1,8 +1,10 @@
 """Learned candidate pruning (FH_FLAG_LEARNED) against the oracle, bit for
-bit: the test reads the ranker's logits from the GPU, reproduces the
-pruning decisions (top softmax probability against the threshold, top-k
-candidates, BoxSelection's candidates otherwise) and has the oracle encode
-each block from those candidates with the same decoded-error selection."""
+bit: the test reads the ranker's logits from the GPU and reproduces the
+pruning decisions.  A confident block (top softmax probability at least the
+threshold) is encoded by the oracle from its top-k candidates with the
+learned search's decoded-error selection; any other block must be exactly
+the full search's block (BoxSelection, every candidate annealed, the
+reference's choice, the quality guard)."""
 import ctypes
 
 import numpy as np
@@ -89,12 +91,26 @@ def test_learned_search_matches_oracle(fh, oracle, quality, threshold, topk, mod
         if pmax[b] >= threshold:
             order = sorted(np.nonzero(valid)[0], key=lambda j: (-z[b, j], j))[:topk]
             cands = [(int(j) >> 6, int(j) & 63, 0) for j in order]
+            want = oracle.bc7_encode_learned(px, cands, quality=quality, seed=21, block_id=b)
         else:
-            cands = _box_candidates(oracle, px, modes, quality)
-        want = oracle.bc7_encode_learned(px, cands, quality=quality, seed=21, block_id=b)
+            cands = "full search"
+            want, _ = oracle.bc7_compress_block(px, quality=quality, seed=21, block_id=b,
+                                                block_modes=modes, q0_guard=True)
         assert np.array_equal(got[b], want), (b, cands)
         checked += 1
     assert checked > 150
+
+
+@pytest.mark.parametrize("quality", [0, 9])
+def test_learned_without_confident_blocks_is_the_full_search(fh, quality):
+    """With a threshold no softmax probability reaches, every block falls back
+    to the full search: the output is the full search's, byte for byte."""
+    img = synth.generate(128, 128, 5)
+    rk = fh.Ranker(seed=4)
+    full = fh.compress(img, quality=quality, seed=8, wm_counter=ctypes.c_uint32(0xFFFFFFFF))
+    got = fh.compress(img, quality=quality, seed=8, wm_counter=ctypes.c_uint32(0xFFFFFFFF),
+                      ranker=rk, threshold=2.0)
+    assert np.array_equal(got, full)
 
 
 def test_learned_search_never_worse_than_its_quality0(fh, oracle):

@@ -6,9 +6,9 @@ The image is 2^20 blocks, so the encode ranks them in four chunks of 2^18
 (the fused inference kernel's chunk).  Blocks sampled from every chunk, and
 the 64 blocks on each side of every chunk boundary, are encoded by the
 oracle from the same pruning decision (top softmax probability against the
-threshold on the ranker's logits, top-4 candidates or BoxSelection's) and
-must match bit for bit; no block of the image may decode worse than its own
-quality-0 encoding."""
+threshold on the ranker's logits: its top-4 candidates, or the full search
+for a block below the threshold) and must match bit for bit; no block of the
+image may decode worse than its own quality-0 encoding."""
 import ctypes
 import os
 from concurrent.futures import ThreadPoolExecutor
@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 from fastc_amd import synth
-from tests.test_bc7_learned_gpu import _box_candidates, _logits
+from tests.test_bc7_learned_gpu import _logits
 
 pytestmark = pytest.mark.gpu
 
@@ -34,16 +34,20 @@ def _premul_errors(img_blocks, dec_blocks):
     return (d * d).sum(axis=(1, 2))
 
 
-def test_learned_4096_matches_oracle_across_chunks(fh, oracle):
+@pytest.mark.parametrize("threshold", [0.6, 0.95])
+def test_learned_4096_matches_oracle_across_chunks(fh, oracle, threshold):
+    """At the benchmark's threshold (0.6) the trained ranker is confident on
+    nearly every block; at 0.95 most blocks run the full search, so that path
+    is checked at this size as well."""
     img = synth.generate(N, N, 3)
     blocks = synth.blocks(img)
     rk = fh.Ranker()
     rk.set_params(np.load(WEIGHTS))
-    print("learned 4096x4096 q=50 top-4 threshold 0.6", flush=True)
+    print(f"learned 4096x4096 q=50 top-4 threshold {threshold}", flush=True)
     got = fh.compress(img, quality=50, seed=3, wm_counter=ctypes.c_uint32(0xFFFFFFFF),
-                      ranker=rk, topk=4, threshold=0.6).reshape(-1, 16)
+                      ranker=rk, topk=4, threshold=threshold).reshape(-1, 16)
     q0 = fh.compress(img, quality=0, seed=3, wm_counter=ctypes.c_uint32(0xFFFFFFFF),
-                     ranker=rk, topk=4, threshold=0.6)
+                     ranker=rk, topk=4, threshold=threshold)
     # no block worse than its own quality-0 encoding (reference decoder)
     e50 = _premul_errors(blocks, synth.blocks(fh.decompress(got.reshape(-1), N, N)))
     e0 = _premul_errors(blocks, synth.blocks(fh.decompress(q0, N, N)))
@@ -67,16 +71,15 @@ def test_learned_4096_matches_oracle_across_chunks(fh, oracle):
             ref, _ = oracle.bc7_compress_block(px, quality=0, block_id=b,
                                                wm_index=int(nsolid_before[b] % 9))
             return b, ref, "plain"
-        if abs(pmax[i] - 0.6) < 1e-4:
+        if abs(pmax[i] - threshold) < 1e-4:
             return b, None, "tie"  # float32 on the GPU could decide either way
-        if pmax[i] >= 0.6:
+        if pmax[i] >= threshold:
             order = sorted(np.nonzero(valid)[0], key=lambda j: (-z[i, j], j))[:4]
             cands = [(int(j) >> 6, int(j) & 63, 0) for j in order]
-            kind = "confident"
-        else:
-            cands = _box_candidates(oracle, px, 0xFF, 50)
-            kind = "fallback"
-        return b, oracle.bc7_encode_learned(px, cands, quality=50, seed=3, block_id=b), kind
+            return b, oracle.bc7_encode_learned(px, cands, quality=50, seed=3, block_id=b), \
+                "confident"
+        ref, _ = oracle.bc7_compress_block(px, quality=50, seed=3, block_id=b, q0_guard=True)
+        return b, ref, "fallback"
 
     with ThreadPoolExecutor(16) as ex:
         results = list(ex.map(want, range(len(picks))))
@@ -86,4 +89,7 @@ def test_learned_4096_matches_oracle_across_chunks(fh, oracle):
         if ref is not None:
             assert np.array_equal(got[b], ref), (b, kind)
     print(f"checked {len(picks)} blocks: {kinds}", flush=True)
-    assert kinds.get("confident", 0) >= 20 and kinds.get("fallback", 0) >= 200, kinds
+    if threshold > 0.9:
+        assert kinds.get("fallback", 0) >= 200 and kinds.get("confident", 0) >= 20, kinds
+    else:
+        assert kinds.get("confident", 0) >= 600, kinds

@@ -1,13 +1,17 @@
 """What the trained ranker gains, on a held-out 2048x2048 image (seed 7,
-q=50), against the same search without it:
+q=50), against the search without it.
 
-* at the 0.6 confidence threshold only about one block in eight is
-  confident, so the search takes about as long as with every block on
-  BoxSelection's candidates (it may not be slower);
-* without the threshold the search is much faster still, and the trained
-  network's top-4 keeps the PSNR an untrained network's top-4 loses;
-* every learned variant stays within 0.1 dB of the reference's full search
-  or above it (the decoded-error choice among candidates raises it)."""
+Without a confident ranking every block runs the full search, so the search
+without the network is the full search itself (threshold 2: byte for byte
+the full search's output).  With the committed network at the 0.6 threshold
+nearly every block is confident and searches only its top-4 candidates:
+
+* the learned search is several times faster than the full search, and its
+  PSNR is no more than 0.1 dB below it, under the reference decoder and under
+  the BC7 specification's (in fact above: among its candidates it chooses by
+  decoded error);
+* an untrained network's top-4 loses about 14 dB where the trained one's
+  keeps the quality, so the speed comes from what the network learned."""
 import os
 
 import numpy as np
@@ -55,18 +59,36 @@ def test_trained_ranker_gains(fh):
             e1.record(stream)
             torch.cuda.synchronize()
             times.append(e0.elapsed_time(e1))
-        dec = fh.decompress(d_out.cpu().numpy(), n, n)
-        return float(np.median(times)), _psnr(img, dec)
+        enc = d_out.cpu().numpy()
+        ref = _psnr(img, fh.decompress(enc, n, n))
+        spec = _psnr(img, fh.decompress(enc, n, n, fmt=3 | fh.DECODE_SPEC))
+        return float(np.median(times)), ref, spec, enc
 
     full = run(None, 0.6)
+    no_net = run(trained, 2.0)  # no block confident: the full search everywhere
     learned = run(trained, 0.6)
-    no_pruning = run(trained, 2.0)  # every block falls back to BoxSelection
-    learned_all = run(trained, 0.0)
     untrained_all = run(untrained, 0.0)
-    print(f"(ms, dB): full {full}, learned {learned}, no pruning {no_pruning}, "
-          f"learned thr 0 {learned_all}, untrained thr 0 {untrained_all}", flush=True)
-    assert learned[1] >= full[1] - 0.1 and learned_all[1] >= full[1] - 0.1
-    assert learned[0] < 1.05 * no_pruning[0]
-    assert learned_all[0] < 0.8 * no_pruning[0]
-    assert learned_all[1] >= no_pruning[1] - 0.05
-    assert learned_all[1] > untrained_all[1] + 0.5
+    print(f"(ms, dB ref, dB spec): full {full[:3]}, no network {no_net[:3]}, learned "
+          f"{learned[:3]}, untrained top-4 {untrained_all[:3]}", flush=True)
+    # byte for byte the full search's blocks (solid blocks aside: their
+    # watermark follows the process-wide counter, which each call advances)
+    blocks = synth.blocks(img)
+    solid = (blocks == blocks[:, :1]).all(1)
+    assert np.array_equal(no_net[3].reshape(-1, 16)[~solid], full[3].reshape(-1, 16)[~solid])
+    assert learned[1] >= full[1] - 0.1 and learned[2] >= full[2] - 0.1
+    assert full[0] / learned[0] >= 2.5
+    assert learned[1] > untrained_all[1] + 5.0
+
+    # the share of searched blocks the network is confident on
+    searched = ~(blocks == blocks[:, :1]).all(1) & ((blocks >> 24) != 0).any(1)
+    sb = np.ascontiguousarray(blocks[searched])
+    d_b = torch.from_numpy(sb.view(np.int32)).cuda()
+    z = torch.empty((sb.shape[0], 512), dtype=torch.float32, device="cuda")
+    trained.forward(d_b.data_ptr(), sb.shape[0], z.data_ptr(), stream.cuda_stream)
+    valid = torch.zeros(512, dtype=torch.bool, device="cuda")
+    for m, k in enumerate([16, 64, 64, 64, 8, 4, 1, 64]):
+        valid[64 * m:64 * m + k] = True
+    z = torch.where(valid, z, torch.full_like(z, -float("inf")))
+    conf = float((torch.softmax(z, 1).max(1).values >= 0.6).float().mean())
+    print(f"confident share {conf:.4f}", flush=True)
+    assert conf >= 0.95
