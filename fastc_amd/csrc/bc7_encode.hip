@@ -2324,9 +2324,44 @@ __device__ __forceinline__ void score_pixels(const Lane &L, const SubConst &sc0,
   }
 }
 
+// The same pass split over a group of G lanes (G = 4, 8 or 16, aligned in
+// the wave) that hold the same candidate: member m scores pixels m, m + G,
+// ...; the sums and index bits are combined over the group.  The per-subset
+// errors are integers, so the split does not change them.
+template <int NSUB>
+__device__ __forceinline__ void score_pixels_group(const Lane &L, const SubConst &sc0,
+                                                   const SubConst &sc1, const SubConst &sc2,
+                                                   int nb, float kf, WeightFn wf, uint32_t nerr[3],
+                                                   uint64_t &nidx, int G, int m) {
+  for (int i = m; i < 16; i += G) {
+    const uint32_t s = NSUB == 1 ? 0u : (L.sub2 >> (2 * i)) & 3u;
+    const SubConst c = NSUB == 1 ? sc0
+                                 : (NSUB == 2 ? pick_sub(s != 0, sc1, sc0)
+                                              : pick_sub(s == 0, sc0, pick_sub(s == 1, sc1, sc2)));
+    int best;
+    const uint32_t me = pixel_err(L.px[i], L.pt[i], c, nb, kf, wf, &best);
+    nerr[0] += s == 0 ? me : 0u;
+    if (NSUB > 1) nerr[1] += s == 1 ? me : 0u;
+    if (NSUB == 3) nerr[2] += s == 2 ? me : 0u;
+    nidx |= (uint64_t)best << (4 * i);
+  }
+  uint32_t lo = (uint32_t)nidx, hi = (uint32_t)(nidx >> 32);
+  for (int off = 1; off < G; off <<= 1) {
+    nerr[0] += __shfl_xor(nerr[0], off);
+    if (NSUB > 1) nerr[1] += __shfl_xor(nerr[1], off);
+    if (NSUB == 3) nerr[2] += __shfl_xor(nerr[2], off);
+    lo |= __shfl_xor(lo, off);
+    hi |= __shfl_xor(hi, off);
+  }
+  nidx = (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
 // One annealing step of every active subset.  Returns true when the
-// candidate is finished.
-__device__ bool lane_step(Lane &L, uint32_t q) {
+// candidate is finished.  With GROUPED, the lanes of an aligned group of G
+// hold the same candidate and share the pixel pass (member m); every other
+// part of the step runs identically in each of them.
+template <bool GROUPED = false>
+__device__ bool lane_step(Lane &L, uint32_t q, int G = 1, int m = 0) {
   const Fit &f = L.f;
   const int nb = 1 << f.nbits_idx;
   uint32_t np1[3], np2[3], e1[3], e2[3];
@@ -2355,9 +2390,17 @@ __device__ bool lane_step(Lane &L, uint32_t q) {
   // the widest subset count among the wave's running lanes picks the path
   // (a wave-uniform branch): the list is ordered by mode, so most waves hold
   // candidates of one subset count
-  if (__ballot(L.nsub == 3)) score_pixels<3>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx);
-  else if (__ballot(L.nsub == 2)) score_pixels<2>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx);
-  else score_pixels<1>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx);
+  if (GROUPED) {
+    if (__ballot(L.nsub == 3)) score_pixels_group<3>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx, G, m);
+    else if (__ballot(L.nsub == 2)) score_pixels_group<2>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx, G, m);
+    else score_pixels_group<1>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx, G, m);
+  } else if (__ballot(L.nsub == 3)) {
+    score_pixels<3>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx);
+  } else if (__ballot(L.nsub == 2)) {
+    score_pixels<2>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx);
+  } else {
+    score_pixels<1>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx);
+  }
 #pragma unroll
   for (int s = 0; s < 3; s++) {
     if (!((L.active >> s) & 1)) continue;
@@ -2443,6 +2486,10 @@ __device__ void lane_finish(Lane &L, Work &w) {
   rc->err = total;
 }
 
+// A wave with at most this many candidates left after the list is used up
+// finishes them in lane groups.
+constexpr int kTailLanes = 16;
+
 // UNROLL annealing steps run between refills of finished lanes.
 // WAVES: the minimum waves per SIMD the register allocation must allow.  3
 // (168 VGPRs, a small spill) suits the long lists of the full search; the
@@ -2463,6 +2510,8 @@ anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
   uint32_t wbase = 0, wend = 0;  // wave-local window of the list
   bool exhausted = false;
   for (;;) {
+    // the list is used up and few candidates remain: the tail below
+    if (exhausted && __popcll(__ballot(L.has)) <= kTailLanes) break;
     // refill idle lanes from the window, fetching a new window when needed
     for (;;) {
       const uint64_t need = __ballot(!L.has);
@@ -2495,6 +2544,41 @@ anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
         if (lane_step(L, prm.quality)) {
           lane_finish(L, w);
           L.has = false;
+        }
+      }
+    }
+  }
+  // Tail: the wave's last candidates (at most kTailLanes) each get a group
+  // of G lanes that share the pixel pass of every step, so the longest
+  // anneals, which set the launch's length, run up to G times fewer
+  // instructions per step.  Their texels stay in the LDS rows of the lanes
+  // that loaded them.
+  const uint64_t act = __ballot(L.has);
+  const int na = __popcll(act);
+  if (na == 0) return;
+  __shared__ Lane s_tail[kTailLanes];
+  __shared__ uint32_t s_row[kTailLanes];
+  if (L.has) {
+    const int k = __popcll(act & lt);
+    s_tail[k] = L;
+    s_row[k] = (uint32_t)lane;
+  }
+  __syncthreads();
+  const int G = na <= 4 ? 16 : (na <= 8 ? 8 : 4);
+  const int g = lane / G, m = lane % G;
+  Lane T;
+  T.has = false;
+  if (g < na) {
+    T = s_tail[g];
+    T.px = s_px + s_row[g] * 33;
+    T.pt = T.px + 16;
+  }
+  while (__ballot(T.has)) {
+    for (int u = 0; u < UNROLL; u++) {
+      if (T.has) {
+        if (lane_step<true>(T, prm.quality, G, m)) {
+          if (m == 0) lane_finish(T, w);
+          T.has = false;
         }
       }
     }
