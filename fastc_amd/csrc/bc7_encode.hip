@@ -34,6 +34,7 @@
 
 #include "../../include/fasthip.h"
 #include "bc7_block_decode.h"
+#include "ranker_tile.h"
 #include "internal.h"
 
 namespace fh {
@@ -1929,13 +1930,9 @@ __global__ void __launch_bounds__(256) stats_mode_kernel(BlockRange r, fh_bc7_bl
 }
 
 // ---------------------------------------------------------------------------
-// Kernel 1b (learned candidate pruning): one wave per block reads the
-// ranker's 512 logits (logit = mode * 64 + slot) for the chunk of blocks
-// starting at rel0.  Solid and transparent blocks are classified as above.
-// If the top softmax probability over the real candidates of the allowed
-// modes reaches the threshold, the top-k candidates (ties to the lower
-// logit) become the block's slots; otherwise the block goes to the fallback
-// list, which select_kernel then searches with BoxSelection.
+// Kernel 1b (learned candidate pruning).  The ranker gives each block 512
+// logits, logit = mode * 64 + slot; these name the real candidates among
+// them and the candidate each one stands for.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool logit_real(int j) {
   const int mode = j >> 6, slot = j & 63;
@@ -1956,100 +1953,236 @@ __device__ __forceinline__ uint32_t cdesc_of_logit(int j) {
   return cdesc_pack(mode, shape, rot, im, 0);
 }
 
-__global__ void __launch_bounds__(1024) learned_select_kernel(BlockRange r, Bc7Params prm,
-                                                              Work w, uint32_t rel0,
-                                                              uint32_t nchunk,
-                                                              const float *logits,
-                                                              uint32_t *fallback,
-                                                              uint32_t *fallback_count) {
+
+// ---------------------------------------------------------------------------
+// The learned search's ranking and selection in one kernel: a workgroup of
+// four waves takes 64 blocks through the ranker (ranker_tile.h: logits
+// bit-identical to the forward pass) and makes each block's decision from
+// the fp32 logits staged in LDS, so they never reach HBM.  The output layer
+// runs in four phases of 128 logits; the softmax is accumulated online over
+// the phases (its total differs from a single sum by rounding only) and the
+// top-k list is exact (largest logit first, the lower logit on ties).  Solid and transparent blocks are
+// marked as the reference writes them, a confident block (top probability
+// at least the threshold) keeps its top-k candidates, and the rest go on the
+// fallback list for the full search.
+// ---------------------------------------------------------------------------
+// Strict order of the top-k list: the larger logit first, the lower logit
+// index on ties.
+// (Bitwise operators keep the comparisons free of branches.)
+__device__ __forceinline__ bool top_before(float va, int ja, float vb, int jb) {
+  return (va > vb) | ((va == vb) & (ja < jb));
+}
+
+// Logits per mode that name a real candidate (logit_real).
+__device__ __forceinline__ int real_logits(int mode) {
+  return mode == 0 ? 16 : (mode == 4 ? 8 : (mode == 5 ? 4 : (mode == 6 ? 1 : 64)));
+}
+
+constexpr int kStageStride = 144;  // fp32 logits per staged row: 128 + 16, so that the
+                                   // row-quad reads and the wave's writes avoid bank conflicts
+static_assert(mlp::kRankRows * kStageStride * 4 <= mlp::kLdsH2, "staging over X and H1");
+
+// Two workgroups per CU (their LDS fits twice): the bound holds the kernel to
+// the registers that allows, so one workgroup's MFMA work overlaps the
+// other's selection.
+__global__ void __launch_bounds__(256, 2) rank_select_kernel(BlockRange r, Bc7Params prm, Work w,
+                                                          const uint16_t *w1, const uint16_t *w2,
+                                                          const uint16_t *w3,
+                                                          const float *params,
+                                                          uint32_t *fallback,
+                                                          uint32_t *fallback_count) {
+  __shared__ __align__(16) uint8_t lds[mlp::kLdsBytes];
   __shared__ uint32_t s_cnt[kNumTypes], s_base[kNumTypes];
   __shared__ uint32_t s_fb, s_fbbase;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t k = blockIdx.x * 16 + wave;  // one block per wave
-  if (threadIdx.x == 0) s_fb = 0;
+  __shared__ uint16_t s_cd[mlp::kRankRows][8];
+  __shared__ uint32_t s_count[mlp::kRankRows];
+  __shared__ uint32_t s_fbslot[mlp::kRankRows];
+  uint16_t *xs = reinterpret_cast<uint16_t *>(lds + mlp::kLdsX);
+  uint16_t *h1 = reinterpret_cast<uint16_t *>(lds + mlp::kLdsH1);
+  uint16_t *h2 = reinterpret_cast<uint16_t *>(lds + mlp::kLdsH2);
+  float *stage = reinterpret_cast<float *>(lds);  // over X and H1 once layer 2 is done
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t n = r.num_blocks;
+  const uint32_t tile0 = blockIdx.x * mlp::kRankRows;
+  if (tid == 0) s_fb = 0;
+  mlp::rank_features(r, n, tile0, tid, xs);
   __syncthreads();
-  bool fell_back = false;
-  uint32_t fb_slot = 0;
-  const bool real = k < nchunk;
-  const uint32_t rel = rel0 + (real ? k : 0);
-  bool searched = false;
-  int count = 0;
-  uint32_t my_desc = 0;
-  if (real) {
-    const uint32_t b = r.first_block + rel;
-    const uint32_t bx = b % r.blocks_wide, by = b / r.blocks_wide;
-    const uint32_t mine =
-        r.img[(size_t)(by * 4 + ((lane & 15) >> 2)) * r.width + bx * 4 + (lane & 3)];
-    const uint32_t p0 = __shfl(mine, 0);
-    const bool solid = !__ballot((lane < 16) && mine != p0);
-    const bool transparent = !__ballot((lane < 16) && (mine >> 24) != 0);
-    if (solid || transparent) {
-      if (lane == 0) w.sel[rel] = sel_pack(solid ? 0 : 1, 0);
-    } else {
-      searched = true;
-    }
+  mlp::rank_hidden<64, mlp::kXs, 2>(xs, w1, params + mlp::kRankOffB1, wave * 64, lane, h1);
+  __syncthreads();
+  mlp::rank_hidden<256, mlp::kHs, 2>(h1, w2, params + mlp::kRankOffB2, wave * 64, lane, h2);
+  __syncthreads();
+
+  // Block srow's decision is made by its four threads (seg 0..3), which hold
+  // the same running softmax (m, sum) and top-k list (lv, lj) throughout.
+  const int srow = tid >> 2, seg = tid & 3;
+  const int kk = prm.topk < 1 ? 1 : (prm.topk > 8 ? 8 : (int)prm.topk);
+  const float *b3 = params + mlp::kRankOffB3;
+  float m = -INFINITY, sum = 0.0f;
+  float lv[8];
+  int lj[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    lv[i] = -INFINITY;
+    lj[i] = 1 << 30;
   }
-  if (searched) {
-    float z[8];
-    bool ok[8];
-    float mx = -INFINITY;
+  // the output layer in four phases of 128 logits (modes 2p and 2p + 1),
+  // 32 per wave; each phase is staged in LDS and folded into the row state
+#pragma unroll 1
+  for (int p = 0; p < 4; p++) {
+    mlp::floatx16 acc[2][1] = {};
+    const int c0 = 128 * p + 32 * wave;
+    mlp::rank_mma<256, mlp::kHs, 1>(h2, w3, c0, lane, acc);
+    const float bias = b3[c0 + (lane & 31)];
+    if (p > 0) __syncthreads();  // the previous phase's stage is read
+#pragma unroll
+    for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+      for (int v = 0; v < 16; v++)
+        stage[mlp::acc_row(rb, v, lane) * kStageStride + 32 * wave + (lane & 31)] =
+            acc[rb][0][v] + bias;
+    __syncthreads();
+    // this thread's logits: 128 p + 16 i + 4 seg + e (i < 8, e < 4)
+    float z[32];
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-      const int j = lane + 64 * i;
-      ok[i] = logit_real(j) && ((prm.block_modes >> (j >> 6)) & 1);
-      z[i] = logits[(size_t)k * 512 + j];
-      if (ok[i]) mx = fmaxf(mx, z[i]);
+      const float4 q =
+          *reinterpret_cast<const float4 *>(stage + srow * kStageStride + 16 * i + 4 * seg);
+      z[4 * i] = q.x;
+      z[4 * i + 1] = q.y;
+      z[4 * i + 2] = q.z;
+      z[4 * i + 3] = q.w;
+    }
+    const int na = ((prm.block_modes >> (2 * p)) & 1) ? real_logits(2 * p) : 0;
+    const int nb = ((prm.block_modes >> (2 * p + 1)) & 1) ? real_logits(2 * p + 1) : 0;
+    // a logit that names no real candidate of an allowed mode drops out as
+    // -inf: never the maximum, exp() = 0 in the sum, never in the top-k
+#pragma unroll
+    for (int k = 0; k < 32; k++) {
+      const int slot = 16 * ((k >> 2) & 3) + 4 * seg + (k & 3);
+      z[k] = slot < (k < 16 ? na : nb) ? z[k] : -INFINITY;
+    }
+    float pm = -INFINITY;
+#pragma unroll
+    for (int k = 0; k < 32; k++) pm = fmaxf(pm, z[k]);
+    pm = fmaxf(pm, __shfl_xor(pm, 1));
+    pm = fmaxf(pm, __shfl_xor(pm, 2));
+    const float mn = fmaxf(m, pm);
+    const float shift = mn > -INFINITY ? mn : 0.0f;
+    float ps = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 32; k++) ps += expf(z[k] - shift);
+    ps += __shfl_xor(ps, 1);
+    ps += __shfl_xor(ps, 2);
+    if (mn > -INFINITY) {  // the same in the row's four threads
+      sum = (m > -INFINITY ? sum * expf(m - mn) : 0.0f) + ps;
+      m = mn;
+    }
+    // top-kk of the running list and this phase's logits: the t-th entry is
+    // the first in list order after entry t - 1
+    const int jbase = 128 * p + 4 * seg;
+    float nv[8];
+    int nj[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      nv[i] = -INFINITY;
+      nj[i] = 1 << 30;
+    }
+    float pv = INFINITY;
+    int pj = -1;
+#pragma unroll 1
+    for (int t = 0; t < kk; t++) {
+      float bv = -INFINITY;
+      int bj = 1 << 30;
+      // j rises with k, so among equal logits the first seen stays
+#pragma unroll
+      for (int k = 0; k < 32; k++) {
+        const int j = jbase + 16 * (k >> 2) + (k & 3);
+        const bool take = top_before(pv, pj, z[k], j) & (z[k] > bv);
+        bv = take ? z[k] : bv;
+        bj = take ? j : bj;
+      }
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const bool after = top_before(pv, pj, lv[i], lj[i]);
+        const bool better = top_before(lv[i], lj[i], bv, bj);
+        const bool take = (seg == 0) & after & better;
+        bv = take ? lv[i] : bv;
+        bj = take ? lj[i] : bj;
+      }
+#pragma unroll
+      for (int off = 1; off < 4; off <<= 1) {
+        const float ov = __shfl_xor(bv, off);
+        const int oj = __shfl_xor(bj, off);
+        const bool take = top_before(ov, oj, bv, bj);
+        bv = take ? ov : bv;
+        bj = take ? oj : bj;
+      }
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        nv[i] = i == t ? bv : nv[i];
+        nj[i] = i == t ? bj : nj[i];
+      }
+      pv = bv;
+      pj = bj;
     }
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-    float sum = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 8; i++)
-      if (ok[i]) sum += expf(z[i] - mx);
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) sum += __shfl_xor(sum, off);
-    // top probability = exp(0) / sum
-    if (!(mx > -INFINITY) || 1.0f / sum < prm.threshold) {
-      fell_back = true;
-      if (lane == 0) fb_slot = atomicAdd(&s_fb, 1u);
-    } else {
-      const int kk = (int)(prm.topk < 1 ? 1 : (prm.topk > 8 ? 8 : prm.topk));
-      for (int t = 0; t < kk; t++) {
-        float bv = -INFINITY;
-        int bj = 1 << 30;
-#pragma unroll
-        for (int i = 0; i < 8; i++)
-          if (ok[i] && z[i] > bv) {  // i ascending: the lower logit wins ties
-            bv = z[i];
-            bj = lane + 64 * i;
-          }
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-          const float ov = __shfl_xor(bv, off);
-          const int oj = __shfl_xor(bj, off);
-          if (ov > bv || (ov == bv && oj < bj)) {
-            bv = ov;
-            bj = oj;
-          }
-        }
-        if (!(bv > -INFINITY)) break;  // uniform
-#pragma unroll
-        for (int i = 0; i < 8; i++)
-          if (lane + 64 * i == bj) ok[i] = false;
-        if (lane == t) my_desc = cdesc_of_logit(bj);
-        count++;
-      }
-      if (lane < count) w.cdesc[(size_t)rel * kMaxCands + lane] = (uint16_t)my_desc;
-      if (lane == 0) w.sel[rel] = sel_pack(2, count);
+    for (int i = 0; i < 8; i++) {
+      lv[i] = nv[i];
+      lj[i] = nj[i];
     }
   }
+
+  // solid and transparent blocks, from the row's four texel rows
+  const uint32_t rel = tile0 + (uint32_t)srow;
+  uint4 px = make_uint4(0, 0, 0, 0);
+  if (rel < n) {
+    const uint32_t b = r.first_block + rel;
+    const uint32_t bx = b % r.blocks_wide, by = b / r.blocks_wide;
+    px = *reinterpret_cast<const uint4 *>(r.img + (size_t)(by * 4 + seg) * r.width + bx * 4);
+  }
+  const uint32_t p0 = __shfl(px.x, lane & ~3);
+  uint32_t flags = ((px.x != p0 || px.y != p0 || px.z != p0 || px.w != p0) ? 1u : 0u) |
+                   (((px.x | px.y | px.z | px.w) >> 24) ? 2u : 0u);
+  flags |= __shfl_xor(flags, 1);
+  flags |= __shfl_xor(flags, 2);
+  uint32_t count = 0, fbslot = 0;
+  if (seg == 0 && rel < n) {
+    if (!(flags & 1u) || !(flags & 2u)) {
+      w.sel[rel] = sel_pack((flags & 1u) ? 1 : 0, 0);  // solid, else transparent
+    } else if (!(m > -INFINITY) || 1.0f / sum < prm.threshold) {
+      // top probability = exp(0) / sum below the threshold: the full search
+      fbslot = atomicAdd(&s_fb, 1u) + 1u;  // 1-based: 0 marks no fallback
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        if (i < kk && lv[i] > -INFINITY) {
+          const uint16_t d = (uint16_t)cdesc_of_logit(lj[i]);
+          s_cd[srow][i] = d;
+          w.cdesc[(size_t)rel * kMaxCands + i] = d;
+          count++;
+        }
+      }
+      w.sel[rel] = sel_pack(2, (int)count);
+    }
+  }
+  if (seg == 0) {
+    s_count[srow] = count;
+    s_fbslot[srow] = fbslot;
+  }
+  __syncthreads();
   // fallback list: one global atomic per workgroup
+  if (tid == 0 && s_fb) s_fbbase = atomicAdd(fallback_count, s_fb);
   __syncthreads();
-  if (threadIdx.x == 0 && s_fb) s_fbbase = atomicAdd(fallback_count, s_fb);
-  __syncthreads();
-  if (fell_back && lane == 0) fallback[s_fbbase + fb_slot] = rel;
-  wg_enqueue(w, lane < count, kTypeOfMode[my_desc & 7], cand_id(rel, (uint32_t)lane), s_cnt,
-             s_base);
+  if (tid < mlp::kRankRows && s_fbslot[tid])
+    fallback[s_fbbase + s_fbslot[tid] - 1u] = tile0 + (uint32_t)tid;
+  // enqueue the confident blocks' candidates: 64 rows x 8 slots
+  for (int u = 0; u < 2; u++) {
+    const int item = 256 * u + tid;
+    const int row = item >> 3, slot = item & 7;
+    const bool valid = (uint32_t)slot < s_count[row];
+    const int type = valid ? kTypeOfMode[s_cd[row][slot] & 7] : 0;
+    wg_enqueue(w, valid, type, cand_id(tile0 + (uint32_t)row, (uint32_t)slot), s_cnt, s_base);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -3208,10 +3341,8 @@ size_t bc7_work_bytes(uint32_t num_blocks, bool learned) {
                  n * bc7::kNumTypes * bc7::kMaxPerType * 4 +
                  n * bc7::kMaxCands * sizeof(bc7::AState) + n * 16 + 8 * 256;
   if (learned) {
-    // q0 errors, candidate errors, annealing list, fallback list, one chunk of logits
-    const size_t c = n < kBc7MlpChunk ? n : kBc7MlpChunk;
-    const size_t rows = (c + 127) & ~(size_t)127;
-    bytes += n * 8 + n * bc7::kMaxCands * (8 + 4 + 4) + n * 4 + rows * 512 * 4 +
+    // q0 errors, candidate errors, annealing list, fallback list
+    bytes += n * 8 + n * bc7::kMaxCands * (8 + 4 + 4) + n * 4 +
              (n * bc7::kMaxCands / bc7::kListChunk + 1) * 4 + 11 * 256;
   }
   return bytes;
@@ -3249,21 +3380,17 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   const bool learned = p.ranker != nullptr;
   unsigned long long *q0err = nullptr;
   uint32_t *fallback = nullptr;
-  float *mlogits = nullptr;
-  const uint32_t mchunk = n < kBc7MlpChunk ? n : kBc7MlpChunk;
   unsigned long long *cerr = nullptr;
   w.alist = nullptr;
   w.aid = nullptr;
   uint32_t *alist = nullptr, *chunk_count = nullptr;
   if (learned) {
-    const size_t rows = ((size_t)mchunk + 127) & ~(size_t)127;
     q0err = reinterpret_cast<unsigned long long *>(carve((size_t)n * 8));
     cerr = reinterpret_cast<unsigned long long *>(carve(nc * 8));
     alist = reinterpret_cast<uint32_t *>(carve(nc * 4));
     w.aid = reinterpret_cast<uint32_t *>(carve(nc * 4));
     chunk_count = reinterpret_cast<uint32_t *>(carve((nc / bc7::kListChunk + 1) * 4));
     fallback = reinterpret_cast<uint32_t *>(carve((size_t)n * 4));
-    mlogits = reinterpret_cast<float *>(carve(rows * 512 * 4));
   }
   uint32_t *fallback_count = w.counters + 20;
   const uint32_t nchunks = (n + 255) / 256;
@@ -3282,22 +3409,15 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   hipLaunchKernelGGL(bc7::chunk_scan_kernel, dim3(1), dim3(1024), 0, stream, s);
   timing_end(t, kTagBc7Scan, stream);
   if (learned) {
-    // rank each chunk's candidates, keep the confident top-k, then run
-    // BoxSelection over the blocks the ranker was not sure about
-    for (uint32_t c0 = 0; c0 < n; c0 += mchunk) {
-      const uint32_t cn = n - c0 < mchunk ? n - c0 : mchunk;
-      BlockRange rc = r;
-      rc.first_block = r.first_block + c0;
-      rc.num_blocks = cn;
-      t = timing_begin(stream);
-      e = ranker_logits(*p.ranker, rc, cn, mlogits, stream);
-      timing_end(t, kTagBc7Rank, stream);
-      if (e != hipSuccess) return e;
-      t = timing_begin(stream);
-      hipLaunchKernelGGL(bc7::learned_select_kernel, dim3((cn + 15) / 16), dim3(1024), 0, stream,
-                         r, p, w, c0, cn, (const float *)mlogits, fallback, fallback_count);
-      timing_end(t, kTagBc7Select, stream);
-    }
+    // rank every block and keep the confident top-k, then run BoxSelection
+    // over the blocks the ranker was not sure about
+    const Ranker &rk = *p.ranker;
+    t = timing_begin(stream);
+    hipLaunchKernelGGL(bc7::rank_select_kernel, dim3((n + mlp::kRankRows - 1) / mlp::kRankRows),
+                       dim3(256), 0, stream, r, p, w, (const uint16_t *)rk.w1_oi,
+                       (const uint16_t *)rk.w2_oi, (const uint16_t *)rk.w3_oi,
+                       (const float *)rk.master, fallback, fallback_count);
+    timing_end(t, kTagBc7Rank, stream);
     t = timing_begin(stream);
     hipLaunchKernelGGL(bc7::select_kernel<bc7::kSelBox>, dim3(ngroups), dim3(256), 0, stream,
                        r, p, w, (const uint32_t *)fallback, (const uint32_t *)fallback_count);

@@ -61,9 +61,6 @@ const uint32_t *bc7_single_color_table(int device);
 // activations).
 size_t bc7_work_bytes(uint32_t num_blocks, bool learned);
 
-// Blocks per ranker forward pass inside bc7_encode.
-constexpr uint32_t kBc7MlpChunk = 1u << 18;
-
 // Largest block count one bc7_encode call takes; callers split bigger jobs.
 constexpr uint32_t kBc7MaxBlocksPerCall = 1u << 20;
 
@@ -217,11 +214,6 @@ struct fh_ranker {
 
 namespace fh {
 
-// Fused inference of the n blocks of r into logits (n x 512 fp32), bit-identical
-// to the unfused forward pass; the hidden activations never leave the chip.
-// Reads only the ranker's weights.
-hipError_t ranker_logits(const Ranker &rk, const BlockRange &r, uint32_t n, float *logits,
-                         hipStream_t s);
 
 // Host-side table builders (tables.cpp); both mirror generator programs the
 // reference documents.

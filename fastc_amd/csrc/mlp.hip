@@ -16,21 +16,10 @@
 #include <stdint.h>
 
 #include "internal.h"
+#include "ranker_tile.h"
 
 namespace fh {
 namespace mlp {
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float floatx16 __attribute__((ext_vector_type(16)));
-
-// float -> bfloat16 bits, round to nearest even (inf/nan kept).
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)(u >> 16);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // ---------------------------------------------------------------------------
 // NT product with fused epilogue.  A workgroup of four waves computes a
@@ -204,110 +193,6 @@ __global__ void __launch_bounds__(256) cast_kernel(const float *W, int nin, int 
 }
 
 
-// ---------------------------------------------------------------------------
-// Fused inference for the learned search.  A workgroup of four waves takes
-// 64 blocks: their features, then each layer as 32x32x16 MFMA tiles whose
-// accumulation runs over K in the same order as gemm_nt_kernel, so the
-// logits are bit-identical to the unfused forward pass; the bf16 hidden
-// activations stay in LDS, and only the fp32 logits are written.
-// ---------------------------------------------------------------------------
-constexpr int kRankOffB1 = 64 * 256, kRankOffB2 = kRankOffB1 + 256 + 256 * 256,
-              kRankOffB3 = kRankOffB2 + 256 + 256 * 512;  // biases in the parameter vector
-constexpr int kRankRows = 64;
-constexpr int kXs = 72;    // LDS row strides in bf16 elements: 16-byte multiples whose
-constexpr int kHs = 264;   // rows fall on different banks for the operand loads
-constexpr int kLdsX = 0, kLdsH1 = kRankRows * kXs * 2, kLdsH2 = kLdsH1 + kRankRows * kHs * 2;
-constexpr int kLdsBytes = kLdsH2 + kRankRows * kHs * 2;
-
-// OUT[64 x NT*32 per wave] = IN[64 x K] W^T + b for wave w's column tiles
-// c0 + 32 t; RELU layers round to bf16 into LDS, the output layer writes fp32
-// logits to global memory.
-template <int K, int SIN, int NT, bool RELU>
-__device__ __forceinline__ void rank_layer(const uint16_t *in, const uint16_t *Wt,
-                                           const float *bias, int c0, int lane, uint16_t *hout,
-                                           float *lout, uint32_t rows_left) {
-  const int r = lane & 31, kh = (lane >> 5) * 8;
-  floatx16 acc[2][NT] = {};
-#pragma unroll 4
-  for (int k = 0; k < K; k += 16) {
-    const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(in + r * SIN + k + kh);
-    const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(in + (32 + r) * SIN + k + kh);
-#pragma unroll
-    for (int t = 0; t < NT; t++) {
-      const bf16x8 b =
-          *reinterpret_cast<const bf16x8 *>(Wt + (size_t)(c0 + t * 32 + r) * K + k + kh);
-      acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, acc[0][t], 0, 0, 0);
-      acc[1][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, acc[1][t], 0, 0, 0);
-    }
-  }
-#pragma unroll
-  for (int t = 0; t < NT; t++) {
-    const int col = c0 + t * 32 + r;
-    const float bv = bias[col];
-#pragma unroll
-    for (int rb = 0; rb < 2; rb++)
-#pragma unroll
-      for (int v = 0; v < 16; v++) {
-        const int row = rb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        float x = acc[rb][t][v] + bv;
-        if (RELU) {
-          if (x < 0.0f) x = 0.0f;
-          hout[row * kHs + col] = f2bf(x);
-        } else if ((uint32_t)row < rows_left) {
-          lout[(size_t)row * 512 + col] = x;
-        }
-      }
-  }
-}
-
-__global__ void __launch_bounds__(256) infer_kernel(BlockRange r, uint32_t n, const uint16_t *w1,
-                                                    const uint16_t *w2, const uint16_t *w3,
-                                                    const float *params, float *logits) {
-  __shared__ __align__(16) uint8_t lds[kLdsBytes];
-  uint16_t *xs = reinterpret_cast<uint16_t *>(lds + kLdsX);
-  uint16_t *h1 = reinterpret_cast<uint16_t *>(lds + kLdsH1);
-  uint16_t *h2 = reinterpret_cast<uint16_t *>(lds + kLdsH2);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint32_t tile0 = blockIdx.x * kRankRows;
-  {
-    // features: one texel row of one block per thread, RGBA / 255 in bf16,
-    // as features_kernel computes them
-    const int row = tid >> 2, y = tid & 3;
-    const bool real = tile0 + row < n;
-    uint4 px = make_uint4(0, 0, 0, 0);
-    if (real) {
-      const uint32_t b = r.first_block + tile0 + row;
-      const uint32_t bx = b % r.blocks_wide, by = b / r.blocks_wide;
-      px = *reinterpret_cast<const uint4 *>(r.img + (size_t)(by * 4 + y) * r.width + bx * 4);
-    }
-    const uint32_t p4[4] = {px.x, px.y, px.z, px.w};
-    uint32_t packed[8];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-      for (int c = 0; c < 4; c += 2) {
-        const uint16_t f0 = real ? f2bf((float)((p4[i] >> (8 * c)) & 0xFFu) / 255.0f) : 0;
-        const uint16_t f1 = real ? f2bf((float)((p4[i] >> (8 * c + 8)) & 0xFFu) / 255.0f) : 0;
-        packed[2 * i + c / 2] = f0 | ((uint32_t)f1 << 16);
-      }
-    *reinterpret_cast<uint4 *>(xs + row * kXs + 16 * y) =
-        make_uint4(packed[0], packed[1], packed[2], packed[3]);
-    *reinterpret_cast<uint4 *>(xs + row * kXs + 16 * y + 8) =
-        make_uint4(packed[4], packed[5], packed[6], packed[7]);
-  }
-  __syncthreads();
-  rank_layer<64, kXs, 2, true>(xs, w1, params + kRankOffB1, wave * 64, lane, h1, nullptr, 0);
-  __syncthreads();
-  rank_layer<256, kHs, 2, true>(h1, w2, params + kRankOffB2, wave * 64, lane, h2, nullptr, 0);
-  __syncthreads();
-  const uint32_t rows_left = n - tile0;
-  float *lrow = logits + (size_t)tile0 * 512;
-  // output layer: wave w writes logit columns 128w..128w+127, 64 at a time
-  rank_layer<256, kHs, 2, false>(h2, w3, params + kRankOffB3, wave * 128, lane, nullptr, lrow,
-                                 rows_left);
-  rank_layer<256, kHs, 2, false>(h2, w3, params + kRankOffB3, wave * 128 + 64, lane, nullptr,
-                                 lrow, rows_left);
-}
 
 }  // namespace mlp
 
@@ -342,6 +227,9 @@ constexpr int kIn = 64, kH = 256, kOut = 512;
 constexpr size_t kOffW1 = 0, kOffB1 = kOffW1 + kIn * kH, kOffW2 = kOffB1 + kH,
                  kOffB2 = kOffW2 + kH * kH, kOffW3 = kOffB2 + kH, kOffB3 = kOffW3 + kH * kOut,
                  kNumParams = kOffB3 + kOut;
+static_assert(mlp::kRankOffB1 == (int)kOffB1 && mlp::kRankOffB2 == (int)kOffB2 &&
+                  mlp::kRankOffB3 == (int)kOffB3,
+              "bias offsets of the inference tile");
 
 size_t ranker_num_params() { return kNumParams; }
 
@@ -451,16 +339,6 @@ hipError_t Ranker::forward(const BlockRange &r, const uint32_t *idx, uint32_t n,
   return launch_gemm(g, s);
 }
 
-hipError_t ranker_logits(const Ranker &rk, const BlockRange &r, uint32_t n, float *logits,
-                         hipStream_t s) {
-  static_assert(mlp::kRankOffB1 == (int)kOffB1 && mlp::kRankOffB2 == (int)kOffB2 &&
-                    mlp::kRankOffB3 == (int)kOffB3,
-                "bias offsets");
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(mlp::infer_kernel, dim3((n + mlp::kRankRows - 1) / mlp::kRankRows),
-                     dim3(256), 0, s, r, n, rk.w1_oi, rk.w2_oi, rk.w3_oi, rk.master, logits);
-  return hipGetLastError();
-}
 
 hipError_t Ranker::backward(const int32_t *labels, const uint32_t *idx, uint32_t n,
                             uint32_t rows, hipStream_t s) {

