@@ -440,6 +440,64 @@ def run_stats(args):
     print(json.dumps(line), flush=True)
 
 
+def bc7_q0_leg(with_cpu=True, steps=50, warmup=5):
+    """BC7 at quality 0 on the 256x256 seed-1 image, the reference's
+    single-thread configuration (CompressImageData, BPTC, iQuality 0, -t 1):
+    the GPU encode with the input resident in HBM (4096 blocks: launch-bound,
+    a parity configuration rather than a throughput one), its bytes against
+    the oracle's, PSNR under both decoders, and the oracle on one thread."""
+    import torch
+    import fastc_amd
+    from fastc_amd import synth
+    W = H = 256
+    img = synth.generate(W, H, 1)
+    out_bytes = W * H
+    dev = torch.device("cuda:0")
+    d_in = torch.from_numpy(img.view(np.int32)).to(dev)
+    d_out = torch.zeros(out_bytes, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        fastc_amd.compress_device(d_in.data_ptr(), d_out.data_ptr(), out_bytes, W, H,
+                                  quality=0, stream=stream.cuda_stream)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    # bytes of one encode with the reference's initial watermark counter
+    got = fastc_amd.compress(img, quality=0, wm_counter=ctypes.c_uint32(0xFFFFFFFF))
+    leg = {"workload": "BC7 encode q=0, 256x256 RGBA8, seed 1", "ms_per_step": ms,
+           "steps": steps, "value": W * H / (ms / 1e3) / 1e6, "unit": "Mtexels/s",
+           "psnr_db": psnr(img, fastc_amd.decompress(got, W, H)),
+           "psnr_spec_db": psnr(img, fastc_amd.decompress(
+               got, W, H, fmt=fastc_amd.eCompressionFormat_BPTC | fastc_amd.DECODE_SPEC))}
+    if with_cpu:
+        from oracle import pyoracle
+        pyoracle.lib()
+        walls = []
+        ref = None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ref, _ = pyoracle.bc7_compress(img, quality=0)
+            walls.append(time.perf_counter() - t0)
+        wall = float(np.median(walls))
+        leg["bit_exact_vs_oracle"] = bool(np.array_equal(got, ref))
+        leg["cpu_baseline"] = {
+            "value": W * H / wall / 1e6, "unit": "Mtexels/s", "cores": 1, "kind": "port",
+            "sample": f"the whole 256x256 image at q=0 on one thread (the reference's -t 1), "
+                      f"median of 3 runs ({wall * 1e3:.1f} ms)",
+            "host": host_info()}
+    return leg
+
+
 def simple_leg(args, fmt, with_cpu=True):
     """DXT1 / DXT5 / ETC1 on a ring of 16 distinct 2048x2048 images (seeds
     2..17, 5% hard edges): one batch graph of 16 surfaces, so the working set
@@ -699,6 +757,8 @@ def main():
             tile_sse = int(errs[:t // 4, :t // 4].sum(dtype=np.uint64))
             cb[key] = fastc_amd.psnr_from_sse((tile_sse, 0), t, t)
         line["cpu_baseline"] = cb
+    line["bc7_q0_256"] = bc7_q0_leg(with_cpu=not args.no_cpu_baseline)
+    print("[bench] BC7 q=0 256x256 done", file=sys.stderr, flush=True)
     if not args.no_other_formats:
         formats = {}
         for name, fmt in (("dxt1", 0), ("dxt5", 1), ("etc1", 2)):
