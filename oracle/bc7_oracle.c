@@ -413,9 +413,10 @@ static double qerr(const clus *c, v4 p1, v4 p2, int nb, uint32_t mask, v4 metric
   return total;
 }
 
-double ora_quantized_error(const uint32_t px[16], const uint8_t *sel, int n,
-                           const float p1[4], const float p2[4], int nbuckets,
-                           uint32_t mask, const int *pbits, uint8_t *indices) {
+double ora_quantized_error_w(const uint32_t px[16], const uint8_t *sel, int n,
+                             const float p1[4], const float p2[4], int nbuckets,
+                             uint32_t mask, const int *pbits, const float w[4],
+                             uint8_t *indices) {
   clus c;
   clus_init(&c, px);
   if (sel) {
@@ -423,7 +424,14 @@ double ora_quantized_error(const uint32_t px[16], const uint8_t *sel, int n,
     for (int i = 0; i < n; i++) c.map[i] = sel[i];
   }
   return qerr(&c, v4_set(p1[0], p1[1], p1[2], p1[3]), v4_set(p2[0], p2[1], p2[2], p2[3]),
-              nbuckets, mask, v4_splat(1.0f), pbits, indices);
+              nbuckets, mask, v4_set(w[0], w[1], w[2], w[3]), pbits, indices);
+}
+
+double ora_quantized_error(const uint32_t px[16], const uint8_t *sel, int n,
+                           const float p1[4], const float p2[4], int nbuckets,
+                           uint32_t mask, const int *pbits, uint8_t *indices) {
+  static const float ones[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+  return ora_quantized_error_w(px, sel, n, p1, p2, nbuckets, mask, pbits, ones, indices);
 }
 
 /* MatrixSquare<float,4>::PowerMethod (MatrixSquare.h:44-105). */
@@ -582,10 +590,24 @@ static const int *m_pbc(const cmode *m, int idx) {
   return kPBits[0];
 }
 
-/* GetErrorMetric with rotation (CompressionMode.h:196-205); uniform weights. */
-static v4 m_metric(const cmode *m) {
-  (void)m;
+/* kErrorMetrics (Compressor.cpp:205-208): per-channel error weights of the
+ * uniform (0) and nonuniform (1) metrics.  QuantizedError multiplies each
+ * channel's absolute difference by its weight before squaring. */
+static v4 metric_weights(uint32_t metric) {
+  if (metric == 1) return v4_set(sqrtf(0.3f), sqrtf(0.56f), sqrtf(0.11f), 1.0f);
   return v4_splat(1.0f);
+}
+
+/* GetErrorMetric (CompressionMode.h:196-205): the weights with the rotated
+ * channel's weight swapped with alpha's; m->metric holds them unrotated. */
+static v4 m_metric(const cmode *m) {
+  const v4 w = m->metric;
+  switch (m_rot(m)) {
+    case 1: return v4_set(w.v[3], w.v[1], w.v[2], w.v[0]);
+    case 2: return v4_set(w.v[0], w.v[3], w.v[2], w.v[1]);
+    case 3: return v4_set(w.v[0], w.v[1], w.v[3], w.v[2]);
+    default: return w;
+  }
 }
 
 /* Counter-based annealing stream (shared definition with the HIP kernels). */
@@ -733,13 +755,13 @@ static double single_color(const cmode *m, v4 p, v4 *p1, v4 *p2, uint8_t *best_p
   return best;
 }
 
-double ora_single_color(int mode, int idx_mode, const float p[4], float p1[4], float p2[4],
-                        int *pbc) {
+double ora_single_color_m(int mode, int idx_mode, uint32_t metric, const float p[4],
+                          float p1[4], float p2[4], int *pbc) {
   cmode m;
   memset(&m, 0, sizeof(m));
   m.a = &kModes[mode & 7];
   m.idx_mode = idx_mode;
-  m.metric = v4_splat(1.0f);
+  m.metric = metric_weights(metric);
   v4 pv, a, b;
   for (int k = 0; k < 4; k++) pv.v[k] = p[k];
   uint8_t bp = 0;
@@ -750,6 +772,11 @@ double ora_single_color(int mode, int idx_mode, const float p[4], float p1[4], f
   }
   *pbc = bp;
   return e;
+}
+
+double ora_single_color(int mode, int idx_mode, const float p[4], float p1[4], float p2[4],
+                        int *pbc) {
+  return ora_single_color_m(mode, idx_mode, 0, p, p1, p2, pbc);
 }
 
 /* ChangePointForDir{Without,With}PbitChange (Compressor.cpp:364-418). */
@@ -1192,10 +1219,10 @@ static double compress_cluster_alpha(cmode *m, const clus *c, v4 *p1, v4 *p2, ui
  * partition shape `shape` of the mode's subset count (SetShapeIndex +
  * SetPartition, RGBAEndpoints.h:205-226). */
 static void fit_setup(cmode *m, clus *c, const uint32_t px[16], int mode, int shape, int subset,
-                      int rot, int idx_mode) {
+                      int rot, int idx_mode, uint32_t metric) {
   memset(m, 0, sizeof(*m));
   m->a = &kModes[mode & 7];
-  m->metric = v4_splat(1.0f);
+  m->metric = metric_weights(metric);
   m->rot = rot;
   m->idx_mode = idx_mode;
   clus_init(c, px);
@@ -1227,11 +1254,12 @@ void ora_clamp_to_grid(int mode, float p1[4], float p2[4], int *pbc) {
   *pbc = bp;
 }
 
-double ora_fit_rgb(const uint32_t px[16], int mode, int shape, int subset, int idx_mode,
-                   ora_fit_trace *tr, float p1[4], float p2[4], uint8_t idx[16], int *pbc) {
+double ora_fit_rgb_m(const uint32_t px[16], int mode, int shape, int subset, int idx_mode,
+                     uint32_t metric, ora_fit_trace *tr, float p1[4], float p2[4],
+                     uint8_t idx[16], int *pbc) {
   cmode m;
   clus c;
-  fit_setup(&m, &c, px, mode, shape, subset, 0, idx_mode);
+  fit_setup(&m, &c, px, mode, shape, subset, 0, idx_mode, metric);
   m.trace = tr;
   v4 a, b;
   uint8_t bp = 0;
@@ -1243,11 +1271,12 @@ double ora_fit_rgb(const uint32_t px[16], int mode, int shape, int subset, int i
   return e;
 }
 
-double ora_fit_alpha(const uint32_t px[16], int mode, int rot, int idx_mode, ora_fit_trace *tr,
-                     float p1[4], float p2[4], uint8_t idx[16], uint8_t aidx[16]) {
+double ora_fit_alpha_m(const uint32_t px[16], int mode, int rot, int idx_mode, uint32_t metric,
+                       ora_fit_trace *tr, float p1[4], float p2[4], uint8_t idx[16],
+                       uint8_t aidx[16]) {
   cmode m;
   clus c;
-  fit_setup(&m, &c, px, mode, 0, 0, rot, idx_mode);
+  fit_setup(&m, &c, px, mode, 0, 0, rot, idx_mode, metric);
   m.trace = tr;
   v4 a, b;
   memset(idx, 0, 16);
@@ -1256,6 +1285,16 @@ double ora_fit_alpha(const uint32_t px[16], int mode, int rot, int idx_mode, ora
   trace_v4(p1, a);
   trace_v4(p2, b);
   return e;
+}
+
+double ora_fit_rgb(const uint32_t px[16], int mode, int shape, int subset, int idx_mode,
+                   ora_fit_trace *tr, float p1[4], float p2[4], uint8_t idx[16], int *pbc) {
+  return ora_fit_rgb_m(px, mode, shape, subset, idx_mode, 0, tr, p1, p2, idx, pbc);
+}
+
+double ora_fit_alpha(const uint32_t px[16], int mode, int rot, int idx_mode, ora_fit_trace *tr,
+                     float p1[4], float p2[4], uint8_t idx[16], uint8_t aidx[16]) {
+  return ora_fit_alpha_m(px, mode, rot, idx_mode, 0, tr, p1, p2, idx, aidx);
 }
 
 /* Fit tag for the counter RNG: identifies (mode, shape entry, subset,
@@ -1426,11 +1465,11 @@ static const uint32_t kThreePartModes = (1u << 0) | (1u << 2);
 static const uint32_t kAlphaModes = (1u << 4) | (1u << 5) | (1u << 6) | (1u << 7);
 
 /* EstimateTwo/ThreeClusterError (Compressor.cpp:1626-1655). */
-static double estimate(const clus *c, int nb) {
+static double estimate(const clus *c, int nb, v4 w) {
   v4 v = v4_sub(c->mx, c->mn);
   if (v4_dot(v, v) == 0) return 0.0;
   double e = 0.0001;
-  e += qerr(c, c->mn, c->mx, nb, 0xFFFFFFFFu, v4_splat(1.0f), NULL, NULL);
+  e += qerr(c, c->mn, c->mx, nb, 0xFFFFFFFFu, w, NULL, NULL);
   return e;
 }
 
@@ -1440,8 +1479,10 @@ typedef struct {
   uint32_t modes;
 } shape_sel;
 
-/* BoxSelection (Compressor.cpp:1670-1750). */
-static shape_sel box_selection(const uint32_t px[16]) {
+/* BoxSelection (Compressor.cpp:1670-1750); its estimates weigh the channels
+ * by the unrotated metric (:1634, 1650). */
+static shape_sel box_selection(const uint32_t px[16], uint32_t metric) {
+  const v4 w = metric_weights(metric);
   shape_sel r;
   r.modes = 0xFF;
   memset(r.sh_idx, 0, sizeof(r.sh_idx));
@@ -1457,7 +1498,7 @@ static shape_sel box_selection(const uint32_t px[16]) {
     double err = 0.0;
     for (int ci = 0; ci < 2; ci++) {
       clus_set_partition(&c, (uint32_t)ci);
-      err += estimate(&c, 8);
+      err += estimate(&c, 8, w);
     }
     if (err < best[0]) {
       best[0] = err;
@@ -1480,7 +1521,7 @@ static shape_sel box_selection(const uint32_t px[16]) {
     double err = 0.0;
     for (int ci = 0; ci < 3; ci++) {
       clus_set_partition(&c, (uint32_t)ci);
-      err += estimate(&c, 4);
+      err += estimate(&c, 4, w);
     }
     if (err < best[1]) {
       best[1] = err;
@@ -1495,13 +1536,18 @@ static shape_sel box_selection(const uint32_t px[16]) {
   return r;
 }
 
-void ora_bc7_box_selection(const uint32_t px[16], uint32_t *nshapes, uint32_t *shape2,
-                           uint32_t *shape3, uint32_t *modes) {
-  shape_sel s = box_selection(px);
+void ora_bc7_box_selection_m(const uint32_t px[16], uint32_t metric, uint32_t *nshapes,
+                             uint32_t *shape2, uint32_t *shape3, uint32_t *modes) {
+  shape_sel s = box_selection(px, metric);
   *nshapes = s.nshapes;
   *shape2 = (uint32_t)s.sh_idx[0];
   *shape3 = (uint32_t)s.sh_idx[1];
   *modes = s.modes;
+}
+
+void ora_bc7_box_selection(const uint32_t px[16], uint32_t *nshapes, uint32_t *shape2,
+                           uint32_t *shape3, uint32_t *modes) {
+  ora_bc7_box_selection_m(px, 0, nshapes, shape2, shape3, modes);
 }
 
 /* CompressOptimalColorBC7 (Compressor.cpp:1424-1458). */
@@ -1566,7 +1612,7 @@ static void compress_clusters(const shape_sel *sel, const uint32_t px[16],
       memset(&m, 0, sizeof(m));
       m.a = &kModes[mode];
       m.sa_steps = s->quality;
-      m.metric = v4_splat(1.0f);
+      m.metric = metric_weights(s->error_metric);
       m.rng_kind = s->rng_kind;
       m.lcg = s->lcg_state;
       m.seed = s->seed;
@@ -1804,7 +1850,7 @@ void ora_bc7_compress_block(const uint32_t px[16], uint8_t out[16], const ora_bc
     if (st) { st->path = 1; st->mode = 6; }
     return;
   }
-  shape_sel sel = box_selection(px);
+  shape_sel sel = box_selection(px, s->error_metric);
   sel.modes &= s->block_modes;
   compress_clusters(&sel, px, s, block_id, out, st);
   const int kept = q0_guard(&sel, px, s, block_id, out);
@@ -1853,15 +1899,15 @@ static void update_estimate(double *est, int mode, double e) {
  * of the subset between its bounding-box corners, quantised as mode 1 (mask
  * 0xFFFCFCFC) and mode 3 (0xFFFEFEFE) endpoints; the estimates are the two
  * errors, the result 0.0001 + the smaller; 0 for a one-colour subset. */
-static double est_two_stats(const clus *c, double se[2]) {
+static double est_two_stats(const clus *c, double se[2], v4 w) {
   const v4 v = v4_sub(c->mx, c->mn);
   if (v4_dot(v, v) == 0) {
     se[0] = se[1] = 0.0;
     return 0.0;
   }
-  const double e1 = qerr(c, c->mn, c->mx, 8, 0xFFFCFCFCu, v4_splat(1.0f), NULL, NULL);
+  const double e1 = qerr(c, c->mn, c->mx, 8, 0xFFFCFCFCu, w, NULL, NULL);
   se[0] = e1 >= 0.0 ? e1 : min_ref(se[0], e1);
-  const double e3 = qerr(c, c->mn, c->mx, 8, 0xFFFEFEFEu, v4_splat(1.0f), NULL, NULL);
+  const double e3 = qerr(c, c->mn, c->mx, 8, 0xFFFEFEFEu, w, NULL, NULL);
   se[1] = e3 >= 0.0 ? e3 : min_ref(se[1], e3);
   double error = 0.0001;
   error += min_ref(e1, e3);
@@ -1871,15 +1917,15 @@ static double est_two_stats(const clus *c, double se[2]) {
 /* EstimateThreeClusterErrorStats (Compressor.cpp:1900-1935): as above with
  * mode 0 (0xFFF0F0F0) and mode 2 (0xFFF8F8F8) endpoints and four index
  * values; here the estimates already include the 0.0001. */
-static double est_three_stats(const clus *c, double se[2]) {
+static double est_three_stats(const clus *c, double se[2], v4 w) {
   const v4 v = v4_sub(c->mx, c->mn);
   if (v4_dot(v, v) == 0) {
     se[0] = se[1] = 0.0;
     return 0.0;
   }
-  const double e0 = 0.0001 + qerr(c, c->mn, c->mx, 4, 0xFFF0F0F0u, v4_splat(1.0f), NULL, NULL);
+  const double e0 = 0.0001 + qerr(c, c->mn, c->mx, 4, 0xFFF0F0F0u, w, NULL, NULL);
   se[0] = e0 >= 0.0 ? e0 : min_ref(se[0], e0);
-  const double e2 = 0.0001 + qerr(c, c->mn, c->mx, 4, 0xFFF8F8F8u, v4_splat(1.0f), NULL, NULL);
+  const double e2 = 0.0001 + qerr(c, c->mn, c->mx, 4, 0xFFF8F8F8u, w, NULL, NULL);
   se[1] = e2 >= 0.0 ? e2 : min_ref(se[1], e2);
   double error = 0.0001;
   error += min_ref(e0, e2);
@@ -1887,7 +1933,9 @@ static double est_three_stats(const clus *c, double se[2]) {
 }
 
 /* The shape scan of the statistics block path (Compressor.cpp:2063-2176). */
-static shape_sel stats_selection(const uint32_t px[16], ora_bc7_log_stats *ls) {
+static shape_sel stats_selection(const uint32_t px[16], ora_bc7_log_stats *ls,
+                                 uint32_t metric) {
+  const v4 w = metric_weights(metric);
   shape_sel r;
   r.modes = 0xFF;
   memset(r.sh_idx, 0, sizeof(r.sh_idx));
@@ -1904,7 +1952,7 @@ static shape_sel stats_selection(const uint32_t px[16], ora_bc7_log_stats *ls) {
       ls->mode_estimate[6] = 0.0;
     } else {
       const double e =
-          0.0001 + qerr(&c, c.mn, c.mx, 4, 0xFEFEFEFEu, v4_splat(1.0f), NULL, NULL);
+          0.0001 + qerr(&c, c.mn, c.mx, 4, 0xFEFEFEFEu, w, NULL, NULL);
       update_estimate(ls->mode_estimate, 6, e);
     }
   }
@@ -1918,7 +1966,7 @@ static shape_sel stats_selection(const uint32_t px[16], ora_bc7_log_stats *ls) {
     for (int ci = 0; ci < 2; ci++) {
       clus_set_partition(&c, (uint32_t)ci);
       double se[2] = {-1.0, -1.0};
-      err += est_two_stats(&c, se);
+      err += est_two_stats(&c, se, w);
       for (int ei = 0; ei < 2; ei++)
         if (se[ei] >= 0.0) est[ei] = est[ei] == -1.0 ? se[ei] : est[ei] + se[ei];
     }
@@ -1947,7 +1995,7 @@ static shape_sel stats_selection(const uint32_t px[16], ora_bc7_log_stats *ls) {
       for (int ci = 0; ci < 3; ci++) {
         clus_set_partition(&c, (uint32_t)ci);
         double se[2] = {-1.0, -1.0};
-        err += est_three_stats(&c, se);
+        err += est_three_stats(&c, se, w);
         for (int ei = 0; ei < 2; ei++)
           if (se[ei] >= 0.0) est[ei] = est[ei] == -1.0 ? se[ei] : est[ei] + se[ei];
       }
@@ -1993,7 +2041,7 @@ void ora_bc7_compress_block_logged(const uint32_t px[16], uint8_t out[16],
       write_transparent(out);
       ls->path = 1;
     } else {
-      shape_sel sel = stats_selection(px, ls);
+      shape_sel sel = stats_selection(px, ls, s->error_metric);
       sel.modes &= s->block_modes;
       compress_clusters(&sel, px, s, block_id, out, NULL);
       q0_guard(&sel, px, s, block_id, out);

@@ -49,6 +49,8 @@ typedef struct {
    * annealed encoding decodes (reference decoder) to a larger premultiplied
    * RGB error than its quality-0 encoding is written as the quality-0 one. */
   int      q0_guard;
+  uint32_t error_metric;  /* m_ErrorMetric (BPTCCompressor.h:73-77, 141-142):
+                           * 0 uniform, 1 nonuniform */
 } ora_bc7_settings;
 
 /* Per-block statistics (mode chosen, path) for tests. */
@@ -184,9 +186,16 @@ int ora_power_method(const float m[16], float eig[4], float *eigval);
 double ora_quantized_error(const uint32_t px[16], const uint8_t *sel, int n,
                            const float p1[4], const float p2[4], int nbuckets,
                            uint32_t mask, const int *pbits, uint8_t *indices);
+/* The same with per-channel error weights w (the uniform metric is all 1). */
+double ora_quantized_error_w(const uint32_t px[16], const uint8_t *sel, int n,
+                             const float p1[4], const float p2[4], int nbuckets,
+                             uint32_t mask, const int *pbits, const float w[4],
+                             uint8_t *indices);
 void ora_bc7_box_selection(const uint32_t px[16], uint32_t *nshapes,
                            uint32_t *shape2, uint32_t *shape3,
                            uint32_t *modes);
+void ora_bc7_box_selection_m(const uint32_t px[16], uint32_t metric, uint32_t *nshapes,
+                             uint32_t *shape2, uint32_t *shape3, uint32_t *modes);
 const uint8_t *ora_table_mode5(void);   /* [256][2] 7-bit endpoint pairs */
 const uint8_t *ora_table_dxt6(void);    /* [256][2][3] */
 /* Partition masks (Shapes.h:23-50), anchors (AnchorTables.h:22-45) and
@@ -235,6 +244,13 @@ double ora_fit_rgb(const uint32_t px[16], int mode, int shape, int subset, int i
  * records the fit of the rotated RGB cluster. */
 double ora_fit_alpha(const uint32_t px[16], int mode, int rot, int idx_mode, ora_fit_trace *tr,
                      float p1[4], float p2[4], uint8_t idx[16], uint8_t aidx[16]);
+/* The two fits above under error metric `metric` (0 uniform, 1 nonuniform). */
+double ora_fit_rgb_m(const uint32_t px[16], int mode, int shape, int subset, int idx_mode,
+                     uint32_t metric, ora_fit_trace *tr, float p1[4], float p2[4],
+                     uint8_t idx[16], int *pbc);
+double ora_fit_alpha_m(const uint32_t px[16], int mode, int rot, int idx_mode, uint32_t metric,
+                       ora_fit_trace *tr, float p1[4], float p2[4], uint8_t idx[16],
+                       uint8_t aidx[16]);
 /* CompressionMode::Pack of given parameters: ep[subset][endpoint][channel]
  * (unquantised floats, quantised by Pack), p-bit combos per subset, the
  * colour (idx) and alpha (aidx) index of every pixel. */
@@ -243,6 +259,8 @@ void ora_bc7_pack(int mode, int shape, int rot, int idx_mode, const float ep[3][
                   uint8_t out[16]);
 double ora_single_color(int mode, int idx_mode, const float p[4], float p1[4], float p2[4],
                         int *pbc);
+double ora_single_color_m(int mode, int idx_mode, uint32_t metric, const float p[4],
+                          float p1[4], float p2[4], int *pbc);
 uint32_t ora_counter_rng(uint32_t key, uint32_t n);
 uint32_t ora_fit_key(uint64_t seed, uint64_t block_id, uint32_t fit_tag);
 

@@ -23,7 +23,8 @@ class Settings(ctypes.Structure):
                 ("rng_kind", ctypes.c_int),
                 ("lcg_state", ctypes.POINTER(ctypes.c_uint32)),
                 ("seed", ctypes.c_uint64),
-                ("q0_guard", ctypes.c_int)]
+                ("q0_guard", ctypes.c_int),
+                ("error_metric", ctypes.c_uint32)]
 
 
 class BlockStats(ctypes.Structure):
@@ -102,6 +103,8 @@ def lib():
                                           ctypes.c_uint32, vp, vp]
         L.ora_quantized_error.restype = ctypes.c_double
         L.ora_bc7_box_selection.argtypes = [vp] + [ctypes.POINTER(ctypes.c_uint32)] * 4
+        L.ora_bc7_box_selection_m.argtypes = [vp, ctypes.c_uint32] + \
+            [ctypes.POINTER(ctypes.c_uint32)] * 4
         L.ora_table_mode5.restype = ctypes.POINTER(ctypes.c_uint8)
         L.ora_table_dxt6.restype = ctypes.POINTER(ctypes.c_uint8)
         L.ora_single_color.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp,
@@ -132,6 +135,20 @@ def lib():
                                     ctypes.POINTER(FitTrace), vp, vp, vp, vp]
         L.ora_fit_alpha.restype = ctypes.c_double
         L.ora_bc7_pack.argtypes = [ctypes.c_int] * 4 + [vp] * 5
+        L.ora_quantized_error_w.argtypes = [vp, vp, ctypes.c_int, vp, vp, ctypes.c_int,
+                                            ctypes.c_uint32, vp, vp, vp]
+        L.ora_quantized_error_w.restype = ctypes.c_double
+        L.ora_single_color_m.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32, vp, vp,
+                                         vp, ctypes.POINTER(ctypes.c_int)]
+        L.ora_single_color_m.restype = ctypes.c_double
+        L.ora_fit_rgb_m.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_uint32, ctypes.POINTER(FitTrace), vp, vp, vp,
+                                    ctypes.POINTER(ctypes.c_int)]
+        L.ora_fit_rgb_m.restype = ctypes.c_double
+        L.ora_fit_alpha_m.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_uint32, ctypes.POINTER(FitTrace), vp, vp, vp,
+                                      vp]
+        L.ora_fit_alpha_m.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -141,15 +158,16 @@ def _p(a):
 
 
 def bc7_compress(img, quality=0, seed=0, rng=RNG_COUNTER, lcg_seed=0, block_modes=0xFF,
-                 rect=None, wm_counter=0xFFFFFFFF, out=None, q0_guard=False):
+                 rect=None, wm_counter=0xFFFFFFFF, out=None, q0_guard=False, metric=0):
     """BPTCC::Compress over an image or sub-rectangle.  Returns (bytes, wm).
-    q0_guard adds the quality guard of the GPU encoder (not in the reference)."""
+    q0_guard adds the quality guard of the GPU encoder (not in the reference);
+    metric is m_ErrorMetric (0 uniform, 1 nonuniform)."""
     img = np.ascontiguousarray(img, dtype=np.uint32)
     h, w = img.shape
     if out is None:
         out = np.zeros((h // 4) * (w // 4) * 16, dtype=np.uint8)
     lcg = ctypes.c_uint32(lcg_seed)
-    s = Settings(quality, block_modes, rng, ctypes.pointer(lcg), seed, int(q0_guard))
+    s = Settings(quality, block_modes, rng, ctypes.pointer(lcg), seed, int(q0_guard), metric)
     wm = ctypes.c_uint32(wm_counter)
     x0, y0, x1, y1 = rect if rect is not None else (0, 0, w, h)
     lib().ora_bc7_compress_job(_p(img), _p(out), w, h, x0, y0, x1, y1, ctypes.byref(s),
@@ -158,7 +176,7 @@ def bc7_compress(img, quality=0, seed=0, rng=RNG_COUNTER, lcg_seed=0, block_mode
 
 
 def bc7_compress_logged(img, quality=0, seed=0, rng=RNG_COUNTER, lcg_seed=0, block_modes=0xFF,
-                        rect=None, wm_counter=0xFFFFFFFF, q0_guard=False):
+                        rect=None, wm_counter=0xFFFFFFFF, q0_guard=False, metric=0):
     """BPTCC::CompressWithStats over an image or sub-rectangle.  Returns
     (bytes, wm, stats), stats a LOG_STATS_DTYPE array, one record per block
     processed."""
@@ -167,7 +185,7 @@ def bc7_compress_logged(img, quality=0, seed=0, rng=RNG_COUNTER, lcg_seed=0, blo
     out = np.zeros((h // 4) * (w // 4) * 16, dtype=np.uint8)
     stats = np.zeros((h // 4) * (w // 4), dtype=LOG_STATS_DTYPE)
     lcg = ctypes.c_uint32(lcg_seed)
-    s = Settings(quality, block_modes, rng, ctypes.pointer(lcg), seed, int(q0_guard))
+    s = Settings(quality, block_modes, rng, ctypes.pointer(lcg), seed, int(q0_guard), metric)
     wm = ctypes.c_uint32(wm_counter)
     x0, y0, x1, y1 = rect if rect is not None else (0, 0, w, h)
     lib().ora_bc7_compress_job_logged(_p(img), _p(out), w, h, x0, y0, x1, y1, ctypes.byref(s),
@@ -183,7 +201,8 @@ SHAPE_SELECTION_DTYPE = np.dtype([("num_shapes", "<u4"),
 
 
 def bc7_compress_selected(img, selections, quality=0, seed=0, rng=RNG_COUNTER, lcg_seed=0,
-                          block_modes=0xFF, rect=None, wm_counter=0xFFFFFFFF, q0_guard=False):
+                          block_modes=0xFF, rect=None, wm_counter=0xFFFFFFFF, q0_guard=False,
+                          metric=0):
     """BPTCC::Compress with a caller's shape selection per processed block.
     Returns (bytes, wm)."""
     img = np.ascontiguousarray(img, dtype=np.uint32)
@@ -191,7 +210,7 @@ def bc7_compress_selected(img, selections, quality=0, seed=0, rng=RNG_COUNTER, l
     h, w = img.shape
     out = np.zeros((h // 4) * (w // 4) * 16, dtype=np.uint8)
     lcg = ctypes.c_uint32(lcg_seed)
-    s = Settings(quality, block_modes, rng, ctypes.pointer(lcg), seed, int(q0_guard))
+    s = Settings(quality, block_modes, rng, ctypes.pointer(lcg), seed, int(q0_guard), metric)
     wm = ctypes.c_uint32(wm_counter)
     x0, y0, x1, y1 = rect if rect is not None else (0, 0, w, h)
     lib().ora_bc7_compress_job_selected(_p(img), _p(out), w, h, x0, y0, x1, y1, ctypes.byref(s),
@@ -220,11 +239,11 @@ def format_block_stats(rec, block_index):
 
 
 def bc7_compress_block(px16, quality=0, seed=0, rng=RNG_COUNTER, lcg_seed=0, wm_index=0,
-                       block_id=0, block_modes=0xFF, q0_guard=False):
+                       block_id=0, block_modes=0xFF, q0_guard=False, metric=0):
     px = np.ascontiguousarray(px16, dtype=np.uint32)
     out = np.zeros(16, dtype=np.uint8)
     lcg = ctypes.c_uint32(lcg_seed)
-    s = Settings(quality, block_modes, rng, ctypes.pointer(lcg), seed, int(q0_guard))
+    s = Settings(quality, block_modes, rng, ctypes.pointer(lcg), seed, int(q0_guard), metric)
     st = BlockStats()
     lib().ora_bc7_compress_block(_p(px), _p(out), ctypes.byref(s), wm_index, block_id,
                                  ctypes.byref(st))
@@ -429,7 +448,7 @@ def clamp_to_grid(mode, p1, p2):
     return a, b, pbc.value
 
 
-def fit_rgb(px, mode, shape=0, subset=0, idx_mode=0):
+def fit_rgb(px, mode, shape=0, subset=0, idx_mode=0, metric=0):
     """One RGB cluster fit at quality 0: (error, p1, p2, indices, combo, trace)."""
     px = np.ascontiguousarray(px, dtype=np.uint32)
     tr = FitTrace()
@@ -437,12 +456,12 @@ def fit_rgb(px, mode, shape=0, subset=0, idx_mode=0):
     p2 = np.zeros(4, np.float32)
     idx = np.zeros(16, np.uint8)
     pbc = ctypes.c_int(0)
-    e = lib().ora_fit_rgb(_p(px), mode, shape, subset, idx_mode, ctypes.byref(tr), _p(p1),
-                          _p(p2), _p(idx), ctypes.byref(pbc))
+    e = lib().ora_fit_rgb_m(_p(px), mode, shape, subset, idx_mode, metric, ctypes.byref(tr),
+                            _p(p1), _p(p2), _p(idx), ctypes.byref(pbc))
     return e, p1, p2, idx, pbc.value, tr
 
 
-def fit_alpha(px, mode, rot, idx_mode=0):
+def fit_alpha(px, mode, rot, idx_mode=0, metric=0):
     """One rotation-mode fit at quality 0: (error, p1, p2, indices, alpha
     indices, trace of the rotated RGB fit)."""
     px = np.ascontiguousarray(px, dtype=np.uint32)
@@ -451,9 +470,37 @@ def fit_alpha(px, mode, rot, idx_mode=0):
     p2 = np.zeros(4, np.float32)
     idx = np.zeros(16, np.uint8)
     aidx = np.zeros(16, np.uint8)
-    e = lib().ora_fit_alpha(_p(px), mode, rot, idx_mode, ctypes.byref(tr), _p(p1), _p(p2),
-                            _p(idx), _p(aidx))
+    e = lib().ora_fit_alpha_m(_p(px), mode, rot, idx_mode, metric, ctypes.byref(tr), _p(p1),
+                              _p(p2), _p(idx), _p(aidx))
     return e, p1, p2, idx, aidx, tr
+
+
+def quantized_error(px, p1, p2, nbuckets, mask=0xFFFFFFFF, weights=(1.0, 1.0, 1.0, 1.0),
+                    sel=None, pbits=None):
+    """RGBACluster::QuantizedError over the pixels sel (all 16 by default)
+    with per-channel weights: (error, indices in cluster order)."""
+    px = np.ascontiguousarray(px, dtype=np.uint32)
+    f1 = np.ascontiguousarray(p1, dtype=np.float32)
+    f2 = np.ascontiguousarray(p2, dtype=np.float32)
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    idx = np.zeros(16, np.uint8)
+    n = 16
+    sp = None
+    if sel is not None:
+        sa = np.ascontiguousarray(sel, dtype=np.uint8)
+        n = len(sa)
+        sp = _p(sa)
+    pb = _p(np.ascontiguousarray(pbits, dtype=np.int32)) if pbits is not None else None
+    e = lib().ora_quantized_error_w(_p(px), sp, n, _p(f1), _p(f2), nbuckets, mask, pb, _p(w),
+                                    _p(idx))
+    return e, idx[:n].copy()
+
+
+def metric_weights(metric):
+    """kErrorMetrics (Compressor.cpp:205-208) as float32."""
+    if metric == 1:
+        return np.sqrt(np.array([0.3, 0.56, 0.11], np.float32)).tolist() + [1.0]
+    return [1.0, 1.0, 1.0, 1.0]
 
 
 def bc7_pack(mode, shape, rot, idx_mode, ep, pbc, idx, aidx):
