@@ -21,6 +21,9 @@ eCompressionFormat_DXT1 = 0
 eCompressionFormat_DXT5 = 1
 eCompressionFormat_ETC1 = 2
 eCompressionFormat_BPTC = 3
+# BPTCC::ErrorMetric (BPTCCompressor.h:73-77)
+eErrorMetric_Uniform = 0
+eErrorMetric_Nonuniform = 1
 # or'ed into a BPTC format for decoding: the BC7 specification's decode
 # (FH_DECODE_SPEC) instead of the reference decoder's
 DECODE_SPEC = 0x100
@@ -226,7 +229,7 @@ def compressed_size(fmt, width, height):
 
 
 def _job(fmt, inp, out, out_size, width, height, rect, quality, seed, block_modes,
-         wm_counter, q0_guard=True, ranker=None, topk=4, threshold=0.6):
+         wm_counter, q0_guard=True, ranker=None, topk=4, threshold=0.6, metric=0):
     x0, y0, x1, y1 = rect if rect is not None else (0, 0, width, height)
     j = FhJob()
     j.format = fmt
@@ -237,7 +240,7 @@ def _job(fmt, inp, out, out_size, width, height, rect, quality, seed, block_mode
     j.x_start, j.y_start, j.x_end, j.y_end = x0, y0, x1, y1
     j.quality = quality
     j.block_modes = block_modes
-    j.error_metric = 0
+    j.error_metric = metric
     j.seed = seed
     j.flags = 0 if q0_guard else FH_FLAG_NO_Q0_GUARD
     if ranker is not None:
@@ -253,7 +256,7 @@ def _job(fmt, inp, out, out_size, width, height, rect, quality, seed, block_mode
 
 def compress(img, fmt=eCompressionFormat_BPTC, quality=0, seed=0, block_modes=0xFF,
              rect=None, out=None, wm_counter=None, q0_guard=True, ranker=None, topk=4,
-             threshold=0.6):
+             threshold=0.6, metric=eErrorMetric_Uniform):
     """Encode a (h, w) uint32 RGBA8 image (R in the low byte) on the GPU.
 
     Mirrors CompressImageData: returns the compressed bytes (uint8 array of
@@ -263,14 +266,15 @@ def compress(img, fmt=eCompressionFormat_BPTC, quality=0, seed=0, block_modes=0x
     process-wide counter is used, like the reference.  `q0_guard=False`
     drops the BPTC quality guard (FH_FLAG_NO_Q0_GUARD).  With a `Ranker`, BPTC
     runs the learned candidate pruning (FH_FLAG_LEARNED) with `topk` and
-    `threshold`."""
+    `threshold`.  `metric` is BPTCC's m_ErrorMetric (eErrorMetric_Uniform or
+    eErrorMetric_Nonuniform)."""
     img = np.ascontiguousarray(img, dtype=np.uint32)
     h, w = img.shape
     size = compressed_size(fmt, w, h)
     if out is None:
         out = np.zeros(size, dtype=np.uint8)
     j = _job(fmt, img.ctypes.data, out.ctypes.data, out.nbytes, w, h, rect, quality, seed,
-             block_modes, wm_counter, q0_guard, ranker, topk, threshold)
+             block_modes, wm_counter, q0_guard, ranker, topk, threshold, metric)
     _check(lib().fh_compress(ctypes.byref(j)))
     return out
 
@@ -303,7 +307,7 @@ def compress_with_stats_device(in_ptr, out_ptr, out_size, stats_ptr, stats_count
 
 
 def compress_with_selection(img, selections, quality=0, seed=0, block_modes=0xFF, rect=None,
-                            wm_counter=None, q0_guard=True):
+                            wm_counter=None, q0_guard=True, metric=0):
     """BPTCC::Compress with a caller's shape selection function
     (fh_compress_with_selection): `selections` is a SHAPE_SELECTION_DTYPE
     array, one entry per block the job processes, in processing order."""
@@ -312,7 +316,7 @@ def compress_with_selection(img, selections, quality=0, seed=0, block_modes=0xFF
     sels = np.ascontiguousarray(selections, dtype=SHAPE_SELECTION_DTYPE)
     out = np.zeros(compressed_size(eCompressionFormat_BPTC, w, h), dtype=np.uint8)
     j = _job(eCompressionFormat_BPTC, img.ctypes.data, out.ctypes.data, out.nbytes, w, h, rect,
-             quality, seed, block_modes, wm_counter, q0_guard, None, 4, 0.6)
+             quality, seed, block_modes, wm_counter, q0_guard, None, 4, 0.6, metric)
     _check(lib().fh_compress_with_selection(ctypes.byref(j), sels.ctypes.data, sels.size))
     return out
 
@@ -343,11 +347,12 @@ def stats_format(stats, block_index):
 
 def compress_device(in_ptr, out_ptr, out_size, width, height, fmt=eCompressionFormat_BPTC,
                     quality=0, seed=0, block_modes=0xFF, stream=0, wm_counter=None,
-                    q0_guard=True, ranker=None, topk=4, threshold=0.6):
+                    q0_guard=True, ranker=None, topk=4, threshold=0.6,
+                    metric=eErrorMetric_Uniform):
     """Encode device-resident data (raw device pointers, e.g. torch
     tensor.data_ptr()) asynchronously on `stream` (a hipStream_t as int)."""
     j = _job(fmt, in_ptr, out_ptr, out_size, width, height, None, quality, seed, block_modes,
-             wm_counter, q0_guard, ranker, topk, threshold)
+             wm_counter, q0_guard, ranker, topk, threshold, metric)
     _check(lib().fh_compress_device(ctypes.byref(j), ctypes.c_void_p(stream)))
 
 
@@ -444,13 +449,15 @@ class Batch:
     settings as in compress()."""
 
     def __init__(self, surfaces, quality=0, seed=0, block_modes=0xFF, q0_guard=True,
-                 ranker=None, topk=4, threshold=0.6, wm_counter_ptr=0):
+                 ranker=None, topk=4, threshold=0.6, wm_counter_ptr=0,
+                 metric=eErrorMetric_Uniform):
         arr = (FhSurface * max(1, len(surfaces)))()
         for i, (fmt, inp, out, w, h) in enumerate(surfaces):
             arr[i] = FhSurface(fmt, inp, out, w, h)
         opt = FhBatchOptions()
         opt.quality = quality
         opt.block_modes = block_modes
+        opt.error_metric = metric
         opt.seed = seed
         opt.flags = 0 if q0_guard else FH_FLAG_NO_Q0_GUARD
         if ranker is not None:

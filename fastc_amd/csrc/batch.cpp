@@ -10,6 +10,7 @@
 // (persistent workgroups per CU, annealing steps between refills, workgroup
 // size of the DXT/ETC1 kernels) by timing the plan itself.
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <string.h>
@@ -121,6 +122,13 @@ hipError_t enqueue(fh_batch *b, hipStream_t s) {
     p.anneal_unroll = b->tune.anneal_unroll;
     p.stats = nullptr;
     p.selections = nullptr;
+    // kErrorMetrics (Compressor.cpp:205-208)
+    const bool nu = b->opt.error_metric == 1;
+    p.nonuniform = nu ? 1u : 0u;
+    p.metric[0] = nu ? sqrtf(0.3f) : 1.0f;
+    p.metric[1] = nu ? sqrtf(0.56f) : 1.0f;
+    p.metric[2] = nu ? sqrtf(0.11f) : 1.0f;
+    p.metric[3] = 1.0f;
     Bc7Scratch sc;
     const size_t prefix_bytes = ((size_t)b->bc7_chunk + 255) & ~(size_t)255;
     sc.local_prefix = b->bc7_scan;
@@ -148,8 +156,11 @@ fh_status fh_batch_create(const fh_surface *surfaces, uint32_t count,
   *out = nullptr;
   fh_batch_options opt = {};
   if (options) opt = *options;
-  if (opt.error_metric != 0)
-    return bfail(FH_ERR_UNSUPPORTED, "only the uniform error metric is implemented");
+  if (opt.error_metric > 1) return bfail(FH_ERR_INVALID_ARGUMENT, "unknown error metric");
+  if (opt.error_metric == 1 && (opt.flags & FH_FLAG_LEARNED))
+    return bfail(FH_ERR_UNSUPPORTED,
+                 "the learned search ranks for the uniform metric: the nonuniform metric "
+                 "runs the reference search only");
   if ((opt.flags & FH_FLAG_LEARNED) && !opt.ranker)
     return bfail(FH_ERR_INVALID_ARGUMENT, "FH_FLAG_LEARNED needs a ranker");
   if (opt.flags & FH_FLAG_CALLER_WATERMARK)
@@ -212,7 +223,9 @@ fh_status fh_batch_create(const fh_surface *surfaces, uint32_t count,
     if (e == hipSuccess)
       e = alloc(b, reinterpret_cast<void **>(&b->bc7_scan),
                 prefix_bytes + ((size_t)b->bc7_chunk + 255) / 256 * 4 + 256);
-    if (e == hipSuccess) e = alloc(b, &b->bc7_work, bc7_work_bytes(b->bc7_chunk, learned) + 256);
+    if (e == hipSuccess)
+      e = alloc(b, &b->bc7_work,
+                bc7_work_bytes(b->bc7_chunk, learned, opt.error_metric == 1) + 256);
   }
   if (e == hipSuccess) e = dxt_prepare();
   if (e == hipSuccess) e = etc1_prepare();

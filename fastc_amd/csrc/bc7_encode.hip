@@ -295,6 +295,118 @@ __device__ __forceinline__ uint32_t qerr(const uint32_t px[16], uint32_t smask, 
   return total;
 }
 
+// ---------------------------------------------------------------------------
+// The nonuniform error metric (m_ErrorMetric = eErrorMetric_Nonuniform,
+// BPTCCompressor.h:73-77; weights kErrorMetrics, Compressor.cpp:205-208).
+// Each channel's absolute difference is multiplied by its weight before
+// squaring, and the four squares are summed in float in channel order from
+// zero (VectorBase::Dot, VectorBase.h:96-101).  The errors are then floats
+// whose values depend on the order of the additions, so every sum below runs
+// in the reference's order: channels, then the pixels of a subset in pixel
+// order.  A float error is kept as its bit pattern in the 32-bit error fields
+// the uniform metric fills with integers: for errors >= 0 the bit patterns
+// order as the values do, so comparisons and the "> 0" tests are shared.
+// ---------------------------------------------------------------------------
+struct Weights {
+  float w[4];
+};
+
+__device__ __forceinline__ float werr(int d0, int d1, int d2, int d3, const Weights &m) {
+  const float e0 = (float)abs(d0) * m.w[0];
+  const float e1 = (float)abs(d1) * m.w[1];
+  const float e2 = (float)abs(d2) * m.w[2];
+  const float e3 = (float)abs(d3) * m.w[3];
+  return ((e0 * e0 + e1 * e1) + e2 * e2) + e3 * e3;
+}
+
+// The weights a rotation mode sees (GetErrorMetric, CompressionMode.h:196-205):
+// rotation r swaps channel r-1's weight with alpha's.
+__device__ __forceinline__ Weights rotate_weights(const Weights &m, int rot) {
+  Weights r = m;
+  if (rot >= 1 && rot <= 3) {
+    r.w[rot - 1] = m.w[3];
+    r.w[3] = m.w[rot - 1];
+  }
+  return r;
+}
+
+__device__ __forceinline__ Weights params_weights(const Bc7Params &prm) {
+  Weights m;
+#pragma unroll
+  for (int k = 0; k < 4; k++) m.w[k] = prm.metric[k];
+  return m;
+}
+
+// QuantizedError under per-channel weights m (qerr's float counterpart):
+// per pixel the weighted error of each of the two neighbouring indices, the
+// second kept only when strictly better, summed in float in pixel order.
+template <int UNROLL>
+__device__ __forceinline__ float qerr_nu(const uint32_t px[16], uint32_t smask, int ptmode,
+                                         uint32_t qp1, uint32_t qp2, int nb, const Weights &m,
+                                         uint64_t &idx) {
+  int a[4], b[4], dir[4];
+  int lsq = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    a[k] = (int)chan(qp1, k);
+    b[k] = (int)chan(qp2, k);
+    dir[k] = b[k] - a[k];
+    lsq += dir[k] * dir[k];
+  }
+  idx = 0;
+  float total = 0.0f;
+  if (lsq == 0) {
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      if (!((smask >> i) & 1)) continue;
+      total += werr((int)chan(px[i], 0) - a[0], (int)chan(px[i], 1) - a[1],
+                    (int)chan(px[i], 2) - a[2], (int)chan(px[i], 3) - a[3], m);
+    }
+    return total;
+  }
+  const int prec = nb == 4 ? 2 : (nb == 8 ? 3 : 4);
+  const float flsq = (float)lsq;
+  const float kf = (float)(nb - 1);
+#pragma unroll UNROLL
+  for (int i = 0; i < 16; i++) {
+    if (!((smask >> i) & 1)) continue;
+    const uint32_t pt = point_of(px[i], ptmode);
+    int d = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) d += ((int)chan(pt, k) - a[k]) * dir[k];
+    const float pct = (float)d / flsq;
+    const float t = pct * kf;
+    int j1 = (int)floorf(t);
+    int j2 = (int)ceilf(t);
+    j1 = j1 < 0 ? 0 : j1;
+    j1 = j1 > nb - 1 ? nb - 1 : j1;
+    j2 = j2 > nb - 1 ? nb - 1 : j2;
+    const uint32_t pix = px[i];
+    int dd[4];
+    {
+      const int w1 = kW[prec - 1][j1], w0 = 64 - w1;
+#pragma unroll
+      for (int k = 0; k < 4; k++) dd[k] = (int)chan(pix, k) - ((a[k] * w0 + b[k] * w1 + 32) >> 6);
+    }
+    const float e1 = werr(dd[0], dd[1], dd[2], dd[3], m);
+    int best = j1;
+    float me = e1;
+    if (j2 > j1) {
+      const int w1 = kW[prec - 1][j2], w0 = 64 - w1;
+#pragma unroll
+      for (int k = 0; k < 4; k++) dd[k] = (int)chan(pix, k) - ((a[k] * w0 + b[k] * w1 + 32) >> 6);
+      const float e2 = werr(dd[0], dd[1], dd[2], dd[3], m);
+      if (e2 < e1) {
+        best = j2;
+        me = e2;
+      }
+    }
+    total += me;
+    idx |= (uint64_t)best << (4 * i);
+  }
+  return total;
+}
+
 // Per-lane description of one subset fit.
 struct Fit {
   int mode;
@@ -326,13 +438,18 @@ __device__ __forceinline__ uint32_t qmask_of(int color_prec, int alpha_prec) {
 }
 
 // CompressSingleColor (Compressor.cpp:252-353) through the precomputed
-// per-value endpoint table.  Returns the per-pixel error.
+// per-value endpoint table.  Returns the per-pixel error; under the
+// nonuniform metric (NU) its float bit pattern, weighted by the unrotated
+// weights mu (Compressor.cpp:334-339) and summed in channel order.
+template <bool NU = false>
 __device__ __forceinline__ uint32_t single_color(const Fit &f, const uint32_t *sctab, uint32_t pixel,
-                                 uint32_t &p1, uint32_t &p2, int &pbc) {
-  uint32_t best = 0xFFFFFFFFu;
+                                 uint32_t &p1, uint32_t &p2, int &pbc,
+                                 const Weights *mu = nullptr) {
+  uint32_t best = NU ? __float_as_uint(FLT_MAX) : 0xFFFFFFFFu;
   const int bpi = f.nbits_idx - 1;
   for (int pbi = 0; pbi < f.npbc; pbi++) {
     uint32_t err = 0, q1 = 0, q2 = 0;
+    float ferr = 0.0f;
 #pragma unroll
     for (int ci = 0; ci < 4; ci++) {
       const uint32_t val = chan(pixel, ci);
@@ -350,10 +467,16 @@ __device__ __forceinline__ uint32_t single_color(const Fit &f, const uint32_t *s
         hi = (e >> 8) & 0xFFu;
         dist = (e >> 16) & 0xFFu;
       }
-      err += dist * dist;
+      if constexpr (NU) {
+        const float e = (float)dist * mu->w[ci];
+        ferr += e * e;
+      } else {
+        err += dist * dist;
+      }
       q1 |= lo << (8 * ci);
       q2 |= hi << (8 * ci);
     }
+    if constexpr (NU) err = __float_as_uint(ferr);
     if (err < best) {
       best = err;
       pbc = pbi;
@@ -587,6 +710,7 @@ __device__ __forceinline__ void set_fit(Fit &f, int mode, int rot, int im) {
 // when no annealing follows (single colour, quality 0, zero error).
 struct SubStart {
   uint32_t err;  // QuantizedError of the start endpoints (or the final error)
+  bool sc;       // CompressSingleColor's result: the subset's error is n * err
   uint32_t p1, p2;  // endpoint state (integers): annealing start, or final endpoints
   uint64_t idx;     // indices at pixel positions of this subset
   int pbc;
@@ -612,10 +736,13 @@ __device__ __forceinline__ int nearest_centroid(const float c[NB][4], const floa
   return minb;
 }
 
-template <int NB>
+// NU: the nonuniform metric, mu its unrotated weights (single colour), mr
+// the weights of this candidate's rotation (QuantizedError).
+template <int NB, bool NU = false>
 __device__ __forceinline__ void kmeans_setup(const Fit &f, uint32_t q, const uint32_t *sctab,
                              const uint32_t px[16], uint32_t smask, int ptmode, int n,
-                             const float avg[4], SubStart &out) {
+                             const float avg[4], SubStart &out,
+                             const Weights *mu = nullptr, const Weights *mr = nullptr) {
   float axis[4];
   principal_axis(px, smask, ptmode, n, avg, axis);
   float mindp = FLT_MAX, maxdp = -FLT_MAX;
@@ -742,9 +869,10 @@ __device__ __forceinline__ void kmeans_setup(const Fit &f, uint32_t q, const uin
 #pragma unroll
         for (int k = 0; k < 4; k++) cl[k] = c[j][k];
       }
-    const uint32_t e =
-        single_color(f, sctab, to_pixel_f(cl, 0xFFFFFFFFu, -1), out.p1, out.p2, out.pbc);
-    out.err = (uint32_t)n * e;
+    const uint32_t e = single_color<NU>(f, sctab, to_pixel_f(cl, 0xFFFFFFFFu, -1), out.p1,
+                                        out.p2, out.pbc, mu);
+    out.err = NU ? e : (uint32_t)n * e;
+    out.sc = true;
     out.idx = ones;
     out.anneal = false;
     return;
@@ -812,7 +940,9 @@ __device__ __forceinline__ void kmeans_setup(const Fit &f, uint32_t q, const uin
   const int nb = 1 << f.nbits_idx;
   const uint32_t q1 = to_pixel_int(g1, f.qmask, pbc_bit(f.pbit_type, pbc, 0));
   const uint32_t q2 = to_pixel_int(g2, f.qmask, pbc_bit(f.pbit_type, pbc, 1));
-  out.err = qerr<1>(px, smask, ptmode, q1, q2, nb, out.idx);
+  if constexpr (NU) out.err = __float_as_uint(qerr_nu<1>(px, smask, ptmode, q1, q2, nb, *mr, out.idx));
+  else out.err = qerr<1>(px, smask, ptmode, q1, q2, nb, out.idx);
+  out.sc = false;
   out.pbc = pbc;
   if (f.pbit_type != PB_NONE) {
     out.p1 = q1;
@@ -825,17 +955,20 @@ __device__ __forceinline__ void kmeans_setup(const Fit &f, uint32_t q, const uin
 }
 
 // CompressCluster, RGB variant (Compressor.cpp:921-1094) up to the annealing.
+template <bool NU = false>
 __device__ __forceinline__ void subset_setup(const Fit &f, uint32_t q, const uint32_t *sctab,
                              const uint32_t px[16], uint32_t smask, int ptmode, int n,
-                             const float avg[4], bool all_same, SubStart &out) {
+                             const float avg[4], bool all_same, SubStart &out,
+                             const Weights *mu = nullptr, const Weights *mr = nullptr) {
   if (all_same) {
     const int first = __ffs(smask) - 1;
     uint32_t pt = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++)
       if (i == first) pt = point_of(px[i], ptmode);
-    const uint32_t e = single_color(f, sctab, pt, out.p1, out.p2, out.pbc);
-    out.err = (uint32_t)n * e;
+    const uint32_t e = single_color<NU>(f, sctab, pt, out.p1, out.p2, out.pbc, mu);
+    out.err = NU ? e : (uint32_t)n * e;
+    out.sc = true;
     out.idx = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++)
@@ -843,17 +976,21 @@ __device__ __forceinline__ void subset_setup(const Fit &f, uint32_t q, const uin
     out.anneal = false;
     return;
   }
-  if (f.nbits_idx == 2) kmeans_setup<4>(f, q, sctab, px, smask, ptmode, n, avg, out);
-  else if (f.nbits_idx == 3) kmeans_setup<8>(f, q, sctab, px, smask, ptmode, n, avg, out);
-  else kmeans_setup<16>(f, q, sctab, px, smask, ptmode, n, avg, out);
+  if (f.nbits_idx == 2) kmeans_setup<4, NU>(f, q, sctab, px, smask, ptmode, n, avg, out, mu, mr);
+  else if (f.nbits_idx == 3) kmeans_setup<8, NU>(f, q, sctab, px, smask, ptmode, n, avg, out, mu, mr);
+  else kmeans_setup<16, NU>(f, q, sctab, px, smask, ptmode, n, avg, out, mu, mr);
 }
 
 // The alpha channel of the rotation modes (Compressor.cpp:697-911): a 1-D
 // k-means and least-squares fit (or the table lookup for a constant alpha).
 // Returns the alpha error; a1q/a2q are the alpha bytes Pack will write
-// (rounded, then quantised to the mode's alpha precision).
+// (rounded, then quantised to the mode's alpha precision).  NU: the
+// nonuniform metric, whose alpha term (a double sum of float per-pixel
+// errors, weighted by the rotated alpha weight aw, Compressor.cpp:712) goes
+// to *nu_err.
+template <bool NU = false>
 __device__ __forceinline__ uint32_t alpha_fit(const Fit &f, const uint32_t px[16], uint64_t &aidx, uint32_t &a1q,
-                              uint32_t &a2q) {
+                              uint32_t &a2q, float aw = 1.0f, double *nu_err = nullptr) {
   float avals[16];
   float amin = FLT_MAX, amax = -FLT_MAX;
 #pragma unroll
@@ -867,6 +1004,7 @@ __device__ __forceinline__ uint32_t alpha_fit(const Fit &f, const uint32_t px[16
   const int nb = 1 << abits;
   float a1 = amin, a2 = amax;
   uint32_t alpha_err = 0;
+  double alpha_nu = 0.0;
   aidx = 0;
   if (a1 == a2) {
     const uint32_t a1be = (uint32_t)a1, a2be = (uint32_t)a2;
@@ -885,6 +1023,11 @@ __device__ __forceinline__ uint32_t alpha_fit(const Fit &f, const uint32_t px[16
       const uint32_t ip = (((uint32_t)a1 * w0) + ((uint32_t)a2 * w1) + 32) >> 6 & 0xFFu;
       const uint32_t d = a1be > ip ? a1be - ip : ip - a1be;
       alpha_err = 16 * d * d;
+      if constexpr (NU) {
+        float pe = aw * (float)d;
+        pe *= pe;
+        alpha_nu = (double)(16.0f * pe);
+      }
     }
   } else {
     float vals[8];
@@ -975,6 +1118,7 @@ __device__ __forceinline__ uint32_t alpha_fit(const Fit &f, const uint32_t px[16
     for (int i = 0; i < 16; i++) {
       const uint32_t val = (uint32_t)avals[i];
       uint32_t me = 0xFFFFFFFFu;
+      float mef = FLT_MAX;
       uint32_t bb = 0;
 #pragma unroll
       for (int j = 0; j < 8; j++) {
@@ -982,19 +1126,30 @@ __device__ __forceinline__ uint32_t alpha_fit(const Fit &f, const uint32_t px[16
         const int w1 = kW[abits - 1][j], w0 = 64 - w1;
         const uint32_t ip = ((a1b * w0) + (a2b * w1) + 32) >> 6 & 0xFFu;
         const uint32_t d = val > ip ? val - ip : ip - val;
-        const uint32_t e = d * d;
-        if (e < me) {
-          me = e;
-          bb = (uint32_t)j;
+        if constexpr (NU) {
+          float pe = aw * (float)d;
+          pe *= pe;
+          if (pe < mef) {
+            mef = pe;
+            bb = (uint32_t)j;
+          }
+        } else {
+          const uint32_t e = d * d;
+          if (e < me) {
+            me = e;
+            bb = (uint32_t)j;
+          }
         }
       }
-      alpha_err += me;
+      if constexpr (NU) alpha_nu += (double)mef;
+      else alpha_err += me;
       aidx |= (uint64_t)bb << (4 * i);
     }
   }
   const uint32_t am = chan(f.qmask, 3);
   a1q = quant_channel(round_half_up(a1) & 0xFFu, am, -1);
   a2q = quant_channel(round_half_up(a2) & 0xFFu, am, -1);
+  if constexpr (NU) *nu_err = alpha_nu;
   return alpha_err;
 }
 
@@ -1255,6 +1410,9 @@ struct Work {
   // learned search: astate[i].id for every state (read when the list is
   // built, without touching the 64-byte states); null in the reference search
   uint32_t *aid;
+  // nonuniform metric: each candidate's total error, the double the
+  // reference compares (per block x kMaxCands); null under the uniform metric
+  double *nu_total;
 };
 
 __device__ __forceinline__ void load_block(const BlockRange &r, uint32_t rel, uint32_t px[16]) {
@@ -1378,10 +1536,8 @@ __device__ __forceinline__ uint32_t interp_err(uint32_t p02, uint32_t p13, const
 // integer 0 <= k <= kf in {3, 7, 15} (checked exhaustively).  So the quotient
 // is taken in integers: an estimate from a reciprocal, corrected by one step
 // with the exact remainder.
-__device__ __forceinline__ uint32_t pixel_err(uint32_t pix, uint32_t pt, const SubConst &c,
-                                              int nb, float kf, WeightFn wf, int *best) {
-  (void)kf;
-  const uint32_t p02 = pix & 0x00FF00FFu, p13 = (pix >> 8) & 0x00FF00FFu;
+__device__ __forceinline__ void pixel_bounds(uint32_t pt, const SubConst &c, int nb, int &j1,
+                                             int &j2) {
   const uint32_t t02 = pt & 0x00FF00FFu, t13 = (pt >> 8) & 0x00FF00FFu;
   const int d = __builtin_amdgcn_sdot2(
       as_s16x2(t02) - as_s16x2(c.a02), as_s16x2(c.d02),
@@ -1394,13 +1550,50 @@ __device__ __forceinline__ uint32_t pixel_err(uint32_t pix, uint32_t pt, const S
   const bool lo = rem < 0, hi = rem >= c.l;
   q += lo ? -1 : (hi ? 1 : 0);
   rem += lo ? c.l : (hi ? -c.l : 0);
-  int j1 = q < 0 ? 0 : (q > nb - 1 ? nb - 1 : q);
-  int j2 = q + (rem != 0 ? 1 : 0);
+  j1 = q < 0 ? 0 : (q > nb - 1 ? nb - 1 : q);
+  j2 = q + (rem != 0 ? 1 : 0);
   j2 = j2 > nb - 1 ? nb - 1 : j2;
   j1 = zero ? 0 : j1;
   j2 = zero ? 0 : j2;
+}
+
+__device__ __forceinline__ uint32_t pixel_err(uint32_t pix, uint32_t pt, const SubConst &c,
+                                              int nb, float kf, WeightFn wf, int *best) {
+  (void)kf;
+  const uint32_t p02 = pix & 0x00FF00FFu, p13 = (pix >> 8) & 0x00FF00FFu;
+  int j1, j2;
+  pixel_bounds(pt, c, nb, j1, j2);
   const uint32_t e1 = interp_err(p02, p13, c, weight_of(wf, (uint32_t)j1));
   const uint32_t e2 = interp_err(p02, p13, c, weight_of(wf, (uint32_t)j2));
+  const bool second = j2 > j1 && e2 < e1;
+  *best = second ? j2 : j1;
+  return second ? e2 : e1;
+}
+
+// The nonuniform metric's pixel pass (the metric is described above qerr_nu).
+// interp_err under the weights (pair lanes: .x channels 0 / 1, .y 2 / 3).
+__device__ __forceinline__ float interp_err_nu(uint32_t p02, uint32_t p13, const SubConst &c,
+                                               uint32_t w1, const Weights &m) {
+  const s16x2 W = {(short)w1, (short)w1};
+  const s16x2 K = {32, 32};
+  const s16x2 S = {6, 6};
+  const s16x2 i02 = as_s16x2(c.a02) + ((as_s16x2(c.d02) * W + K) >> S);
+  const s16x2 i13 = as_s16x2(c.a13) + ((as_s16x2(c.d13) * W + K) >> S);
+  const s16x2 e02 = as_s16x2(p02) - i02;
+  const s16x2 e13 = as_s16x2(p13) - i13;
+  return werr(e02.x, e13.x, e02.y, e13.y, m);
+}
+
+// pixel_err under the weights: the same two neighbouring indices, the second
+// kept only when strictly better.
+__device__ __forceinline__ float pixel_err_nu(uint32_t pix, uint32_t pt, const SubConst &c,
+                                              int nb, WeightFn wf, const Weights &m,
+                                              int *best) {
+  const uint32_t p02 = pix & 0x00FF00FFu, p13 = (pix >> 8) & 0x00FF00FFu;
+  int j1, j2;
+  pixel_bounds(pt, c, nb, j1, j2);
+  const float e1 = interp_err_nu(p02, p13, c, weight_of(wf, (uint32_t)j1), m);
+  const float e2 = interp_err_nu(p02, p13, c, weight_of(wf, (uint32_t)j2), m);
   const bool second = j2 > j1 && e2 < e1;
   *best = second ? j2 : j1;
   return second ? e2 : e1;
@@ -1500,6 +1693,45 @@ __device__ __forceinline__ double shape_estimate(const uint32_t px[16], int shap
   return err;
 }
 
+// shape_estimate under the nonuniform metric's unrotated weights
+// (Compressor.cpp:1634, 1650): each subset's error is a float sum in pixel
+// order, added to 0.0001 in double.
+template <int NS, int NB>
+__device__ __forceinline__ double shape_estimate_nu(const uint32_t px[16], int shape,
+                                                    const Weights &m) {
+  const uint32_t m1 = NS == 2 ? (uint32_t)kShape2[shape] : (uint32_t)kShape3[shape][0];
+  const uint32_t m2 = NS == 3 ? (uint32_t)kShape3[shape][1] : 0u;
+  uint32_t lo[NS], hi[NS];
+  subset_corners<NS>(px, m1, m2, lo, hi);
+  SubConst c[NS];
+  bool flat[NS];
+#pragma unroll
+  for (int t = 0; t < NS; t++) {
+    c[t] = sub_const(lo[t], hi[t]);
+    flat[t] = lo[t] == hi[t];
+  }
+  const WeightFn wf = weight_fn(NB);
+  float e[NS];
+#pragma unroll
+  for (int t = 0; t < NS; t++) e[t] = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const uint32_t s = ((m1 >> i) & 1u) ? 1u + ((m2 >> i) & 1u) : 0u;
+    const SubConst k = NS == 2 ? pick_sub(s != 0, c[1], c[0])
+                               : pick_sub(s == 0, c[0], pick_sub(s == 1, c[1], c[NS - 1]));
+    int best;
+    const float me = pixel_err_nu(px[i], px[i], k, NB, wf, m, &best);
+#pragma unroll
+    for (int t = 0; t < NS; t++) e[t] += s == (uint32_t)t ? me : 0.0f;
+  }
+  double err = 0.0;
+#pragma unroll
+  for (int t = 0; t < NS; t++) {
+    if (!flat[t]) err += 0.0001 + (double)e[t];
+  }
+  return err;
+}
+
 // ---------------------------------------------------------------------------
 // Kernel 1: classification and BoxSelection (Compressor.cpp:1670-1750,
 // 1819-1860), lane i scoring shape i; the candidates go to per-type queues.
@@ -1508,6 +1740,7 @@ __device__ __forceinline__ double shape_estimate(const uint32_t px[16], int shap
 // block by a whole wave (lane i scores two- and three-subset shape i); the
 // block's candidates in CompressClusters' order go to cd.  Returns the
 // packed class and count.
+template <bool NU>
 __device__ uint32_t box_select_block(const uint32_t px[16], int lane, const Bc7Params &prm,
                                      uint16_t cd[kMaxBoxCands]) {
   bool solid = true, transparent = true, opaque = true;
@@ -1524,8 +1757,15 @@ __device__ uint32_t box_select_block(const uint32_t px[16], int lane, const Bc7P
   } else if (transparent) {
     sel = sel_pack(1, 0);
   } else {
-    const double err2 = shape_estimate<2, 8>(px, lane);
-    const double err3 = opaque ? shape_estimate<3, 4>(px, lane) : 0.0;
+    double err2, err3;
+    if constexpr (NU) {
+      const Weights m = params_weights(prm);
+      err2 = shape_estimate_nu<2, 8>(px, lane, m);
+      err3 = opaque ? shape_estimate_nu<3, 4>(px, lane, m) : 0.0;
+    } else {
+      err2 = shape_estimate<2, 8>(px, lane);
+      err3 = opaque ? shape_estimate<3, 4>(px, lane) : 0.0;
+    }
     double v2 = err2, v3 = err3;
     int i2 = lane, i3 = lane;
 #pragma unroll
@@ -1852,7 +2092,7 @@ __device__ uint32_t given_select_block(const uint32_t px[16], const Bc7Params &p
 // SEL: which selection (BoxSelection, the statistics scan, the caller's).
 enum { kSelBox = 0, kSelStats = 1, kSelGiven = 2 };
 
-template <int SEL>
+template <int SEL, bool NU = false>
 __global__ void __launch_bounds__(256) select_kernel(BlockRange r, Bc7Params prm, Work w,
                                                      const uint32_t *list,
                                                      const uint32_t *list_count) {
@@ -1887,7 +2127,7 @@ __global__ void __launch_bounds__(256) select_kernel(BlockRange r, Bc7Params prm
     uint16_t cd[kSlots];
     uint32_t sel = SEL == kSelStats ? stats_select_block(px, lane, prm, cd, prm.stats + rel)
                  : SEL == kSelGiven ? given_select_block(px, prm, prm.selections[rel], cd)
-                                    : box_select_block(px, lane, prm, cd);
+                                    : box_select_block<NU>(px, lane, prm, cd);
     // the learned search's low-confidence blocks (the fallback list) run the
     // full search
     if (list && sel_cls(sel) == 2) sel |= kSelFull;
@@ -2194,10 +2434,18 @@ __global__ void __launch_bounds__(256, 2) rank_select_kernel(BlockRange r, Bc7Pa
 // state.  Returns the mask of subsets that annealing continues (quality > 0).
 // MODE >= 0 specialises the fit for that mode at compile time (k.mode must
 // equal it); -1 reads the mode from k.
-template <int MODE>
+//
+// NU: the nonuniform metric with unrotated weights *mu.  The subset errors
+// are float bit patterns (a single-colour subset's is the per-pixel error,
+// flagged in st.pbc bit 8 + subset, its error being n times it), the alpha
+// term a double in st.extra_err / st.pad, and the candidate's total, the
+// double sum the reference compares (Compressor.cpp:1300-1371), in *tot_nu.
+template <int MODE, bool NU = false>
 __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t px[16],
                                                   uint32_t quality, const uint32_t *sctab,
-                                                  CandRec &rec, AState &st, uint32_t &total) {
+                                                  CandRec &rec, AState &st, uint32_t &total,
+                                                  const Weights *mu = nullptr,
+                                                  double *tot_nu = nullptr) {
   Fit f;
   const ModeAttr attr = MODE >= 0 ? kModeTab[MODE < 0 ? 0 : MODE] : kModes[k.mode];
   const int nsub = attr.subsets;
@@ -2212,13 +2460,17 @@ __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t 
   rec.aidx_lo = rec.aidx_hi = 0;
   st.extra_err = 0;
   st.pad = 0;
-  uint32_t active = 0, pbcs = 0;
+  uint32_t active = 0, pbcs = 0, scf = 0;
   total = 0;
+  double tnu = 0.0;
   uint64_t cidx = 0;
   if (attr.has_rot) {
     uint64_t aidx;
     uint32_t a1q, a2q;
-    const uint32_t aerr = alpha_fit(f, px, aidx, a1q, a2q);
+    Weights mr;
+    double aerr_nu = 0.0;
+    if constexpr (NU) mr = rotate_weights(*mu, rot);
+    const uint32_t aerr = alpha_fit<NU>(f, px, aidx, a1q, a2q, NU ? mr.w[3] : 1.0f, &aerr_nu);
     // the RGB fit sees the stale statistics of the unrotated cluster
     float avg[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -2228,7 +2480,7 @@ __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t 
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) avg[kk] = avg[kk] / 16.0f;
     SubStart s0;
-    subset_setup(f, quality, sctab, px, 0xFFFFu, rot + 1, 16, avg, false, s0);
+    subset_setup<NU>(f, quality, sctab, px, 0xFFFFu, rot + 1, 16, avg, false, s0, mu, &mr);
     rec.aidx_lo = (uint32_t)aidx;
     rec.aidx_hi = (uint32_t)(aidx >> 32);
     // Pack writes the RGB of the fit and the fitted alpha, no p-bit
@@ -2238,6 +2490,16 @@ __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t 
     st.p2[0] = s0.p2;
     st.err[0] = s0.err;
     st.extra_err = aerr;
+    if constexpr (NU) {
+      // rgbError + alphaError (Compressor.cpp:918)
+      const double rgb = s0.sc ? 16.0 * (double)__uint_as_float(s0.err)
+                               : (double)__uint_as_float(s0.err);
+      tnu = rgb + aerr_nu;
+      const uint64_t ab = (uint64_t)__double_as_longlong(aerr_nu);
+      st.extra_err = (uint32_t)ab;
+      st.pad = (uint32_t)(ab >> 32);
+      scf = s0.sc ? 1u : 0u;
+    }
     cidx = s0.idx;
     total = s0.err + aerr;
     active = s0.anneal ? 1u : 0u;
@@ -2264,7 +2526,12 @@ __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t 
 #pragma unroll
       for (int kk = 0; kk < 4; kk++) avg[kk] = avg[kk] / (float)n;
       SubStart ss;
-      subset_setup(f, quality, sctab, px, smask, 0, n, avg, same, ss);
+      subset_setup<NU>(f, quality, sctab, px, smask, 0, n, avg, same, ss, mu, mu);
+      if constexpr (NU) {
+        tnu += ss.sc ? (double)n * (double)__uint_as_float(ss.err)
+                     : (double)__uint_as_float(ss.err);
+        scf |= (ss.sc ? 1u : 0u) << s;
+      }
       const int pb0 = f.pbit_type == PB_NONE ? -1 : pbc_bit(f.pbit_type, ss.pbc, 0);
       const int pb1 = f.pbit_type == PB_NONE ? -1 : pbc_bit(f.pbit_type, ss.pbc, 1);
       const uint32_t e1 = to_pixel_int(ss.p1, f.qmask, pb0);
@@ -2290,6 +2557,10 @@ __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t 
   rec.meta = pbcs | ((uint32_t)rot << 8) | ((uint32_t)im << 10);
   rec.err = total;  // the quality-0 result; annealing overwrites active ones
   st.pbc = pbcs;
+  if constexpr (NU) {
+    st.pbc |= scf << 8;
+    *tot_nu = tnu;
+  }
   st.idx_lo = (uint32_t)cidx;
   st.idx_hi = (uint32_t)(cidx >> 32);
   return active;
@@ -2299,7 +2570,7 @@ __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t 
 // each mode gets its own register allocation.
 constexpr int kPxStride = 17;  // odd stride: lanes reading texel i hit distinct banks
 
-template <int MODE>
+template <int MODE, bool NU = false>
 __global__ void __launch_bounds__(64) setup_kernel(BlockRange r, Bc7Params prm, Work w,
                                                    const uint32_t *sctab) {
   const int lane = threadIdx.x;
@@ -2333,9 +2604,14 @@ __global__ void __launch_bounds__(64) setup_kernel(BlockRange r, Bc7Params prm, 
       CandRec rec;
       AState st;
       uint32_t total = 0;
-      const uint32_t active = fit_candidate<MODE>(k, px, prm.quality, sctab, rec, st, total);
+      double tnu = 0.0;
+      Weights mu;
+      if constexpr (NU) mu = params_weights(prm);
+      const uint32_t active =
+          fit_candidate<MODE, NU>(k, px, prm.quality, sctab, rec, st, total, &mu, &tnu);
       st.id = id;
       w.rec[(size_t)rel * kMaxCands + c] = rec;
+      if constexpr (NU) w.nu_total[(size_t)rel * kMaxCands + c] = tnu;
       // mode 6, shape entry 0 followed by entry 1 (opaque blocks with two
       // shapes): the second slot gets the same fit and its own annealing state
       bool twin = false;
@@ -2343,7 +2619,10 @@ __global__ void __launch_bounds__(64) setup_kernel(BlockRange r, Bc7Params prm, 
         const uint32_t d1 = w.cdesc[(size_t)rel * kMaxCands + c + 1];
         twin = (d1 & 7) == 6 && ((d1 >> 12) & 7) == 1;
       }
-      if (twin) w.rec[(size_t)rel * kMaxCands + c + 1] = rec;
+      if (twin) {
+        w.rec[(size_t)rel * kMaxCands + c + 1] = rec;
+        if constexpr (NU) w.nu_total[(size_t)rel * kMaxCands + c + 1] = tnu;
+      }
       if (active) {
         st.info = (uint32_t)k.mode | ((uint32_t)k.rot << 3) | ((uint32_t)k.im << 5) |
                   ((uint32_t)k.nsub << 6) | ((uint32_t)k.shape << 8) | (active << 14) |
@@ -2385,8 +2664,15 @@ struct Lane {
   uint32_t key[3], ctr[3];
   uint64_t idx;        // indices of all subsets (best so far)
   uint32_t extra;
+  // nonuniform metric: the candidate's weights (its rotation applied), the
+  // single-colour subsets (their error is n times the stored one) and the
+  // alpha term
+  Weights m;
+  uint32_t scf;
+  double extra_nu;
 };
 
+template <bool NU = false>
 __device__ void lane_load(Lane &L, const BlockRange &r, const Bc7Params &prm, const AState &st) {
   L.has = true;
   L.id = st.id;
@@ -2426,6 +2712,12 @@ __device__ void lane_load(Lane &L, const BlockRange &r, const Bc7Params &prm, co
   }
   L.idx = (uint64_t)st.idx_lo | ((uint64_t)st.idx_hi << 32);
   L.extra = st.extra_err;
+  if constexpr (NU) {
+    L.m = rotate_weights(params_weights(prm), rot);
+    L.scf = (st.pbc >> 8) & 7u;
+    L.extra_nu = __longlong_as_double((long long)((uint64_t)st.extra_err |
+                                                  ((uint64_t)st.pad << 32)));
+  }
 }
 
 // The 16-pixel pass for up to NSUB subsets: errors per subset and the
@@ -2453,6 +2745,28 @@ __device__ __forceinline__ void score_pixels(const Lane &L, const SubConst &sc0,
       nerr[1] += s == 1 ? me : 0u;
       if (NSUB == 3) nerr[2] += s == 2 ? me : 0u;
     }
+    nidx |= (uint64_t)best << (4 * i);
+  }
+}
+
+// The pixel pass under the nonuniform metric: float errors, each subset's
+// summed in pixel order (adding 0 to the other subsets' sums leaves them
+// unchanged).
+template <int NSUB>
+__device__ __forceinline__ void score_pixels_nu(const Lane &L, const SubConst &sc0,
+                                                const SubConst &sc1, const SubConst &sc2, int nb,
+                                                WeightFn wf, float nerr[3], uint64_t &nidx) {
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const uint32_t s = NSUB == 1 ? 0u : (L.sub2 >> (2 * i)) & 3u;
+    const SubConst c = NSUB == 1 ? sc0
+                                 : (NSUB == 2 ? pick_sub(s != 0, sc1, sc0)
+                                              : pick_sub(s == 0, sc0, pick_sub(s == 1, sc1, sc2)));
+    int best;
+    const float me = pixel_err_nu(L.px[i], L.pt[i], c, nb, wf, L.m, &best);
+    nerr[0] += s == 0 ? me : 0.0f;
+    if (NSUB > 1) nerr[1] += s == 1 ? me : 0.0f;
+    if (NSUB == 3) nerr[2] += s == 2 ? me : 0.0f;
     nidx |= (uint64_t)best << (4 * i);
   }
 }
@@ -2493,7 +2807,7 @@ __device__ __forceinline__ void score_pixels_group(const Lane &L, const SubConst
 // candidate is finished.  With GROUPED, the lanes of an aligned group of G
 // hold the same candidate and share the pixel pass (member m); every other
 // part of the step runs identically in each of them.
-template <bool GROUPED = false>
+template <bool GROUPED = false, bool NU = false>
 __device__ bool lane_step(Lane &L, uint32_t q, int G = 1, int m = 0) {
   const Fit &f = L.f;
   const int nb = 1 << f.nbits_idx;
@@ -2523,7 +2837,14 @@ __device__ bool lane_step(Lane &L, uint32_t q, int G = 1, int m = 0) {
   // the widest subset count among the wave's running lanes picks the path
   // (a wave-uniform branch): the list is ordered by mode, so most waves hold
   // candidates of one subset count
-  if (GROUPED) {
+  if constexpr (NU) {
+    float nerf[3] = {0.0f, 0.0f, 0.0f};
+    if (__ballot(L.nsub == 3)) score_pixels_nu<3>(L, sc0, sc1, sc2, nb, wf, nerf, nidx);
+    else if (__ballot(L.nsub == 2)) score_pixels_nu<2>(L, sc0, sc1, sc2, nb, wf, nerf, nidx);
+    else score_pixels_nu<1>(L, sc0, sc1, sc2, nb, wf, nerf, nidx);
+#pragma unroll
+    for (int s = 0; s < 3; s++) nerr[s] = __float_as_uint(nerf[s]);
+  } else if (GROUPED) {
     if (__ballot(L.nsub == 3)) score_pixels_group<3>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx, G, m);
     else if (__ballot(L.nsub == 2)) score_pixels_group<2>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx, G, m);
     else score_pixels_group<1>(L, sc0, sc1, sc2, nb, kf, wf, nerr, nidx, G, m);
@@ -2552,16 +2873,22 @@ __device__ bool lane_step(Lane &L, uint32_t q, int G = 1, int m = 0) {
         // A single-precision estimate of the probability decides unless rr
         // lies within 1e-4 (relative) of it, which covers the estimate's
         // error; then the double expression of the reference decides.  The
-        // errors are integers below 2^24, so their float difference is exact.
-        const float diff = (float)L.cur[s] - (float)err;
+        // uniform metric's errors are integers below 2^24, so their float
+        // difference is exact; the nonuniform metric's are floats, whose
+        // difference has a relative error of at most 2^-23 where it is not
+        // exact, well inside the margin.
+        const float cf = NU ? __uint_as_float(L.cur[s]) : (float)L.cur[s];
+        const float ef = NU ? __uint_as_float(err) : (float)err;
+        const float diff = cf - ef;
         const float pf =
             __expf(__fdividef(0.1f * diff * (float)((int)q - 1), (float)L.energy[s]));
         if (rr != 0.0f && rr < pf * 0.9999f) {
           accept = true;
         } else if (!(rr != 0.0f && rr > pf * 1.0001f)) {
           const float temp = (float)L.energy[s] / (float)((int)q - 1);
-          const double pr =
-              exp(((double)0.1f * ((double)L.cur[s] - (double)err)) / (double)temp);
+          const double cd = NU ? (double)cf : (double)L.cur[s];
+          const double ed = NU ? (double)ef : (double)err;
+          const double pr = exp(((double)0.1f * (cd - ed)) / (double)temp);
           accept = (double)rr < pr;
         }
       }
@@ -2591,16 +2918,31 @@ __device__ bool lane_step(Lane &L, uint32_t q, int G = 1, int m = 0) {
   return L.active == 0;
 }
 
+template <bool NU = false>
 __device__ void lane_finish(Lane &L, Work &w) {
   CandRec *rc = &w.rec[L.id];
   const Fit &f = L.f;
   uint32_t total = L.extra;
+  // nonuniform metric: the reference's double total, subsets in order, then
+  // the alpha term of a rotation mode (Compressor.cpp:918, 1300-1371)
+  double tnu = 0.0;
   uint32_t pbcs = rc->meta & 0x3Fu;
   for (int s = 0; s < L.nsub; s++) {
     const uint32_t bp1 = s == 0 ? L.bp1[0] : (s == 1 ? L.bp1[1] : L.bp1[2]);
     const uint32_t bp2 = s == 0 ? L.bp2[0] : (s == 1 ? L.bp2[1] : L.bp2[2]);
     const int pb = s == 0 ? L.bpbc[0] : (s == 1 ? L.bpbc[1] : L.bpbc[2]);
-    total += s == 0 ? L.best[0] : (s == 1 ? L.best[1] : L.best[2]);
+    const uint32_t be = s == 0 ? L.best[0] : (s == 1 ? L.best[1] : L.best[2]);
+    total += be;
+    if constexpr (NU) {
+      double e = (double)__uint_as_float(be);
+      if ((L.scf >> s) & 1) {
+        int n = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) n += ((L.sub2 >> (2 * i)) & 3) == (uint32_t)s ? 1 : 0;
+        e = (double)n * e;
+      }
+      tnu += e;
+    }
     const int pb0 = f.pbit_type == PB_NONE ? -1 : pbc_bit(f.pbit_type, pb, 0);
     const int pb1 = f.pbit_type == PB_NONE ? -1 : pbc_bit(f.pbit_type, pb, 1);
     uint32_t e1 = to_pixel_int(bp1, f.qmask, pb0);
@@ -2617,6 +2959,7 @@ __device__ void lane_finish(Lane &L, Work &w) {
   rc->cidx_hi = (uint32_t)(L.idx >> 32);
   rc->meta = (rc->meta & ~0x3Fu) | pbcs | kMetaAnnealed;
   rc->err = total;
+  if constexpr (NU) w.nu_total[L.id] = tnu + L.extra_nu;
 }
 
 // A wave with at most this many candidates left after the list is used up
@@ -2628,7 +2971,10 @@ constexpr int kTailLanes = 16;
 // (168 VGPRs, a small spill) suits the long lists of the full search; the
 // learned search's short, filtered list runs faster with the uncapped
 // allocation (WAVES 1, two waves fit).
-template <int UNROLL, int WAVES>
+// NU: the nonuniform metric (float errors; the tail below, which splits a
+// candidate's pixel sums over lanes, would change their order and is not
+// used: the lanes finish their own candidates).
+template <int UNROLL, int WAVES, bool NU = false>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES)))
 anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
   const int lane = threadIdx.x;
@@ -2644,7 +2990,7 @@ anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
   bool exhausted = false;
   for (;;) {
     // the list is used up and few candidates remain: the tail below
-    if (exhausted && __popcll(__ballot(L.has)) <= kTailLanes) break;
+    if (!NU && exhausted && __popcll(__ballot(L.has)) <= kTailLanes) break;
     // refill idle lanes from the window, fetching a new window when needed
     for (;;) {
       const uint64_t need = __ballot(!L.has);
@@ -2667,20 +3013,21 @@ anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
         // 1, 3, 7, 6, 4, 5, and the rotation modes' candidates anneal the
         // longest, so starting with them keeps the tail of the launch short
         const uint32_t item = total - 1u - (wbase + rank);
-        lane_load(L, r, prm, w.astate[w.alist ? w.alist[item] : item]);
+        lane_load<NU>(L, r, prm, w.astate[w.alist ? w.alist[item] : item]);
       }
       wbase += min(avail, (uint32_t)__popcll(need));
     }
     if (!__ballot(L.has)) break;
     for (int u = 0; u < UNROLL; u++) {
       if (L.has) {
-        if (lane_step(L, prm.quality)) {
-          lane_finish(L, w);
+        if (lane_step<false, NU>(L, prm.quality)) {
+          lane_finish<NU>(L, w);
           L.has = false;
         }
       }
     }
   }
+  if constexpr (NU) return;
   // Tail: the wave's last candidates (at most kTailLanes) each get a group
   // of G lanes that share the pixel pass of every step, so the longest
   // anneals, which set the launch's length, run up to G times fewer
@@ -2873,13 +3220,26 @@ __global__ void __launch_bounds__(256) pack_kernel(BlockRange r, Bc7Scratch scr,
     // first strict minimum
     const CandRec *rc = &w.rec[(size_t)rel * kMaxCands];
     const int count = sel_count(sel);
-    uint32_t best = 0xFFFFFFFFu;
     int bc = -1;
-    for (int i = 0; i < count; i++) {
-      const uint32_t e = rc[i].err;
-      if (e < best) {
-        best = e;
-        bc = i;
+    if (w.nu_total) {
+      // the nonuniform metric's double totals (the reference's best_err
+      // starts at DBL_MAX, Compressor.cpp:1756)
+      const double *tn = w.nu_total + (size_t)rel * kMaxCands;
+      double best = DBL_MAX;
+      for (int i = 0; i < count; i++) {
+        if (tn[i] < best) {
+          best = tn[i];
+          bc = i;
+        }
+      }
+    } else {
+      uint32_t best = 0xFFFFFFFFu;
+      for (int i = 0; i < count; i++) {
+        const uint32_t e = rc[i].err;
+        if (e < best) {
+          best = e;
+          bc = i;
+        }
       }
     }
     if (bc >= 0) {
@@ -3289,7 +3649,10 @@ hipError_t bc7_upload_tables(int device) {
 
 void launch_anneal(uint32_t grid, const BlockRange &r, const Bc7Params &p, const bc7::Work &w,
                    hipStream_t stream) {
-  if (w.alist)  // learned search: four steps between refills
+  if (w.nu_total)  // nonuniform metric (the full search or given selections)
+    hipLaunchKernelGGL((bc7::anneal_kernel<4, 1, true>), dim3(grid), dim3(64), 0, stream, r, p,
+                       w);
+  else if (w.alist)  // learned search: four steps between refills
     hipLaunchKernelGGL((bc7::anneal_kernel<4, 1>), dim3(grid), dim3(64), 0, stream, r, p, w);
   else if (p.anneal_unroll >= 4)
     hipLaunchKernelGGL((bc7::anneal_kernel<4, 3>), dim3(grid), dim3(64), 0, stream, r, p, w);
@@ -3333,7 +3696,7 @@ hipError_t bc7_scatter(const SurfDesc *surfs, const uint32_t *block_surf, uint32
   return hipGetLastError();
 }
 
-size_t bc7_work_bytes(uint32_t num_blocks, bool learned) {
+size_t bc7_work_bytes(uint32_t num_blocks, bool learned, bool nonuniform) {
   const size_t n = num_blocks;
   // counters, sel, cdesc, rec, queues, astate, q0, alignment slack
   size_t bytes = 64 * 4 + n * 4 + n * bc7::kMaxCands * 2 +
@@ -3345,6 +3708,7 @@ size_t bc7_work_bytes(uint32_t num_blocks, bool learned) {
     bytes += n * 8 + n * bc7::kMaxCands * (8 + 4 + 4) + n * 4 +
              (n * bc7::kMaxCands / bc7::kListChunk + 1) * 4 + 11 * 256;
   }
+  if (nonuniform) bytes += n * bc7::kMaxCands * 8 + 256;  // candidate totals
   return bytes;
 }
 
@@ -3383,6 +3747,11 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   unsigned long long *cerr = nullptr;
   w.alist = nullptr;
   w.aid = nullptr;
+  w.nu_total = nullptr;
+  // the nonuniform metric runs the reference search only (the host entry
+  // points reject it with the ranker or statistics)
+  const bool nu = p.nonuniform != 0 && !learned && !p.stats;
+  if (nu) w.nu_total = reinterpret_cast<double *>(carve(nc * 8));
   uint32_t *alist = nullptr, *chunk_count = nullptr;
   if (learned) {
     q0err = reinterpret_cast<unsigned long long *>(carve((size_t)n * 8));
@@ -3432,6 +3801,11 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
     hipLaunchKernelGGL(bc7::select_kernel<bc7::kSelGiven>, dim3(ngroups), dim3(256), 0, stream,
                        r, p, w, (const uint32_t *)nullptr, (const uint32_t *)nullptr);
     timing_end(t, kTagBc7Select, stream);
+  } else if (nu) {
+    t = timing_begin(stream);
+    hipLaunchKernelGGL((bc7::select_kernel<bc7::kSelBox, true>), dim3(ngroups), dim3(256), 0,
+                       stream, r, p, w, (const uint32_t *)nullptr, (const uint32_t *)nullptr);
+    timing_end(t, kTagBc7Select, stream);
   } else {
     t = timing_begin(stream);
     hipLaunchKernelGGL(bc7::select_kernel<bc7::kSelBox>, dim3(ngroups), dim3(256), 0, stream,
@@ -3440,14 +3814,33 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   }
   t = timing_begin(stream);
   // queue (type) order: modes 0, 2, 1, 3, 7, 6, 4, 5
-  hipLaunchKernelGGL(bc7::setup_kernel<0>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
-  hipLaunchKernelGGL(bc7::setup_kernel<2>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
-  hipLaunchKernelGGL(bc7::setup_kernel<1>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
-  hipLaunchKernelGGL(bc7::setup_kernel<3>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
-  hipLaunchKernelGGL(bc7::setup_kernel<7>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
-  hipLaunchKernelGGL(bc7::setup_kernel<6>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
-  hipLaunchKernelGGL(bc7::setup_kernel<4>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
-  hipLaunchKernelGGL(bc7::setup_kernel<5>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+  if (nu) {
+    hipLaunchKernelGGL((bc7::setup_kernel<0, true>), dim3(persistent), dim3(64), 0, stream, r,
+                       p, w, sct);
+    hipLaunchKernelGGL((bc7::setup_kernel<2, true>), dim3(persistent), dim3(64), 0, stream, r,
+                       p, w, sct);
+    hipLaunchKernelGGL((bc7::setup_kernel<1, true>), dim3(persistent), dim3(64), 0, stream, r,
+                       p, w, sct);
+    hipLaunchKernelGGL((bc7::setup_kernel<3, true>), dim3(persistent), dim3(64), 0, stream, r,
+                       p, w, sct);
+    hipLaunchKernelGGL((bc7::setup_kernel<7, true>), dim3(persistent), dim3(64), 0, stream, r,
+                       p, w, sct);
+    hipLaunchKernelGGL((bc7::setup_kernel<6, true>), dim3(persistent), dim3(64), 0, stream, r,
+                       p, w, sct);
+    hipLaunchKernelGGL((bc7::setup_kernel<4, true>), dim3(persistent), dim3(64), 0, stream, r,
+                       p, w, sct);
+    hipLaunchKernelGGL((bc7::setup_kernel<5, true>), dim3(persistent), dim3(64), 0, stream, r,
+                       p, w, sct);
+  } else {
+    hipLaunchKernelGGL(bc7::setup_kernel<0>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+    hipLaunchKernelGGL(bc7::setup_kernel<2>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+    hipLaunchKernelGGL(bc7::setup_kernel<1>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+    hipLaunchKernelGGL(bc7::setup_kernel<3>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+    hipLaunchKernelGGL(bc7::setup_kernel<7>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+    hipLaunchKernelGGL(bc7::setup_kernel<6>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+    hipLaunchKernelGGL(bc7::setup_kernel<4>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+    hipLaunchKernelGGL(bc7::setup_kernel<5>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+  }
   timing_end(t, kTagBc7Setup, stream);
   if (learned) {
     if (p.quality > 0) {

@@ -210,8 +210,12 @@ fh_status validate(const fh_job *j, bool device) {
   const size_t need = (size_t)(j->width / 4) * (j->height / 4) * block_bytes(j->format);
   if (!j->out || j->out_size < need)
     return fail(FH_ERR_BUFFER_TOO_SMALL, "Not enough space for compressed data!");
-  if (j->format == FH_FORMAT_BPTC && j->error_metric != 0)
-    return fail(FH_ERR_UNSUPPORTED, "only the uniform error metric is implemented");
+  if (j->format == FH_FORMAT_BPTC && j->error_metric > 1)
+    return fail(FH_ERR_INVALID_ARGUMENT, "unknown error metric");
+  if (j->format == FH_FORMAT_BPTC && j->error_metric == 1 && (j->flags & FH_FLAG_LEARNED))
+    return fail(FH_ERR_UNSUPPORTED,
+                "the learned search ranks for the uniform metric: the nonuniform metric "
+                "runs the reference search only");
   // a mask without any of modes 0-7 leaves no candidate for any searched
   // block (the reference asserts, Compressor.cpp:1857)
   if (j->format == FH_FORMAT_BPTC && j->block_modes && !(j->block_modes & 0xFFu))
@@ -230,11 +234,11 @@ fh_status validate(const fh_job *j, bool device) {
 
 // Bytes of device scratch one BC7 call of up to `blocks` blocks takes: the
 // watermark scan's prefixes, then the work area.
-size_t bc7_call_bytes(uint32_t blocks, bool learned, size_t *work_off) {
+size_t bc7_call_bytes(uint32_t blocks, bool learned, bool nonuniform, size_t *work_off) {
   const uint32_t nchunks = (blocks + 255) / 256;
   const size_t prefix_bytes = ((size_t)blocks + 255) & ~(size_t)255;
   *work_off = (prefix_bytes + (size_t)nchunks * 4 + 255) & ~(size_t)255;
-  return *work_off + bc7_work_bytes(blocks, learned) + 256;
+  return *work_off + bc7_work_bytes(blocks, learned, nonuniform) + 256;
 }
 
 // Runs the encoder on device buffers.  Caller holds st->mu.  With
@@ -260,8 +264,9 @@ fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8
       const uint32_t per = count < chunk ? count : chunk;
       const size_t prefix_bytes = ((size_t)per + 255) & ~(size_t)255;
       const bool learned = (j->flags & FH_FLAG_LEARNED) != 0;
+      const bool nonuniform = j->error_metric == 1 && !dstats;
       size_t work_off = 0;
-      const size_t need = bc7_call_bytes(per, learned, &work_off);
+      const size_t need = bc7_call_bytes(per, learned, nonuniform, &work_off);
       uint8_t *sbase = fixed_work;
       if (fixed_work) {
         if (need > fixed_size) return fail(FH_ERR_INVALID_ARGUMENT, "job too large for the work area");
@@ -303,6 +308,12 @@ fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8
       p.anneal_unroll = 4;  // annealing steps between refills: 4 measured fastest for both searches
       p.stats = nullptr;
       p.selections = nullptr;
+      // kErrorMetrics (Compressor.cpp:205-208)
+      p.nonuniform = nonuniform ? 1u : 0u;
+      p.metric[0] = nonuniform ? sqrtf(0.3f) : 1.0f;
+      p.metric[1] = nonuniform ? sqrtf(0.56f) : 1.0f;
+      p.metric[2] = nonuniform ? sqrtf(0.11f) : 1.0f;
+      p.metric[3] = 1.0f;
       // jobs larger than one call's work area run as consecutive calls on the
       // stream; the watermark counter carries across them in order
       for (uint32_t done = 0; done < count; done += chunk) {
@@ -516,8 +527,20 @@ extern "C" {
 
 fh_status fh_compress(const fh_job *j) { return compress_host(j, false, nullptr, 0); }
 
+// CompressWithStats under the nonuniform metric is not built: its shape scan
+// logs the uniform estimates only.
+fh_status stats_metric_ok(const fh_job *j) {
+  if (j && j->error_metric == 1)
+    return fail(FH_ERR_UNSUPPORTED,
+                "CompressWithStats runs the uniform metric only (the nonuniform one runs "
+                "through fh_compress / fh_compress_device)");
+  return FH_OK;
+}
+
 fh_status fh_compress_with_stats(const fh_job *j, fh_bc7_block_stats *stats, size_t stats_count) {
-  const fh_status s = stats_job_ok(j);
+  fh_status s = stats_job_ok(j);
+  if (s != FH_OK) return s;
+  s = stats_metric_ok(j);
   if (s != FH_OK) return s;
   return compress_host(j, true, stats, stats_count);
 }
@@ -525,6 +548,8 @@ fh_status fh_compress_with_stats(const fh_job *j, fh_bc7_block_stats *stats, siz
 fh_status fh_compress_with_stats_device(const fh_job *j, fh_bc7_block_stats *stats,
                                         size_t stats_count, void *hip_stream) {
   fh_status s = stats_job_ok(j);
+  if (s != FH_OK) return s;
+  s = stats_metric_ok(j);
   if (s != FH_OK) return s;
   s = validate(j, true);
   if (s != FH_OK) return s;
@@ -657,7 +682,8 @@ fh_status fh_compress_device_noalloc(const fh_job *j, void *hip_stream) {
   hipError_t e = ensure_init(st);
   if (e != hipSuccess) return fail_hip(e, "device initialisation");
   size_t work_off = 0;
-  if (bc7_call_bytes(kNoAllocMaxBlocks, false, &work_off) > st->mem.scratch_bytes)
+  if (bc7_call_bytes(kNoAllocMaxBlocks, false, j->error_metric == 1, &work_off) >
+      st->mem.scratch_bytes)
     return fail(FH_ERR_HIP, "no-allocation work area smaller than its block limit needs");
   return run_device(st, j, j->in, j->out, reinterpret_cast<hipStream_t>(hip_stream), first,
                     count, nullptr, nullptr, st->mem.scratch, st->mem.scratch_bytes);

@@ -43,6 +43,11 @@ struct Bc7Params {
   // a caller's shape selection per block of the range (device; null = the
   // library's own search); not combined with the ranker or statistics
   const fh_shape_selection *selections;
+  // m_ErrorMetric (BPTCCompressor.h:73-77): 0 the uniform metric, 1 the
+  // nonuniform one with per-channel weights `metric` (kErrorMetrics,
+  // Compressor.cpp:205-208); not combined with the ranker or statistics
+  uint32_t nonuniform;
+  float metric[4];
 };
 
 // Device scratch needed by the BC7 watermark prefix scan.
@@ -59,7 +64,7 @@ const uint32_t *bc7_single_color_table(int device);
 // Device work area bc7_encode needs for num_blocks blocks (candidate
 // records, queues and annealing states, and with the ranker its
 // activations).
-size_t bc7_work_bytes(uint32_t num_blocks, bool learned);
+size_t bc7_work_bytes(uint32_t num_blocks, bool learned, bool nonuniform = false);
 
 // Largest block count one bc7_encode call takes; callers split bigger jobs.
 constexpr uint32_t kBc7MaxBlocksPerCall = 1u << 20;

@@ -119,7 +119,9 @@ typedef struct fh_job {
   /* BPTC only (BPTCC::CompressionSettings) */
   uint32_t quality;       /* simulated-annealing steps; the reference sets it from iQuality */
   uint32_t block_modes;   /* mask of BC7 modes (bit m = mode m); 0 means all */
-  uint32_t error_metric;  /* 0 = uniform (the only metric the reference's tc uses) */
+  uint32_t error_metric;  /* m_ErrorMetric: 0 uniform (tc's), 1 nonuniform (weights
+                           * sqrt(0.3), sqrt(0.56), sqrt(0.11), 1; reference search
+                           * only: not with FH_FLAG_LEARNED or statistics) */
   uint64_t seed;          /* annealing random stream key */
   uint32_t *wm_counter;   /* with FH_FLAG_CALLER_WATERMARK: in/out watermark counter */
   uint32_t flags;

@@ -121,3 +121,27 @@ def test_modes_that_leave_no_candidate_are_rejected():
     img[4:8, 0:4] = 0xFF123456
     if not fastc_amd.device_available():
         assert L.fh_compress_with_selection(ctypes.byref(j), sels.ctypes.data, 4) == 7
+
+
+def test_error_metric_validation():
+    """m_ErrorMetric: 0 and 1 exist (BPTCCompressor.h:73-77); the nonuniform
+    metric runs the reference search, so it is refused with the learned
+    search and with CompressWithStats, before any device work."""
+    L = fastc_amd.lib()
+    img = np.zeros((8, 8), np.uint32)
+    out = np.zeros(64, np.uint8)
+    j = fastc_amd._job(3, img.ctypes.data, out.ctypes.data, 64, 8, 8, None, 0, 0, 0xFF, None,
+                       metric=2)
+    assert L.fh_compress(ctypes.byref(j)) == 1
+    assert b"metric" in L.fh_last_error()
+    rk = fastc_amd.FhJob()  # a non-null ranker handle is never dereferenced here
+    j = fastc_amd._job(3, img.ctypes.data, out.ctypes.data, 64, 8, 8, None, 0, 0, 0xFF, None,
+                       metric=1)
+    j.flags |= fastc_amd.FH_FLAG_LEARNED
+    j.ranker = ctypes.cast(ctypes.pointer(rk), ctypes.c_void_p)
+    assert L.fh_compress(ctypes.byref(j)) == 6
+    assert b"uniform" in L.fh_last_error()
+    stats = np.zeros(4, dtype=fastc_amd.BLOCK_STATS_DTYPE)
+    j = fastc_amd._job(3, img.ctypes.data, out.ctypes.data, 64, 8, 8, None, 0, 0, 0xFF, None,
+                       metric=1)
+    assert L.fh_compress_with_stats(ctypes.byref(j), stats.ctypes.data, 4) == 6
