@@ -6,6 +6,10 @@
 //   fasthip::FasTC::DecompressionJob     Base/include/FasTC/CompressionJob.h:144-169
 //   fasthip::SCompressionSettings        Core/include/FasTC/TexComp.h:30-76
 //   fasthip::CompressImageData           Core/include/FasTC/TexComp.h:81-88
+//   fasthip::CompressImage               Core/include/FasTC/TexComp.h:78-79
+//   fasthip::CompressedImage             Core/include/FasTC/CompressedImage.h:25-75
+//   fasthip::FasTC::CompressionJobList   Base/include/FasTC/CompressionJob.h:172-209
+//   fasthip::BPTCC::CompressAtomic       BPTCEncoder/include/FasTC/BPTCCompressor.h:184-189
 //   fasthip::BPTCC::CompressionSettings  BPTCEncoder/include/FasTC/BPTCCompressor.h:123-158
 //   fasthip::BPTCC::Compress/Decompress  BPTCCompressor.h:166-167, 216
 //   fasthip::BPTCC::CompressWithStats    BPTCCompressor.h:173-174
@@ -24,8 +28,10 @@
 
 #include <stdint.h>
 #include <stdio.h>
+#include <string.h>
 
 #include <ostream>
+#include <thread>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -103,6 +109,45 @@ inline size_t GetCompressedSize(uint32_t w, uint32_t h, ECompressionFormat fmt) 
   size_t n = 0;
   return fh_compressed_size((uint32_t)fmt, w, h, &n) == FH_OK ? n : 0;
 }
+
+// A list of textures for BPTCC::CompressAtomic (CompressionJob.h:172-209,
+// Base/src/CompressionJob.cpp:29-106): room for nJobs jobs, AddJob false
+// once full, a finished flag per job, and the shared job / block indices the
+// callers advance.  GetJob(idx) returns job idx (the reference's returns the
+// job at m_CurrentJobIndex whatever idx is, CompressionJob.cpp:92-98).
+class CompressionJobList {
+ public:
+  explicit CompressionJobList(uint32_t nJobs)
+      : m_TotalNumJobs(nJobs), m_Finished(nJobs, 0), m_CurrentJobIndex(0),
+        m_CurrentBlockIndex(0) {
+    m_Jobs.reserve(nJobs);
+  }
+  CompressionJobList(const CompressionJobList &) = default;
+  CompressionJobList &operator=(const CompressionJobList &) = default;
+
+  bool AddJob(const CompressionJob &cj) {
+    if (m_Jobs.size() == m_TotalNumJobs) return false;
+    m_Jobs.push_back(cj);
+    return true;
+  }
+  uint32_t GetTotalNumJobs() const { return m_TotalNumJobs; }
+  uint32_t GetNumJobs() const { return (uint32_t)m_Jobs.size(); }
+  const CompressionJob *GetJob(uint32_t idx) const {
+    return idx < m_Jobs.size() ? &m_Jobs[idx] : nullptr;
+  }
+  uint32_t *GetFinishedFlag(uint32_t idx) const {
+    return idx < m_Jobs.size() ? const_cast<uint32_t *>(&m_Finished[idx]) : nullptr;
+  }
+
+ private:
+  std::vector<CompressionJob> m_Jobs;
+  uint32_t m_TotalNumJobs;
+  std::vector<uint32_t> m_Finished;
+
+ public:
+  uint32_t m_CurrentJobIndex;
+  uint32_t m_CurrentBlockIndex;
+};
 
 }  // namespace FasTC
 
@@ -252,6 +297,29 @@ inline void Decompress(const FasTC::DecompressionJob &dj) {
                               dj.Height()));
 }
 
+// BPTCC::CompressAtomic (Compressor.cpp:1541-1574): threads that call it with
+// the same list work through it together and each returns once every job is
+// done.  A caller claims a whole job at a time through m_CurrentJobIndex
+// (the reference shares the blocks of the current job through
+// m_CurrentBlockIndex; here one job is one GPU encode) and encodes it with
+// the default settings, as the reference's CompressBC7Block does, then sets
+// its finished flag.
+inline void CompressAtomic(FasTC::CompressionJobList &cjl) {
+  const uint32_t n = cjl.GetNumJobs();
+  for (;;) {
+    uint32_t idx = __atomic_load_n(&cjl.m_CurrentJobIndex, __ATOMIC_ACQUIRE);
+    if (idx >= n) break;
+    if (!__atomic_compare_exchange_n(&cjl.m_CurrentJobIndex, &idx, idx + 1, false,
+                                     __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE))
+      continue;
+    Compress(*cjl.GetJob(idx));
+    __atomic_store_n(cjl.GetFinishedFlag(idx), 1u, __ATOMIC_RELEASE);
+  }
+  // wait for the jobs other callers claimed
+  for (uint32_t i = 0; i < n; i++)
+    while (!__atomic_load_n(cjl.GetFinishedFlag(i), __ATOMIC_ACQUIRE)) std::this_thread::yield();
+}
+
 }  // namespace BPTCC
 
 namespace DXTC {
@@ -337,6 +405,71 @@ inline bool CompressImageData(const unsigned char *data, unsigned int width, uns
     }
   }
   return true;
+}
+
+// CompressedImage (Core/include/FasTC/CompressedImage.h:25-75,
+// Core/src/CompressedImage.cpp): the compressed blocks of a width x height
+// image in one format, and their decode.  The reference derives it from its
+// Image<Pixel> class; this one keeps the members a caller of CompressImage
+// reads.
+class CompressedImage {
+ public:
+  CompressedImage(uint32_t width, uint32_t height, FasTC::ECompressionFormat format,
+                  const uint8_t *data)
+      : m_Width(width), m_Height(height), m_Format(format),
+        m_Data(data, data + GetCompressedSize(width, height, format)) {}
+  static uint32_t GetCompressedSize(uint32_t width, uint32_t height,
+                                    FasTC::ECompressionFormat format) {
+    return (uint32_t)FasTC::GetCompressedSize(width, height, format);
+  }
+  uint32_t GetCompressedSize() const { return GetCompressedSize(m_Width, m_Height, m_Format); }
+  uint32_t GetUncompressedSize() const { return m_Width * m_Height * 4; }
+  uint32_t GetWidth() const { return m_Width; }
+  uint32_t GetHeight() const { return m_Height; }
+  const uint8_t *GetCompressedData() const { return m_Data.data(); }
+  FasTC::ECompressionFormat GetFormat() const { return m_Format; }
+  // CompressedImage::DecompressImage: false when outBuf is smaller than the
+  // RGBA8 image; decoded on the GPU.
+  bool DecompressImage(uint8_t *outBuf, uint32_t outBufSz) const {
+    if (outBufSz < GetUncompressedSize()) return false;
+    return fh_decompress((uint32_t)m_Format, m_Data.data(), m_Data.size(), outBuf, m_Width,
+                         m_Height) == FH_OK;
+  }
+
+ private:
+  uint32_t m_Width, m_Height;
+  FasTC::ECompressionFormat m_Format;
+  std::vector<uint8_t> m_Data;
+};
+
+// CompressImage (Core/src/TexComp.cpp:367-424) for any image type with the
+// reference Image's interface: GetWidth(), GetHeight(), ComputePixels() and
+// operator()(x, y) returning a pixel whose Pack() is the RGBA8 word.  A size
+// that is not a multiple of the block size is padded with zeros (with the
+// reference's warning); returns null when the encode fails.  The caller
+// owns the result.
+template <typename ImageT>
+CompressedImage *CompressImage(ImageT *img, const SCompressionSettings &settings) {
+  if (!img) return nullptr;
+  uint32_t width = img->GetWidth();
+  uint32_t height = img->GetHeight();
+  if (width % 4 || height % 4) {
+    fprintf(stderr, "TexComp -- %s\n",
+            "WARNING - Image size is not a multiple of block size. Padding with zeros...");
+    width = (width + 3) / 4 * 4;
+    height = (height + 3) / 4 * 4;
+  }
+  std::vector<uint32_t> data((size_t)width * height, 0);
+  img->ComputePixels();
+  for (uint32_t j = 0; j < img->GetHeight(); j++)
+    for (uint32_t i = 0; i < img->GetWidth(); i++)
+      data[(size_t)j * width + i] = (uint32_t)(*img)(i, j).Pack();
+  const uint32_t sz = CompressedImage::GetCompressedSize(width, height, settings.format);
+  std::vector<uint8_t> cmp(sz);
+  if (!CompressImageData(reinterpret_cast<const unsigned char *>(data.data()), width, height,
+                         cmp.data(), sz, settings))
+    return nullptr;
+  return new CompressedImage(width, height, settings.format, cmp.data());
 }
 
 }  // namespace fasthip

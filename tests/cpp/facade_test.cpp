@@ -8,17 +8,40 @@
 //   facade_test stats in.raw w h outdir  -> BPTCC::CompressWithStats (q=0) on a
 //                                           sub-job and CompressImageData with a
 //                                           log stream: blocks and log text
+//   facade_test image in.raw w h outdir  -> CompressImage of the top-left
+//                                           (w-2) x (h-2) pixels (zero padding),
+//                                           its blocks and DecompressImage
+//   facade_test atomic in.raw w h outdir -> BPTCC::CompressAtomic from two
+//                                           threads over three jobs
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fasthip/fastc.hpp"
 
 using namespace fasthip;
+
+// The reference Image's interface as CompressImage reads it: a w x h image
+// of RGBA8 words.
+struct TestPixel {
+  uint32_t v;
+  uint32_t Pack() const { return v; }
+};
+struct TestImage {
+  uint32_t w, h;
+  std::vector<uint32_t> px;  // stride `stride`
+  uint32_t stride;
+  bool computed = false;
+  uint32_t GetWidth() const { return w; }
+  uint32_t GetHeight() const { return h; }
+  void ComputePixels() { computed = true; }
+  TestPixel operator()(uint32_t x, uint32_t y) const { return TestPixel{px[(size_t)y * stride + x]}; }
+};
 
 static int validate() {
   std::vector<uint8_t> img(8 * 8 * 4, 0), out(8 * 8, 0);
@@ -34,6 +57,13 @@ static int validate() {
   cpu.bUseGPU = false;
   if (CompressImageData(img.data(), 8, 8, out.data(), (unsigned)out.size(), cpu)) fails++;
   if (FasTC::GetCompressedSize(8, 8, FasTC::eCompressionFormat_DXT1) != 32) fails++;
+  // CompressImage of no image; a job list that is full
+  if (CompressImage(static_cast<TestImage *>(nullptr), s) != nullptr) fails++;
+  FasTC::CompressionJobList list(1);
+  FasTC::CompressionJob j0(FasTC::eCompressionFormat_BPTC, img.data(), out.data(), 8, 8);
+  if (!list.AddJob(j0) || list.AddJob(j0) || list.GetNumJobs() != 1) fails++;
+  if (list.GetJob(1) != nullptr || list.GetFinishedFlag(1) != nullptr) fails++;
+  if (*list.GetFinishedFlag(0) != 0 || list.GetTotalNumJobs() != 1) fails++;
   if (!fh_device_available()) {
     // a valid job must report the missing device, never fall back to CPU
     FasTC::CompressionJob cj(FasTC::eCompressionFormat_BPTC, img.data(), out.data(), 8, 8);
@@ -135,6 +165,66 @@ static int stats(const char *in, unsigned w, unsigned h, const std::string &dir)
   return 0;
 }
 
+static bool load(const char *in, unsigned w, unsigned h, std::vector<uint8_t> &img) {
+  img.resize((size_t)w * h * 4);
+  FILE *f = fopen(in, "rb");
+  if (!f) return false;
+  const bool ok = fread(img.data(), 1, img.size(), f) == img.size();
+  fclose(f);
+  return ok;
+}
+
+static int image(const char *in, unsigned w, unsigned h, const std::string &dir) {
+  std::vector<uint8_t> raw;
+  if (!load(in, w, h, raw)) return 2;
+  TestImage ti;
+  ti.w = w - 2;
+  ti.h = h - 2;
+  ti.stride = w;
+  ti.px.resize((size_t)w * h);
+  memcpy(ti.px.data(), raw.data(), raw.size());
+  SCompressionSettings s;
+  s.format = FasTC::eCompressionFormat_BPTC;
+  s.iQuality = 0;
+  CompressedImage *ci = CompressImage(&ti, s);
+  if (!ci || !ti.computed || ci->GetWidth() != w || ci->GetHeight() != h) return 3;
+  std::vector<uint8_t> dec(ci->GetUncompressedSize());
+  if (ci->DecompressImage(dec.data(), 16)) return 4;  // too small
+  if (!ci->DecompressImage(dec.data(), (uint32_t)dec.size())) return 5;
+  save(dir + "/image.bin",
+       std::vector<uint8_t>(ci->GetCompressedData(), ci->GetCompressedData() + ci->GetCompressedSize()));
+  save(dir + "/image_dec.bin", dec);
+  delete ci;
+  printf("image ok\n");
+  return 0;
+}
+
+static int atomic(const char *in, unsigned w, unsigned h, const std::string &dir) {
+  std::vector<uint8_t> img;
+  if (!load(in, w, h, img)) return 2;
+  // three jobs: the image, its top half, its left-half columns as a narrow image
+  const unsigned h2 = h / 2, w2 = w / 2;
+  std::vector<uint8_t> top(img.begin(), img.begin() + (size_t)w * h2 * 4), left((size_t)w2 * h * 4);
+  for (unsigned y = 0; y < h; y++) memcpy(&left[(size_t)y * w2 * 4], &img[(size_t)y * w * 4], w2 * 4);
+  std::vector<uint8_t> o0((size_t)w * h), o1((size_t)w * h2), o2((size_t)w2 * h);
+  FasTC::CompressionJobList list(3);
+  list.AddJob(FasTC::CompressionJob(FasTC::eCompressionFormat_BPTC, img.data(), o0.data(), w, h));
+  list.AddJob(FasTC::CompressionJob(FasTC::eCompressionFormat_BPTC, top.data(), o1.data(), w, h2));
+  list.AddJob(FasTC::CompressionJob(FasTC::eCompressionFormat_BPTC, left.data(), o2.data(), w2, h));
+  std::thread t1([&] { BPTCC::CompressAtomic(list); });
+  std::thread t2([&] { BPTCC::CompressAtomic(list); });
+  t1.join();
+  t2.join();
+  for (uint32_t i = 0; i < 3; i++)
+    if (*list.GetFinishedFlag(i) != 1) return 3;
+  BPTCC::CompressAtomic(list);  // nothing left: returns at once
+  save(dir + "/atomic0.bin", o0);
+  save(dir + "/atomic1.bin", o1);
+  save(dir + "/atomic2.bin", o2);
+  printf("atomic ok\n");
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc >= 2 && !strcmp(argv[1], "validate")) return validate();
   if (argc >= 6 && !strcmp(argv[1], "encode"))
@@ -143,6 +233,10 @@ int main(int argc, char **argv) {
     return select(argv[2], (unsigned)atoi(argv[3]), (unsigned)atoi(argv[4]), argv[5]);
   if (argc >= 6 && !strcmp(argv[1], "stats"))
     return stats(argv[2], (unsigned)atoi(argv[3]), (unsigned)atoi(argv[4]), argv[5]);
+  if (argc >= 6 && !strcmp(argv[1], "image"))
+    return image(argv[2], (unsigned)atoi(argv[3]), (unsigned)atoi(argv[4]), argv[5]);
+  if (argc >= 6 && !strcmp(argv[1], "atomic"))
+    return atomic(argv[2], (unsigned)atoi(argv[3]), (unsigned)atoi(argv[4]), argv[5]);
   fprintf(stderr, "usage: facade_test validate | encode|select|stats in.raw w h outdir\n");
   return 2;
 }

@@ -22,7 +22,7 @@ def facade_bin(tmp_path_factory):
     subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"),
                     os.path.join(ROOT, "tests", "cpp", "facade_test.cpp"),
                     "-L", libdir, "-lfasthip", f"-Wl,-rpath,{libdir}",
-                    "-Wl,--allow-shlib-undefined", "-o", str(out)], check=True)
+                    "-Wl,--allow-shlib-undefined", "-pthread", "-o", str(out)], check=True)
     return str(out)
 
 
@@ -98,3 +98,43 @@ def test_facade_shape_selection_function(fh, oracle, facade_bin, tmp_path):
     ref, _ = oracle.bc7_compress_selected(img, sels)
     got = np.fromfile(tmp_path / "select.bin", dtype=np.uint8)
     assert np.array_equal(oracle.bc7_decompress(got, 64, 64), oracle.bc7_decompress(ref, 64, 64))
+
+
+@pytest.mark.gpu
+def test_facade_compress_image(fh, facade_bin, tmp_path):
+    """CompressImage (Core/src/TexComp.cpp:367-424) of a 62x62 image: padded
+    with zeros to 64x64 (with the reference's warning), the blocks of the
+    padded image, and DecompressImage's decode of them."""
+    img = synth.generate(64, 64, 4)
+    raw = tmp_path / "in.raw"
+    img.tofile(raw)
+    r = subprocess.run([facade_bin, "image", str(raw), "64", "64", str(tmp_path)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Padding with zeros" in r.stderr
+    padded = np.zeros_like(img)
+    padded[:62, :62] = img[:62, :62]
+    want = fh.compress(padded, quality=0, wm_counter=ctypes.c_uint32(0xFFFFFFFF))
+    got = np.fromfile(tmp_path / "image.bin", dtype=np.uint8)
+    dec = np.fromfile(tmp_path / "image_dec.bin", dtype=np.uint32).reshape(64, 64)
+    # watermark entries of solid blocks follow the process-wide counter
+    assert np.array_equal(fh.decompress(got, 64, 64), fh.decompress(want, 64, 64))
+    assert np.array_equal(dec, fh.decompress(got, 64, 64))
+
+
+@pytest.mark.gpu
+def test_facade_compress_atomic(fh, facade_bin, tmp_path):
+    """BPTCC::CompressAtomic from two threads over a list of three jobs: each
+    job is encoded once, with the default settings (50 annealing steps)."""
+    img = synth.generate(64, 64, 5)
+    raw = tmp_path / "in.raw"
+    img.tofile(raw)
+    r = subprocess.run([facade_bin, "atomic", str(raw), "64", "64", str(tmp_path)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    parts = [img, np.ascontiguousarray(img[:32]), np.ascontiguousarray(img[:, :32])]
+    for k, part in enumerate(parts):
+        h, w = part.shape
+        want = fh.compress(part, quality=50, wm_counter=ctypes.c_uint32(0xFFFFFFFF))
+        got = np.fromfile(tmp_path / f"atomic{k}.bin", dtype=np.uint8)
+        assert np.array_equal(fh.decompress(got, w, h), fh.decompress(want, w, h)), k
