@@ -749,6 +749,45 @@ def main():
                                weights=os.path.relpath(args.ranker, ROOT))
     if valu is not None:
         line["valu"] = valu
+    # useful-work count of the annealing (one extra encode per search, not
+    # timed): the reference's QuantizedError point evaluations for this
+    # encode and the GPU's pixel scores, with VALU instructions per
+    # evaluation from the committed counter passes
+    def counted(rk):
+        fastc_amd.set_work_counting(True)
+        try:
+            fastc_amd.compress_device(d_in.data_ptr(), d_out.data_ptr(), out_bytes, W, H,
+                                      quality=args.quality, seed=args.seed, stream=sh, ranker=rk,
+                                      topk=args.topk, threshold=args.threshold)
+            torch.cuda.synchronize()
+            return fastc_amd.work_counts()
+        finally:
+            fastc_amd.set_work_counting(False)
+
+    pc_valu = {}
+    if os.path.exists(prof):
+        with open(prof) as f:
+            pcj = json.load(f)
+        pc_valu["full"] = (pcj.get("full_search_anneal") or {}).get("SQ_INSTS_VALU")
+        pc_valu["learned"] = (pcj.get("valu") or {}).get("SQ_INSTS_VALU")
+    op = {"definition": "anneal_point_evals: QuantizedError point evaluations the reference's "
+                        "annealing makes for this encode (counted on the GPU; equal to the "
+                        "oracle's count on test images); gpu_pixel_scores: pixel scores the "
+                        "GPU's annealing makes (16 per candidate step); valu_*: annealing "
+                        "kernel SQ_INSTS_VALU from " + prof_name + " (not this run) x 64 lanes "
+                        "per count"}
+    for name, rk in (("full", None), ("learned", ranker)):
+        if name == "learned" and ranker is None:
+            continue
+        n_ref, n_gpu = counted(rk)
+        entry = {"anneal_point_evals": n_ref, "gpu_pixel_scores": n_gpu,
+                 "point_evals_per_block": n_ref / nblocks}
+        v = pc_valu.get(name)
+        if v and n_ref:
+            entry["valu_lane_insts_per_point_eval"] = v * 64 / n_ref
+            entry["valu_lane_insts_per_pixel_score"] = v * 64 / n_gpu
+        op[name] = entry
+    line["op_count"] = op
     if not args.no_cpu_baseline:
         cb = cpu_baseline(img, args.quality, args.cpu_tile, args.seed)
         t = args.cpu_tile

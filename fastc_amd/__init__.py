@@ -124,6 +124,10 @@ def lib():
         L.fh_kernel_time.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_uint32)]
         L.fh_kernel_time.restype = ctypes.c_int
+        L.fh_set_work_counting.argtypes = [ctypes.c_int]
+        L.fh_set_work_counting.restype = ctypes.c_int
+        L.fh_bc7_work_counts.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
+        L.fh_bc7_work_counts.restype = ctypes.c_int
         L.fh_bc7_exhaustive_labels.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
                                                ctypes.c_void_p, ctypes.c_void_p]
         L.fh_bc7_exhaustive_labels.restype = ctypes.c_int
@@ -187,7 +191,8 @@ def lib():
 
 EXPORTED_SYMBOLS = ["fh_compress", "fh_compress_device", "fh_decompress",
                     "fh_compressed_size", "fh_last_error", "fh_device_available",
-                    "fh_set_kernel_timing", "fh_kernel_time", "fh_bc7_exhaustive_labels",
+                    "fh_set_kernel_timing", "fh_kernel_time", "fh_set_work_counting",
+                    "fh_bc7_work_counts", "fh_bc7_exhaustive_labels",
                     "fh_ranker_create", "fh_ranker_destroy", "fh_ranker_num_params",
                     "fh_ranker_init", "fh_ranker_set_params", "fh_ranker_get_params",
                     "fh_ranker_forward", "fh_ranker_gradients", "fh_ranker_train",
@@ -354,6 +359,19 @@ def compress_device(in_ptr, out_ptr, out_size, width, height, fmt=eCompressionFo
     j = _job(fmt, in_ptr, out_ptr, out_size, width, height, None, quality, seed, block_modes,
              wm_counter, q0_guard, ranker, topk, threshold, metric)
     _check(lib().fh_compress_device(ctypes.byref(j), ctypes.c_void_p(stream)))
+
+
+def set_work_counting(enable):
+    """Count the BC7 annealing's work (fh_set_work_counting)."""
+    _check(lib().fh_set_work_counting(1 if enable else 0))
+
+
+def work_counts():
+    """(reference QuantizedError point evaluations, GPU pixel scores) of the
+    last BPTC encode on the current device (fh_bc7_work_counts)."""
+    c = (ctypes.c_uint64 * 2)()
+    _check(lib().fh_bc7_work_counts(c))
+    return int(c[0]), int(c[1])
 
 
 def set_kernel_timing(enable):

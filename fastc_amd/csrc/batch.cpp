@@ -125,6 +125,7 @@ hipError_t enqueue(fh_batch *b, hipStream_t s) {
     // kErrorMetrics (Compressor.cpp:205-208)
     const bool nu = b->opt.error_metric == 1;
     p.nonuniform = nu ? 1u : 0u;
+    p.work_counts = nullptr;
     p.metric[0] = nu ? sqrtf(0.3f) : 1.0f;
     p.metric[1] = nu ? sqrtf(0.56f) : 1.0f;
     p.metric[2] = nu ? sqrtf(0.11f) : 1.0f;

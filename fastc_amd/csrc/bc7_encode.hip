@@ -2974,10 +2974,38 @@ constexpr int kTailLanes = 16;
 // NU: the nonuniform metric (float errors; the tail below, which splits a
 // candidate's pixel sums over lanes, would change their order and is not
 // used: the lanes finish their own candidates).
-template <int UNROLL, int WAVES, bool NU = false>
+// COUNT: also count the work into prm.work_counts: [0] the QuantizedError
+// point evaluations the reference's annealing makes (each step of a subset
+// scores that subset's points, Compressor.cpp:583-624), [1] the pixel scores
+// this kernel makes (16 per step of a candidate, whichever subsets are
+// still annealing).
+template <int UNROLL, int WAVES, bool NU = false, bool COUNT = false>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES)))
 anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
   const int lane = threadIdx.x;
+  unsigned long long n_ref = 0, n_gpu = 0;  // COUNT
+  auto count_step = [&](const Lane &x) {
+    // points of the subsets still annealing: the subset codes' low and high
+    // bits give the sizes of subsets 1 and 2
+    const uint32_t lo = x.sub2 & 0x55555555u, hi = (x.sub2 >> 1) & 0x55555555u;
+    const uint32_t n1 = (uint32_t)__popc(lo & ~hi), n2 = (uint32_t)__popc(hi & ~lo);
+    const uint32_t n0 = 16u - n1 - n2;
+    n_ref += ((x.active & 1u) ? n0 : 0u) + ((x.active & 2u) ? n1 : 0u) +
+             ((x.active & 4u) ? n2 : 0u);
+    n_gpu += 16u;
+  };
+  auto flush_counts = [&]() {
+    if constexpr (COUNT) {
+      for (int off = 32; off >= 1; off >>= 1) {
+        n_ref += __shfl_xor(n_ref, off);
+        n_gpu += __shfl_xor(n_gpu, off);
+      }
+      if (lane == 0) {
+        atomicAdd(&prm.work_counts[0], n_ref);
+        atomicAdd(&prm.work_counts[1], n_gpu);
+      }
+    }
+  };
   const uint64_t lt = (1ull << lane) - 1ull;
   const uint32_t total = *(volatile uint32_t *)&w.counters[w.alist ? 18 : 16];
   // per lane 16 texels then 16 points, odd row stride: conflict-free
@@ -3020,6 +3048,7 @@ anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
     if (!__ballot(L.has)) break;
     for (int u = 0; u < UNROLL; u++) {
       if (L.has) {
+        if constexpr (COUNT) count_step(L);
         if (lane_step<false, NU>(L, prm.quality)) {
           lane_finish<NU>(L, w);
           L.has = false;
@@ -3027,7 +3056,10 @@ anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
       }
     }
   }
-  if constexpr (NU) return;
+  if constexpr (NU) {
+    flush_counts();
+    return;
+  }
   // Tail: the wave's last candidates (at most kTailLanes) each get a group
   // of G lanes that share the pixel pass of every step, so the longest
   // anneals, which set the launch's length, run up to G times fewer
@@ -3035,7 +3067,10 @@ anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
   // that loaded them.
   const uint64_t act = __ballot(L.has);
   const int na = __popcll(act);
-  if (na == 0) return;
+  if (na == 0) {
+    flush_counts();
+    return;
+  }
   __shared__ Lane s_tail[kTailLanes];
   __shared__ uint32_t s_row[kTailLanes];
   if (L.has) {
@@ -3056,6 +3091,9 @@ anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
   while (__ballot(T.has)) {
     for (int u = 0; u < UNROLL; u++) {
       if (T.has) {
+        if constexpr (COUNT) {
+          if (m == 0) count_step(T);  // the group scores the 16 pixels once
+        }
         if (lane_step<true>(T, prm.quality, G, m)) {
           if (m == 0) lane_finish(T, w);
           T.has = false;
@@ -3063,6 +3101,7 @@ anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
       }
     }
   }
+  flush_counts();
 }
 
 // ---------------------------------------------------------------------------
@@ -3649,7 +3688,14 @@ hipError_t bc7_upload_tables(int device) {
 
 void launch_anneal(uint32_t grid, const BlockRange &r, const Bc7Params &p, const bc7::Work &w,
                    hipStream_t stream) {
-  if (w.nu_total)  // nonuniform metric (the full search or given selections)
+  if (p.work_counts && !w.nu_total) {  // the counting instances
+    if (w.alist)
+      hipLaunchKernelGGL((bc7::anneal_kernel<4, 1, false, true>), dim3(grid), dim3(64), 0,
+                         stream, r, p, w);
+    else
+      hipLaunchKernelGGL((bc7::anneal_kernel<4, 3, false, true>), dim3(grid), dim3(64), 0,
+                         stream, r, p, w);
+  } else if (w.nu_total)  // nonuniform metric (the full search or given selections)
     hipLaunchKernelGGL((bc7::anneal_kernel<4, 1, true>), dim3(grid), dim3(64), 0, stream, r, p,
                        w);
   else if (w.alist)  // learned search: four steps between refills

@@ -9,6 +9,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -62,8 +63,11 @@ struct DeviceState {
   size_t stats_size = 0;
   void *sel_buf = nullptr;            // fh_compress_with_selection's selections
   size_t sel_size = 0;
+  unsigned long long *work_counts = nullptr;  // fh_bc7_work_counts' two counters
   bool init = false;
 };
+
+std::atomic<int> g_work_counting{0};
 
 std::mutex g_devices_mu;
 std::vector<DeviceState *> g_devices;
@@ -308,6 +312,16 @@ fh_status run_device(DeviceState *st, const fh_job *j, const uint8_t *din, uint8
       p.anneal_unroll = 4;  // annealing steps between refills: 4 measured fastest for both searches
       p.stats = nullptr;
       p.selections = nullptr;
+      p.work_counts = nullptr;
+      if (g_work_counting.load() && !fixed_work && !nonuniform) {
+        if (!st->work_counts) {
+          e = hipMalloc(reinterpret_cast<void **>(&st->work_counts), 16);
+          if (e != hipSuccess) return fail_hip(e, "work counter allocation");
+        }
+        e = fill_u32(reinterpret_cast<uint32_t *>(st->work_counts), 4, 0u, stream);
+        if (e != hipSuccess) return fail_hip(e, "work counter reset");
+        p.work_counts = st->work_counts;
+      }
       // kErrorMetrics (Compressor.cpp:205-208)
       p.nonuniform = nonuniform ? 1u : 0u;
       p.metric[0] = nonuniform ? sqrtf(0.3f) : 1.0f;
@@ -893,6 +907,29 @@ fh_status fh_bc7_candidate_errors(const uint32_t *blocks, uint32_t n, float *err
 }
 
 const char *fh_last_error(void) { return t_last_error.c_str(); }
+
+fh_status fh_set_work_counting(int enable) {
+  g_work_counting.store(enable ? 1 : 0);
+  return FH_OK;
+}
+
+fh_status fh_bc7_work_counts(uint64_t counts[2]) {
+  if (!counts) return fail(FH_ERR_INVALID_ARGUMENT, "null counts");
+  fh_status s;
+  const int dev = current_device(&s);
+  if (dev < 0) return s;
+  DeviceState *st = device_state(dev);
+  std::lock_guard<std::mutex> g(st->mu);
+  counts[0] = counts[1] = 0;
+  if (!st->work_counts) return FH_OK;
+  hipError_t e = hipDeviceSynchronize();
+  unsigned long long c[2] = {0, 0};
+  if (e == hipSuccess) e = hipMemcpy(c, st->work_counts, sizeof(c), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return fail_hip(e, "work counter read");
+  counts[0] = c[0];
+  counts[1] = c[1];
+  return FH_OK;
+}
 
 int fh_device_available(void) {
   int n = 0;

@@ -48,6 +48,9 @@ struct Bc7Params {
   // Compressor.cpp:205-208); not combined with the ranker or statistics
   uint32_t nonuniform;
   float metric[4];
+  // work counting (device words, null = off): the annealing's reference
+  // QuantizedError point evaluations and this encoder's pixel scores
+  unsigned long long *work_counts;
 };
 
 // Device scratch needed by the BC7 watermark prefix scan.

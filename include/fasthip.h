@@ -444,6 +444,19 @@ fh_status fh_batch_blocks(const fh_batch *batch, uint32_t format, uint32_t *bloc
 fh_status fh_set_kernel_timing(int enable);
 fh_status fh_kernel_time(const char *kernel, double *total_ms, uint32_t *launches);
 
+/* Work counting for the BC7 annealing; no reference counterpart (a
+ * diagnostic of the annealing's useful work, for the VALU roofline).  While
+ * enabled, each BPTC encode (fh_compress, fh_compress_device, the selection
+ * entry points; not the no-allocation entry point or the nonuniform metric)
+ * clears and then fills two counters of its device: counts[0] the
+ * QuantizedError point evaluations the reference's annealing makes for the
+ * same encode (every step of a subset scores that subset's points,
+ * Compressor.cpp:583-624), counts[1] the pixel scores the GPU's annealing
+ * makes (16 per step of a candidate).  fh_bc7_work_counts waits for the
+ * device and returns the counters of the calling thread's current device. */
+fh_status fh_set_work_counting(int enable);
+fh_status fh_bc7_work_counts(uint64_t counts[2]);
+
 #ifdef __cplusplus
 }
 #endif

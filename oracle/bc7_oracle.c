@@ -844,6 +844,16 @@ static int accept_new(cmode *m, double new_err, double old_err, float temp) {
   return r < p;
 }
 
+/* QuantizedError point evaluations of the annealing loops run by this
+ * thread (one per point of the subset per step). */
+static _Thread_local uint64_t g_anneal_points;
+
+uint64_t ora_anneal_point_count(int reset) {
+  const uint64_t n = g_anneal_points;
+  if (reset) g_anneal_points = 0;
+  return n;
+}
+
 /* OptimizeEndpointsForCluster (Compressor.cpp:538-630). */
 static double optimize_endpoints(cmode *m, const clus *c, v4 *p1, v4 *p2, uint8_t *best_idx,
                                  uint8_t *best_pbc) {
@@ -873,6 +883,7 @@ static double optimize_endpoints(cmode *m, const clus *c, v4 *p1, v4 *p2, uint8_
     int npbc = 0;
     pick_neighbor(m, *p1, *p2, cur_pbc, &np1, &np2, &npbc, &vp1, &vp2, &vpbc, 1);
     double err = qerr(c, np1, np2, nb, qmask, m_metric(m), m_pbc(m, npbc), idx);
+    g_anneal_points += c->n;
     if (accept_new(m, err, cur, temp)) {
       cur = err;
       *p1 = np1;

@@ -262,6 +262,11 @@ double ora_single_color(int mode, int idx_mode, const float p[4], float p1[4], f
 double ora_single_color_m(int mode, int idx_mode, uint32_t metric, const float p[4],
                           float p1[4], float p2[4], int *pbc);
 uint32_t ora_counter_rng(uint32_t key, uint32_t n);
+/* QuantizedError point evaluations made by the annealing loops
+ * (OptimizeEndpointsForCluster, Compressor.cpp:583-624) on the calling
+ * thread since the last reset: one per point of the annealed subset per
+ * step.  reset != 0 clears the count after reading it. */
+uint64_t ora_anneal_point_count(int reset);
 uint32_t ora_fit_key(uint64_t seed, uint64_t block_id, uint32_t fit_tag);
 
 #ifdef __cplusplus

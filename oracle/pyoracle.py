@@ -149,6 +149,8 @@ def lib():
                                       ctypes.c_uint32, ctypes.POINTER(FitTrace), vp, vp, vp,
                                       vp]
         L.ora_fit_alpha_m.restype = ctypes.c_double
+        L.ora_anneal_point_count.argtypes = [ctypes.c_int]
+        L.ora_anneal_point_count.restype = ctypes.c_uint64
         _lib = L
     return _lib
 

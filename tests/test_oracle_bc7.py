@@ -479,3 +479,21 @@ def test_internal_error_equals_decoded_error(oracle):
                 assert err == sse, (mode, slot, err, sse)
                 checked[mode] += 1
     assert checked[1] == 240 and min(checked.values()) >= 20, checked
+
+
+def test_anneal_point_count(oracle):
+    """The oracle's count of annealing QuantizedError point evaluations
+    (OptimizeEndpointsForCluster, Compressor.cpp:583-624): none at quality 0;
+    with mode 6 alone (one subset of 16 points) a multiple of 16, and at
+    least 16 * q for a block whose fit is not exact (a subset anneals for at
+    least q steps while its best error stays above 0)."""
+    L = oracle.lib()
+    rng = np.random.default_rng(3)
+    px = (rng.integers(0, 2**24, 16) | 0xFF000000).astype(np.uint32)
+    L.ora_anneal_point_count(1)
+    oracle.bc7_compress_block(px, quality=0)
+    assert L.ora_anneal_point_count(1) == 0
+    for q in (1, 7, 30):
+        out, st = oracle.bc7_compress_block(px, quality=q, seed=9, block_modes=0x40)
+        n = L.ora_anneal_point_count(1)
+        assert st.best_error > 0 and n % 16 == 0 and n >= 16 * q, (q, n)
