@@ -738,11 +738,13 @@ __device__ __forceinline__ int nearest_centroid(const float c[NB][4], const floa
 
 // NU: the nonuniform metric, mu its unrotated weights (single colour), mr
 // the weights of this candidate's rotation (QuantizedError).
+// km: this lane's k-means accumulators in LDS (word (j * 5 + k) * 64 of
+// bucket j, k = 4 the count).
 template <int NB, bool NU = false>
 __device__ __forceinline__ void kmeans_setup(const Fit &f, uint32_t q, const uint32_t *sctab,
                              const uint32_t px[16], uint32_t smask, int ptmode, int n,
                              const float avg[4], SubStart &out,
-                             const Weights *mu = nullptr, const Weights *mr = nullptr) {
+                             const Weights *mu, const Weights *mr, float *km) {
   float axis[4];
   principal_axis(px, smask, ptmode, n, avg, axis);
   float mindp = FLT_MAX, maxdp = -FLT_MAX;
@@ -778,76 +780,33 @@ __device__ __forceinline__ void kmeans_setup(const Fit &f, uint32_t q, const uin
   bool fixed = false;
   int guard = 0;
   while (!fixed && guard++ < 4096) {
-    if constexpr (NB <= 8) {
-      float sum[NB][4];
+    // every point into its nearest centroid's LDS sums, in point order, so
+    // each bucket adds its points in the order the reference's per-bucket
+    // loop does (Compressor.cpp:997-1009)
 #pragma unroll
-      for (int j = 0; j < NB; j++) {
-        cnt[j] = 0;
-#pragma unroll
-        for (int k = 0; k < 4; k++) sum[j][k] = 0.0f;
-      }
-      // assign every point to its nearest centroid (first minimum), then
-      // accumulate the buckets in point order
+    for (int j = 0; j < NB * 5; j++) km[j * 64] = 0.0f;
 #pragma unroll 1
-      for (int i = 0; i < 16; i++) {
-        if (!((smask >> i) & 1)) continue;
-        float v[4];
-        unpack4f(point_of(px[i], ptmode), v);
-        const int minb = nearest_centroid<NB>(c, v);
+    for (int i = 0; i < 16; i++) {
+      if (!((smask >> i) & 1)) continue;
+      float v[4];
+      unpack4f(point_of(px[i], ptmode), v);
+      float *b = km + nearest_centroid<NB>(c, v) * 5 * 64;
 #pragma unroll
-        for (int j = 0; j < NB; j++)
-          if (minb == j) {
-            cnt[j]++;
+      for (int k = 0; k < 4; k++) b[k * 64] += v[k];
+      b[4 * 64] += 1.0f;
+    }
+    fixed = true;
 #pragma unroll
-            for (int k = 0; k < 4; k++) sum[j][k] += v[k];
-          }
-      }
-      fixed = true;
+    for (int j = 0; j < NB; j++) {
+      const float cn = km[(j * 5 + 4) * 64];
+      float s[4];
 #pragma unroll
-      for (int j = 0; j < NB; j++) {
-        float s[4];
+      for (int k = 0; k < 4; k++) s[k] = cn != 0.0f ? km[(j * 5 + k) * 64] / cn : 0.0f;
+      if (s[0] != c[j][0] || s[1] != c[j][1] || s[2] != c[j][2] || s[3] != c[j][3])
+        fixed = false;
 #pragma unroll
-        for (int k = 0; k < 4; k++) s[k] = cnt[j] != 0 ? sum[j][k] / (float)cnt[j] : 0.0f;
-        if (s[0] != c[j][0] || s[1] != c[j][1] || s[2] != c[j][2] || s[3] != c[j][3])
-          fixed = false;
-#pragma unroll
-        for (int k = 0; k < 4; k++) c[j][k] = s[k];
-      }
-    } else {
-      // 16 buckets: assignments first (4 bits per point), then one bucket
-      // at a time, so that no second set of 16 sums is live
-      uint64_t bucket = 0;
-#pragma unroll 1
-      for (int i = 0; i < 16; i++) {
-        if (!((smask >> i) & 1)) continue;
-        float v[4];
-        unpack4f(point_of(px[i], ptmode), v);
-        bucket |= (uint64_t)nearest_centroid<NB>(c, v) << (4 * i);
-      }
-      fixed = true;
-#pragma unroll
-      for (int j = 0; j < NB; j++) {
-        uint32_t cn = 0;
-        float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll 1
-        for (int i = 0; i < 16; i++) {
-          if (!((smask >> i) & 1) || ((bucket >> (4 * i)) & 15) != (uint64_t)j) continue;
-          float v[4];
-          unpack4f(point_of(px[i], ptmode), v);
-          cn++;
-#pragma unroll
-          for (int k = 0; k < 4; k++) s[k] += v[k];
-        }
-        if (cn != 0) {
-#pragma unroll
-          for (int k = 0; k < 4; k++) s[k] = s[k] / (float)cn;
-        }
-        if (s[0] != c[j][0] || s[1] != c[j][1] || s[2] != c[j][2] || s[3] != c[j][3])
-          fixed = false;
-#pragma unroll
-        for (int k = 0; k < 4; k++) c[j][k] = s[k];
-        cnt[j] = cn;
-      }
+      for (int k = 0; k < 4; k++) c[j][k] = s[k];
+      cnt[j] = (uint32_t)cn;
     }
   }
   int nfilled = 0, last = 0;
@@ -959,7 +918,7 @@ template <bool NU = false>
 __device__ __forceinline__ void subset_setup(const Fit &f, uint32_t q, const uint32_t *sctab,
                              const uint32_t px[16], uint32_t smask, int ptmode, int n,
                              const float avg[4], bool all_same, SubStart &out,
-                             const Weights *mu = nullptr, const Weights *mr = nullptr) {
+                             const Weights *mu, const Weights *mr, float *km) {
   if (all_same) {
     const int first = __ffs(smask) - 1;
     uint32_t pt = 0;
@@ -976,9 +935,12 @@ __device__ __forceinline__ void subset_setup(const Fit &f, uint32_t q, const uin
     out.anneal = false;
     return;
   }
-  if (f.nbits_idx == 2) kmeans_setup<4, NU>(f, q, sctab, px, smask, ptmode, n, avg, out, mu, mr);
-  else if (f.nbits_idx == 3) kmeans_setup<8, NU>(f, q, sctab, px, smask, ptmode, n, avg, out, mu, mr);
-  else kmeans_setup<16, NU>(f, q, sctab, px, smask, ptmode, n, avg, out, mu, mr);
+  if (f.nbits_idx == 2)
+    kmeans_setup<4, NU>(f, q, sctab, px, smask, ptmode, n, avg, out, mu, mr, km);
+  else if (f.nbits_idx == 3)
+    kmeans_setup<8, NU>(f, q, sctab, px, smask, ptmode, n, avg, out, mu, mr, km);
+  else
+    kmeans_setup<16, NU>(f, q, sctab, px, smask, ptmode, n, avg, out, mu, mr, km);
 }
 
 // The alpha channel of the rotation modes (Compressor.cpp:697-911): a 1-D
@@ -2444,8 +2406,8 @@ template <int MODE, bool NU = false>
 __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t px[16],
                                                   uint32_t quality, const uint32_t *sctab,
                                                   CandRec &rec, AState &st, uint32_t &total,
-                                                  const Weights *mu = nullptr,
-                                                  double *tot_nu = nullptr) {
+                                                  const Weights *mu, double *tot_nu,
+                                                  float *km) {
   Fit f;
   const ModeAttr attr = MODE >= 0 ? kModeTab[MODE < 0 ? 0 : MODE] : kModes[k.mode];
   const int nsub = attr.subsets;
@@ -2480,7 +2442,7 @@ __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t 
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) avg[kk] = avg[kk] / 16.0f;
     SubStart s0;
-    subset_setup<NU>(f, quality, sctab, px, 0xFFFFu, rot + 1, 16, avg, false, s0, mu, &mr);
+    subset_setup<NU>(f, quality, sctab, px, 0xFFFFu, rot + 1, 16, avg, false, s0, mu, &mr, km);
     rec.aidx_lo = (uint32_t)aidx;
     rec.aidx_hi = (uint32_t)(aidx >> 32);
     // Pack writes the RGB of the fit and the fitted alpha, no p-bit
@@ -2526,7 +2488,7 @@ __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t 
 #pragma unroll
       for (int kk = 0; kk < 4; kk++) avg[kk] = avg[kk] / (float)n;
       SubStart ss;
-      subset_setup<NU>(f, quality, sctab, px, smask, 0, n, avg, same, ss, mu, mu);
+      subset_setup<NU>(f, quality, sctab, px, smask, 0, n, avg, same, ss, mu, mu, km);
       if constexpr (NU) {
         tnu += ss.sc ? (double)n * (double)__uint_as_float(ss.err)
                      : (double)__uint_as_float(ss.err);
@@ -2570,11 +2532,19 @@ __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t 
 // each mode gets its own register allocation.
 constexpr int kPxStride = 17;  // odd stride: lanes reading texel i hit distinct banks
 
+// Index entries of the widest k-means a mode's set-up runs (the colour
+// indices; mode 4 with index mode 1 has 3-bit colour indices).
+constexpr int kSetupBuckets(int mode) {
+  return mode == 6 ? 16 : ((mode == 0 || mode == 1 || mode == 4) ? 8 : 4);
+}
+
 template <int MODE, bool NU = false>
 __global__ void __launch_bounds__(64) setup_kernel(BlockRange r, Bc7Params prm, Work w,
                                                    const uint32_t *sctab) {
   const int lane = threadIdx.x;
   __shared__ uint32_t s_px[64 * kPxStride];
+  // the k-means sums and counts, per bucket and channel a row of 64 lanes
+  __shared__ float s_km[kSetupBuckets(MODE) * 5 * 64];
   {
     constexpr int kTypeOf[8] = {0, 2, 1, 3, 6, 7, 5, 4};
     const int t = kTypeOf[MODE];
@@ -2608,7 +2578,8 @@ __global__ void __launch_bounds__(64) setup_kernel(BlockRange r, Bc7Params prm, 
       Weights mu;
       if constexpr (NU) mu = params_weights(prm);
       const uint32_t active =
-          fit_candidate<MODE, NU>(k, px, prm.quality, sctab, rec, st, total, &mu, &tnu);
+          fit_candidate<MODE, NU>(k, px, prm.quality, sctab, rec, st, total, &mu, &tnu,
+                                  s_km + lane);
       st.id = id;
       w.rec[(size_t)rel * kMaxCands + c] = rec;
       if constexpr (NU) w.nu_total[(size_t)rel * kMaxCands + c] = tnu;
@@ -3503,6 +3474,7 @@ __global__ void __launch_bounds__(64) label_kernel(BlockRange r, int mode, uint3
                                                    const uint32_t *sctab,
                                                    unsigned long long *best, float *errs) {
   __shared__ uint32_t s_px[64 * kPxStride];
+  __shared__ float s_km[16 * 5 * 64];  // k-means sums and counts (up to 16 buckets)
   const uint64_t items = (uint64_t)r.num_blocks * nslots;
   for (uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; item < items;
        item += (uint64_t)gridDim.x * blockDim.x) {
@@ -3520,7 +3492,8 @@ __global__ void __launch_bounds__(64) label_kernel(BlockRange r, int mode, uint3
     CandRec rec;
     AState st;
     uint32_t total = 0;
-    fit_candidate<-1>(k, px, 0, sctab, rec, st, total);
+    fit_candidate<-1>(k, px, 0, sctab, rec, st, total, nullptr, nullptr,
+                      s_km + threadIdx.x);
     BitWriter bw;
     pack_mode(k.mode, k.shape, rec, bw);
     const uint4 blk = make_uint4((uint32_t)bw.lo, (uint32_t)(bw.lo >> 32), (uint32_t)bw.hi,
