@@ -2936,6 +2936,10 @@ __device__ void lane_finish(Lane &L, Work &w) {
 // A wave with at most this many candidates left after the list is used up
 // finishes them in lane groups.
 constexpr int kTailLanes = 16;
+// Lanes per candidate for the rotation modes (4 and 5), whose annealing
+// chains are the longest (hundreds to a few thousand steps against tens for
+// the other modes): run one lane each, they would set the launch's length.
+constexpr int kRotGroup = 4;
 
 // UNROLL annealing steps run between refills of finished lanes.
 // WAVES: the minimum waves per SIMD the register allocation must allow.  3
@@ -2978,9 +2982,87 @@ anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
     }
   };
   const uint64_t lt = (1ull << lane) - 1ull;
-  const uint32_t total = *(volatile uint32_t *)&w.counters[w.alist ? 18 : 16];
+  const uint32_t listed = *(volatile uint32_t *)&w.counters[w.alist ? 18 : 16];
   // per lane 16 texels then 16 points, odd row stride: conflict-free
   __shared__ uint32_t s_px[64 * 33];
+  // Learned search: the list is in state order, and the set-up kernels
+  // append the rotation modes' states last (counters[19] holds the first of
+  // them): the list's rotation states are its suffix from rot_begin.  (The
+  // full search's list is mostly rotation states, whose grouped steps would
+  // cost more than its short tail: it runs them one lane each.)
+  uint32_t rot_begin = listed;
+  if (!NU && w.alist) {
+    const uint32_t first_rot = *(volatile uint32_t *)&w.counters[19];
+    uint32_t lo = 0, hi = listed;  // first list entry >= first_rot
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (w.alist[mid] < first_rot) lo = mid + 1;
+      else hi = mid;
+    }
+    rot_begin = lo;
+  }
+  // Phase A, in the first eighth of the launch (a quarter of the waves that
+  // fit at once): the rotation states in
+  // groups of kRotGroup lanes that share each step's pixel pass (every
+  // member holds the same state and texels), fetched from the end of the
+  // suffix.  A group that finishes takes the next; once the suffix is used
+  // up the wave's groups finish theirs and the wave joins phase B.  The
+  // other waves start on phase B at once, so the longest chains run from
+  // the start at a quarter of a lane's instructions per step while the rest
+  // of the list proceeds beside them.
+  if (!NU && rot_begin < listed && blockIdx.x < (gridDim.x + 7) / 8) {
+    const uint32_t nrot = listed - rot_begin;
+    const int m = lane % kRotGroup;
+    static_assert(kRotGroup == 4, "the group leaders' mask");
+    const uint64_t leaders = 0x1111111111111111ull;
+    Lane T;
+    T.px = s_px + lane * 33;
+    T.pt = T.px + 16;
+    T.has = false;
+    uint32_t abase = 0, aend = 0;
+    bool aexh = false;
+    for (;;) {
+      for (;;) {
+        const uint64_t need = __ballot(!T.has) & leaders;
+        if (!need || aexh) break;
+        if (abase >= aend) {
+          uint32_t b0 = 0;
+          if (lane == 0) b0 = atomicAdd(&w.counters[21], 16u);
+          b0 = __shfl(b0, 0);
+          if (b0 >= nrot) {
+            aexh = true;
+            break;
+          }
+          abase = b0;
+          aend = min(b0 + 16u, nrot);
+        }
+        const uint32_t avail = aend - abase;
+        // rank of this lane's group among the groups that need a state
+        const uint64_t below = (1ull << (lane & ~(kRotGroup - 1))) - 1ull;
+        const uint32_t rank = (uint32_t)__popcll(need & below);
+        const bool wants = ((need >> (lane & ~(kRotGroup - 1))) & 1) != 0;
+        if (wants && !T.has && rank < avail) {
+          const uint32_t item = listed - 1u - (abase + rank);
+          lane_load<NU>(T, r, prm, w.astate[w.alist ? w.alist[item] : item]);
+        }
+        abase += min(avail, (uint32_t)__popcll(need));
+      }
+      if (!__ballot(T.has)) break;
+      for (int u = 0; u < UNROLL; u++) {
+        if (T.has) {
+          if constexpr (COUNT) {
+            if (m == 0) count_step(T);
+          }
+          if (lane_step<true>(T, prm.quality, kRotGroup, m)) {
+            if (m == 0) lane_finish(T, w);
+            T.has = false;
+          }
+        }
+      }
+    }
+  }
+  // Phase B: the other states, one lane each, from the end of the prefix
+  const uint32_t total = rot_begin;
   Lane L;
   L.px = s_px + lane * 33;
   L.pt = L.px + 16;
@@ -3009,8 +3091,10 @@ anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
       const uint32_t rank = (uint32_t)__popcll(need & lt);
       if (!L.has && rank < avail) {
         // the list runs from its end: the set-up kernels append modes 0, 2,
-        // 1, 3, 7, 6, 4, 5, and the rotation modes' candidates anneal the
-        // longest, so starting with them keeps the tail of the launch short
+        // 3, 7, 6, 1, 4, 5.  The rotation modes' candidates anneal the
+        // longest, so the full search starts with them; then come the
+        // numerous mode-1 candidates, and the short mode-7 chains are among
+        // the last loaded, which keeps the tail of the launch short
         const uint32_t item = total - 1u - (wbase + rank);
         lane_load<NU>(L, r, prm, w.astate[w.alist ? w.alist[item] : item]);
       }
@@ -3553,6 +3637,9 @@ __global__ void __launch_bounds__(256) fill_kernel(uint32_t *p, uint32_t n, uint
   if (i < n) p[i] = v;
 }
 
+// The annealing states appended so far (before the rotation modes' set-up).
+__global__ void mark_kernel(uint32_t *counters) { counters[19] = counters[16]; }
+
 __global__ void reset_kernel(uint32_t *counters, int n) {
   const int i = threadIdx.x;
   if (i < n) counters[i] = 0;
@@ -3832,13 +3919,12 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
     timing_end(t, kTagBc7Select, stream);
   }
   t = timing_begin(stream);
-  // queue (type) order: modes 0, 2, 1, 3, 7, 6, 4, 5
+  // annealing-state order: modes 0, 2, 3, 7, 6, 1, then the rotation modes 4
+  // and 5 (the mark between them records where they begin)
   if (nu) {
     hipLaunchKernelGGL((bc7::setup_kernel<0, true>), dim3(persistent), dim3(64), 0, stream, r,
                        p, w, sct);
     hipLaunchKernelGGL((bc7::setup_kernel<2, true>), dim3(persistent), dim3(64), 0, stream, r,
-                       p, w, sct);
-    hipLaunchKernelGGL((bc7::setup_kernel<1, true>), dim3(persistent), dim3(64), 0, stream, r,
                        p, w, sct);
     hipLaunchKernelGGL((bc7::setup_kernel<3, true>), dim3(persistent), dim3(64), 0, stream, r,
                        p, w, sct);
@@ -3846,6 +3932,9 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
                        p, w, sct);
     hipLaunchKernelGGL((bc7::setup_kernel<6, true>), dim3(persistent), dim3(64), 0, stream, r,
                        p, w, sct);
+    hipLaunchKernelGGL((bc7::setup_kernel<1, true>), dim3(persistent), dim3(64), 0, stream, r,
+                       p, w, sct);
+    hipLaunchKernelGGL(bc7::mark_kernel, dim3(1), dim3(1), 0, stream, w.counters);
     hipLaunchKernelGGL((bc7::setup_kernel<4, true>), dim3(persistent), dim3(64), 0, stream, r,
                        p, w, sct);
     hipLaunchKernelGGL((bc7::setup_kernel<5, true>), dim3(persistent), dim3(64), 0, stream, r,
@@ -3853,10 +3942,11 @@ hipError_t bc7_encode(const BlockRange &r, const Bc7Params &p, const Bc7Scratch 
   } else {
     hipLaunchKernelGGL(bc7::setup_kernel<0>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
     hipLaunchKernelGGL(bc7::setup_kernel<2>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
-    hipLaunchKernelGGL(bc7::setup_kernel<1>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
     hipLaunchKernelGGL(bc7::setup_kernel<3>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
     hipLaunchKernelGGL(bc7::setup_kernel<7>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
     hipLaunchKernelGGL(bc7::setup_kernel<6>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+    hipLaunchKernelGGL(bc7::setup_kernel<1>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
+    hipLaunchKernelGGL(bc7::mark_kernel, dim3(1), dim3(1), 0, stream, w.counters);
     hipLaunchKernelGGL(bc7::setup_kernel<4>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
     hipLaunchKernelGGL(bc7::setup_kernel<5>, dim3(persistent), dim3(64), 0, stream, r, p, w, sct);
   }
