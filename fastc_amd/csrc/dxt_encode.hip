@@ -26,8 +26,11 @@ struct Tables {
   uint8_t expand6[64];
   uint8_t omatch5[256][2];
   uint8_t omatch6[256][2];
-  uint8_t quant_rb[256 + 16];
-  uint8_t quant_g[256 + 16];
+  // stb__QuantRBTab / QuantGTab over the dither's whole index range
+  // [-256, 527], offset by 256: quant[i + 256] = the 5-6-bit quantisation of
+  // clamp(i - 8, 0, 255), expanded back to 8 bits
+  uint8_t quant_rb[784];
+  uint8_t quant_g[784];
 };
 
 __constant__ Tables kT;
@@ -42,6 +45,17 @@ __device__ __forceinline__ uint32_t as16bit(int r, int g, int b) {
 }
 
 __device__ __forceinline__ int ch(uint32_t p, int k) { return (int)((p >> (8 * k)) & 0xFFu); }
+
+typedef short short2_t __attribute__((ext_vector_type(2)));
+
+// r * vr + g * vg + b * vb for texel p, exactly: red and green as a pair of
+// 16-bit lanes in one two-way dot product (|v| < 2^15)
+__device__ __forceinline__ int dot3(uint32_t p, int vr, int vg, int vb) {
+  const uint32_t rg = __builtin_amdgcn_perm(0u, p, 0x0c010c00u);  // r, 0, g, 0
+  const uint32_t w = ((uint32_t)vr & 0xFFFFu) | ((uint32_t)vg << 16);
+  return __builtin_amdgcn_sdot2(__builtin_bit_cast(short2_t, rg), __builtin_bit_cast(short2_t, w),
+                                ch(p, 2) * vb, false);
+}
 
 // stb__EvalColors: color[0] = max16, color[1] = min16, color[2..3] = thirds.
 __device__ __forceinline__ void eval_colors(int col[4][3], uint32_t c0, uint32_t c1) {
@@ -58,13 +72,22 @@ __device__ __forceinline__ void eval_colors(int col[4][3], uint32_t c0, uint32_t
   }
 }
 
-__device__ __forceinline__ int qtab(int ch_, int idx) {
-  idx = idx < 0 ? 0 : (idx > 271 ? 271 : idx);
-  return ch_ == 1 ? kT.quant_g[idx] : kT.quant_rb[idx];
+// The kernels copy the two tables into LDS (q: red/blue then green, 784
+// bytes each), where the lookups of the dither's serial chain are short.
+__device__ __forceinline__ int qtab(const uint8_t *q, int ch_, int idx) {
+  return q[(ch_ == 1 ? 784 : 0) + idx + 256];
+}
+
+__device__ __forceinline__ void load_quant(uint8_t *q) {
+  for (int i = (int)threadIdx.x; i < 784; i += (int)blockDim.x) {
+    q[i] = kT.quant_rb[i];
+    q[784 + i] = kT.quant_g[i];
+  }
+  __syncthreads();
 }
 
 // stb__DitherBlock
-__device__ void dither_block(const uint32_t blk[16], uint32_t dst[16]) {
+__device__ void dither_block(const uint32_t blk[16], uint32_t dst[16], const uint8_t *q) {
 #pragma unroll
   for (int i = 0; i < 16; i++) dst[i] = 0;
 #pragma unroll
@@ -75,13 +98,13 @@ __device__ void dither_block(const uint32_t blk[16], uint32_t dst[16]) {
       const int b0 = ch(blk[y * 4 + 0], c), b1 = ch(blk[y * 4 + 1], c);
       const int b2 = ch(blk[y * 4 + 2], c), b3 = ch(blk[y * 4 + 3], c);
       // ep1 = e1 (current row), ep2 = e2 (previous row)
-      const int d0 = qtab(c, b0 + ((3 * e2[1] + 5 * e2[0]) >> 4) + 8);
+      const int d0 = qtab(q, c, b0 + ((3 * e2[1] + 5 * e2[0]) >> 4) + 8);
       e1[0] = b0 - d0;
-      const int d1 = qtab(c, b1 + ((7 * e1[0] + 3 * e2[2] + 5 * e2[1] + e2[0]) >> 4) + 8);
+      const int d1 = qtab(q, c, b1 + ((7 * e1[0] + 3 * e2[2] + 5 * e2[1] + e2[0]) >> 4) + 8);
       e1[1] = b1 - d1;
-      const int d2 = qtab(c, b2 + ((7 * e1[1] + 3 * e2[3] + 5 * e2[2] + e2[1]) >> 4) + 8);
+      const int d2 = qtab(q, c, b2 + ((7 * e1[1] + 3 * e2[3] + 5 * e2[2] + e2[1]) >> 4) + 8);
       e1[2] = b2 - d2;
-      const int d3 = qtab(c, b3 + ((7 * e1[2] + 5 * e2[3] + e2[2]) >> 4) + 8);
+      const int d3 = qtab(q, c, b3 + ((7 * e1[2] + 5 * e2[3] + e2[2]) >> 4) + 8);
       e1[3] = b3 - d3;
       dst[y * 4 + 0] |= (uint32_t)d0 << (8 * c);
       dst[y * 4 + 1] |= (uint32_t)d1 << (8 * c);
@@ -97,9 +120,19 @@ __device__ void dither_block(const uint32_t blk[16], uint32_t dst[16]) {
   }
 }
 
-__device__ __forceinline__ int pick_step(int dot, int half, int c0p, int c3p) {
-  if (dot < half) return dot < c0p ? 1 : 3;
-  return dot < c3p ? 2 : 0;
+// One dithered index choice of stb__MatchColorsBlock: dot is the pixel's
+// projection (times 16, error feedback added).  The index is 1 or 3 below the
+// half-way point and 2 or 0 above it; it goes to mask at bit pos.  Returns the
+// pixel's projection minus that of its palette entry (s0..s3: the entries'
+// projections).
+__device__ __forceinline__ int match_step(int dp, int dot, int half, int c0p, int c3p, int s0,
+                                          int s1, int s2, int s3, uint32_t &mask, int pos) {
+  const bool below = dot < c0p, above = dot < c3p;
+  const int t_lo = below ? s1 : s3, t_hi = above ? s2 : s0;
+  const uint32_t i_lo = below ? 1u : 3u, i_hi = above ? 2u : 0u;
+  const bool lo = dot < half;
+  mask |= (lo ? i_lo : i_hi) << pos;
+  return dp - (lo ? t_lo : t_hi);
 }
 
 // stb__MatchColorsBlock with dithering.
@@ -109,7 +142,7 @@ __device__ uint32_t match_colors(const uint32_t blk[16], const int col[4][3]) {
   const int dirb = col[0][2] - col[1][2];
   int dots[16], stops[4];
 #pragma unroll
-  for (int i = 0; i < 16; i++) dots[i] = ch(blk[i], 0) * dirr + ch(blk[i], 1) * dirg + ch(blk[i], 2) * dirb;
+  for (int i = 0; i < 16; i++) dots[i] = dot3(blk[i], dirr, dirg, dirb);
 #pragma unroll
   for (int i = 0; i < 4; i++) stops[i] = col[i][0] * dirr + col[i][1] * dirg + col[i][2] * dirb;
   const int c0p = ((stops[1] + stops[3]) >> 1) << 4;
@@ -120,25 +153,15 @@ __device__ uint32_t match_colors(const uint32_t blk[16], const int col[4][3]) {
 #pragma unroll
   for (int y = 0; y < 4; y++) {
     const int *dp = &dots[y * 4];
-    int st;
-    uint32_t lm;
-    int dot = (dp[0] << 4) + (3 * e2[1] + 5 * e2[0]);
-    st = pick_step(dot, half, c0p, c3p);
-    e1[0] = dp[0] - stops[st];
-    lm = (uint32_t)st;
-    dot = (dp[1] << 4) + (7 * e1[0] + 3 * e2[2] + 5 * e2[1] + e2[0]);
-    st = pick_step(dot, half, c0p, c3p);
-    e1[1] = dp[1] - stops[st];
-    lm |= (uint32_t)st << 2;
-    dot = (dp[2] << 4) + (7 * e1[1] + 3 * e2[3] + 5 * e2[2] + e2[1]);
-    st = pick_step(dot, half, c0p, c3p);
-    e1[2] = dp[2] - stops[st];
-    lm |= (uint32_t)st << 4;
-    dot = (dp[3] << 4) + (7 * e1[2] + 5 * e2[3] + e2[2]);
-    st = pick_step(dot, half, c0p, c3p);
-    e1[3] = dp[3] - stops[st];
-    lm |= (uint32_t)st << 6;
-    mask |= lm << (y * 8);
+    const int p = y * 8;
+    e1[0] = match_step(dp[0], (dp[0] << 4) + (3 * e2[1] + 5 * e2[0]), half, c0p, c3p, stops[0],
+                       stops[1], stops[2], stops[3], mask, p);
+    e1[1] = match_step(dp[1], (dp[1] << 4) + (7 * e1[0] + 3 * e2[2] + 5 * e2[1] + e2[0]), half,
+                       c0p, c3p, stops[0], stops[1], stops[2], stops[3], mask, p + 2);
+    e1[2] = match_step(dp[2], (dp[2] << 4) + (7 * e1[1] + 3 * e2[3] + 5 * e2[2] + e2[1]), half,
+                       c0p, c3p, stops[0], stops[1], stops[2], stops[3], mask, p + 4);
+    e1[3] = match_step(dp[3], (dp[3] << 4) + (7 * e1[2] + 5 * e2[3] + e2[2]), half, c0p, c3p,
+                       stops[0], stops[1], stops[2], stops[3], mask, p + 6);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       const int t = e1[k];
@@ -159,29 +182,47 @@ __device__ void optimize_colors(const uint32_t blk[16], uint32_t &max16, uint32_
     for (int i = 1; i < 16; i++) {
       const int v = ch(blk[i], c);
       muv += v;
-      if (v < minv) minv = v;
-      else if (v > maxv) maxv = v;
+      // stb_dxt's "if (v < min) ... else if (v > max)" with min <= max
+      // throughout is the plain running minimum and maximum
+      minv = min(minv, v);
+      maxv = max(maxv, v);
     }
     mu[c] = (muv + 8) >> 4;
     mn[c] = minv;
     mx[c] = maxv;
   }
-  int cov[6] = {0, 0, 0, 0, 0, 0};
+  // sum (x - mx)(y - my) = sum xy - my sum x - mx sum y + 16 mx my, exactly
+  // in integers: the products are of bytes
+  int sp[6] = {0, 0, 0, 0, 0, 0}, sx[3] = {0, 0, 0};
 #pragma unroll
   for (int i = 0; i < 16; i++) {
-    const int r = ch(blk[i], 0) - mu[0];
-    const int g = ch(blk[i], 1) - mu[1];
-    const int b = ch(blk[i], 2) - mu[2];
-    cov[0] += r * r;
-    cov[1] += r * g;
-    cov[2] += r * b;
-    cov[3] += g * g;
-    cov[4] += g * b;
-    cov[5] += b * b;
+    const int r = ch(blk[i], 0), g = ch(blk[i], 1), b = ch(blk[i], 2);
+    sp[0] += r * r;
+    sp[1] += r * g;
+    sp[2] += r * b;
+    sp[3] += g * g;
+    sp[4] += g * b;
+    sp[5] += b * b;
+    sx[0] += r;
+    sx[1] += g;
+    sx[2] += b;
   }
+  int cov[6];
+  {
+    const int pr[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const int x = pr[k][0], y = pr[k][1];
+      cov[k] = sp[k] - mu[y] * sx[x] - mu[x] * sx[y] + 16 * mu[x] * mu[y];
+    }
+  }
+  // cov / 255.0f, correctly rounded: |cov| < 2^24, and cov / 255 is an
+  // integer or not a dyadic rational, at least 2^-33 (relative) from any
+  // float rounding boundary, so the double product (within 2^-52 of it)
+  // rounds to the same float
   float covf[6];
 #pragma unroll
-  for (int i = 0; i < 6; i++) covf[i] = (float)cov[i] / 255.0f;
+  for (int i = 0; i < 6; i++) covf[i] = (float)((double)cov[i] * (1.0 / 255.0));
   float vfr = (float)(mx[0] - mn[0]);
   float vfg = (float)(mx[1] - mn[1]);
   float vfb = (float)(mx[2] - mn[2]);
@@ -208,11 +249,16 @@ __device__ void optimize_colors(const uint32_t blk[16], uint32_t &max16, uint32_
     v_g = (int)((double)vfg * magn);
     v_b = (int)((double)vfb * magn);
   }
+  // |v| <= 587 already: the clamp only tells the compiler that 24-bit
+  // multiplies are exact
+  v_r = min(max(v_r, -1024), 1024);
+  v_g = min(max(v_g, -1024), 1024);
+  v_b = min(max(v_b, -1024), 1024);
   int mind = 0x7fffffff, maxd = -0x7fffffff;
   uint32_t minp = blk[0], maxp = blk[0];
 #pragma unroll
   for (int i = 0; i < 16; i++) {
-    const int dot = ch(blk[i], 0) * v_r + ch(blk[i], 1) * v_g + ch(blk[i], 2) * v_b;
+    const int dot = dot3(blk[i], v_r, v_g, v_b);
     if (dot < mind) {
       mind = dot;
       minp = blk[i];
@@ -250,37 +296,39 @@ __device__ bool refine_block(const uint32_t blk[16], uint32_t &max16, uint32_t &
     nmax = ((uint32_t)kT.omatch5[r][0] << 11) | ((uint32_t)kT.omatch6[g][0] << 5) | kT.omatch5[b][0];
     nmin = ((uint32_t)kT.omatch5[r][1] << 11) | ((uint32_t)kT.omatch6[g][1] << 5) | kT.omatch5[b][1];
   } else {
-    const int w1tab[4] = {3, 0, 2, 1};
-    const int prods[4] = {0x090000, 0x000900, 0x040102, 0x010402};
-    int akku = 0, a1r = 0, a1g = 0, a1b = 0, a2r = 0, a2g = 0, a2b = 0;
-    uint32_t cm = mask;
+    // index k's weights are w1 = 3, 0, 2, 1 (k = 0..3) and w2 = 3 - w1:
+    // the sums of w1*w1, w2*w2 and w1*w2 follow from the index counts
+    // (stb_dxt accumulates them as the bytes of one word, which never carry)
+    const uint32_t lo = mask & 0x55555555u, hi = (mask >> 1) & 0x55555555u;
+    const int n1 = __popc(lo & ~hi), n2 = __popc(hi & ~lo), n3 = __popc(lo & hi);
+    const int n0 = 16 - n1 - n2 - n3;
+    const int xx = 9 * n0 + 4 * n2 + n3;
+    const int yy = 9 * n1 + n2 + 4 * n3;
+    const int xy = 2 * (n2 + n3);
+    // red and blue in the two 16-bit halves of one word: each half's sum
+    // stays below 3 * 16 * 255 < 2^16, and r | b << 16 < 2^24
+    uint32_t a1rb = 0, a2rb = 0, a1g = 0, a2g = 0;
 #pragma unroll
-    for (int i = 0; i < 16; i++, cm >>= 2) {
-      const int step = (int)(cm & 3);
-      const int w1 = w1tab[step];
-      const int r = ch(blk[i], 0), g = ch(blk[i], 1), b = ch(blk[i], 2);
-      akku += prods[step];
-      a1r += w1 * r;
+    for (int i = 0; i < 16; i++) {
+      const uint32_t w1 = (0x1203u >> (((mask >> (2 * i)) & 3u) * 4u)) & 15u;
+      const uint32_t rb = blk[i] & 0x00FF00FFu, g = (blk[i] >> 8) & 0xFFu;
+      a1rb += w1 * rb;
       a1g += w1 * g;
-      a1b += w1 * b;
-      a2r += r;
+      a2rb += rb;
       a2g += g;
-      a2b += b;
     }
-    a2r = 3 * a2r - a1r;
-    a2g = 3 * a2g - a1g;
-    a2b = 3 * a2b - a1b;
-    const int xx = akku >> 16;
-    const int yy = (akku >> 8) & 0xff;
-    const int xy = (akku >> 0) & 0xff;
+    const int a1r = (int)(a1rb & 0xFFFFu), a1b = (int)(a1rb >> 16), a1gi = (int)a1g;
+    const int a2r = 3 * (int)(a2rb & 0xFFFFu) - a1r;
+    const int a2gi = 3 * (int)a2g - a1gi;
+    const int a2b = 3 * (int)(a2rb >> 16) - a1b;
     // 3.0f * 31.0f / 255.0f is evaluated left to right in float
     const float frb = ((3.0f * 31.0f) / 255.0f) / (float)(xx * yy - xy * xy);
     const float fg = (frb * 63.0f) / 31.0f;
     nmax = (uint32_t)sclamp((float)(a1r * yy - a2r * xy) * frb + 0.5f, 0, 31) << 11;
-    nmax |= (uint32_t)sclamp((float)(a1g * yy - a2g * xy) * fg + 0.5f, 0, 63) << 5;
+    nmax |= (uint32_t)sclamp((float)(a1gi * yy - a2gi * xy) * fg + 0.5f, 0, 63) << 5;
     nmax |= (uint32_t)sclamp((float)(a1b * yy - a2b * xy) * frb + 0.5f, 0, 31);
     nmin = (uint32_t)sclamp((float)(a2r * xx - a1r * xy) * frb + 0.5f, 0, 31) << 11;
-    nmin |= (uint32_t)sclamp((float)(a2g * xx - a1g * xy) * fg + 0.5f, 0, 63) << 5;
+    nmin |= (uint32_t)sclamp((float)(a2gi * xx - a1gi * xy) * fg + 0.5f, 0, 63) << 5;
     nmin |= (uint32_t)sclamp((float)(a2b * xx - a1b * xy) * frb + 0.5f, 0, 31);
   }
   min16 = nmin;
@@ -289,7 +337,7 @@ __device__ bool refine_block(const uint32_t blk[16], uint32_t &max16, uint32_t &
 }
 
 // stb__CompressColorBlock in STB_DXT_DITHER mode (one refinement pass).
-__device__ uint2 compress_color(const uint32_t blk[16]) {
+__device__ uint2 compress_color(const uint32_t blk[16], const uint8_t *q) {
   uint32_t mask, max16, min16;
   bool constant = true;
 #pragma unroll
@@ -301,7 +349,7 @@ __device__ uint2 compress_color(const uint32_t blk[16]) {
     min16 = ((uint32_t)kT.omatch5[r][1] << 11) | ((uint32_t)kT.omatch6[g][1] << 5) | kT.omatch5[b][1];
   } else {
     uint32_t dblk[16];
-    dither_block(blk, dblk);
+    dither_block(blk, dblk, q);
     optimize_colors(dblk, max16, min16);
     int col[4][3];
     if (max16 != min16) {
@@ -336,26 +384,24 @@ __device__ uint2 compress_alpha(const uint32_t blk[16]) {
 #pragma unroll
   for (int i = 1; i < 16; i++) {
     const int a = (int)(blk[i] >> 24);
-    if (a < mn) mn = a;
-    else if (a > mx) mx = a;
+    mn = min(mn, a);  // stb_dxt's if / else-if, with mn <= mx throughout
+    mx = max(mx, a);
   }
   const int dist = mx - mn;
-  const int dist4 = dist * 4, dist2 = dist * 2;
   int bias = dist < 8 ? dist - 1 : dist / 2 + 2;
   bias -= mn * 7;
+  // stb_dxt finds the index by binary long division of a = 7 v + bias, which
+  // lies in [0, 8 dist) when dist > 0: q = floor(a / dist) = (a * m) >> 19
+  // with m = ceil(2^19 / dist), exact for a < 2^11.  When dist = 0, a = -1
+  // and stb_dxt's steps give q = 0, as m = 0 does.
+  const uint32_t m = dist ? ((1u << 19) + (uint32_t)dist - 1u) / (uint32_t)dist : 0u;
   uint64_t bits = 0;
 #pragma unroll
   for (int i = 0; i < 16; i++) {
-    int a = (int)(blk[i] >> 24) * 7 + bias;
-    int t = a >= dist4 ? -1 : 0;
-    int ind = t & 4;
-    a -= dist4 & t;
-    t = a >= dist2 ? -1 : 0;
-    ind += t & 2;
-    a -= dist2 & t;
-    ind += (a >= dist);
-    ind = -ind & 7;
-    ind ^= (2 > ind);
+    const uint32_t a = (uint32_t)((int)(blk[i] >> 24) * 7 + bias) & 0x7FFu;
+    const uint32_t q = (a * m) >> 19;
+    uint32_t ind = (0u - q) & 7u;
+    ind ^= (uint32_t)(ind < 2u);
     bits |= (uint64_t)ind << (3 * i);
   }
   const uint64_t word = (uint64_t)(uint32_t)mx | ((uint64_t)(uint32_t)mn << 8) | (bits << 16);
@@ -364,6 +410,8 @@ __device__ uint2 compress_alpha(const uint32_t blk[16]) {
 
 template <int DXT5>
 __global__ void __launch_bounds__(256) encode_kernel(BlockRange r) {
+  __shared__ uint8_t s_q[2 * 784];
+  load_quant(s_q);
   const uint32_t rel = blockIdx.x * blockDim.x + threadIdx.x;
   if (rel >= r.num_blocks) return;
   const uint32_t b = r.first_block + rel;
@@ -378,7 +426,7 @@ __global__ void __launch_bounds__(256) encode_kernel(BlockRange r) {
     blk[y * 4 + 2] = row.z;
     blk[y * 4 + 3] = row.w;
   }
-  const uint2 c = compress_color(blk);
+  const uint2 c = compress_color(blk, s_q);
   if (DXT5) {
     const uint2 a = compress_alpha(blk);
     reinterpret_cast<uint4 *>(r.out)[b] = make_uint4(a.x, a.y, c.x, c.y);
@@ -393,6 +441,8 @@ template <int DXT5>
 __global__ void batch_kernel(const SurfDesc *surfs, const uint32_t *block_surf, uint32_t total,
                              uint32_t *counter) {
   __shared__ uint32_t s_chunk;
+  __shared__ uint8_t s_q[2 * 784];
+  load_quant(s_q);
   for (;;) {
     if (threadIdx.x == 0) s_chunk = atomicAdd(counter, blockDim.x);
     __syncthreads();
@@ -414,7 +464,7 @@ __global__ void batch_kernel(const SurfDesc *surfs, const uint32_t *block_surf, 
       blk[y * 4 + 2] = row.z;
       blk[y * 4 + 3] = row.w;
     }
-    const uint2 c = compress_color(blk);
+    const uint2 c = compress_color(blk, s_q);
     if (DXT5) {
       const uint2 a = compress_alpha(blk);
       reinterpret_cast<uint4 *>(sd.out)[b] = make_uint4(a.x, a.y, c.x, c.y);
@@ -439,8 +489,8 @@ void build_dxt_tables(dxt::Tables &t) {
   };
   for (int i = 0; i < 32; i++) t.expand5[i] = (uint8_t)((i << 3) | (i >> 2));
   for (int i = 0; i < 64; i++) t.expand6[i] = (uint8_t)((i << 2) | (i >> 4));
-  for (int i = 0; i < 256 + 16; i++) {
-    const int v = i - 8 < 0 ? 0 : (i - 8 > 255 ? 255 : i - 8);
+  for (int i = 0; i < 784; i++) {
+    const int v = i - 264 < 0 ? 0 : (i - 264 > 255 ? 255 : i - 264);
     t.quant_rb[i] = t.expand5[mul8(v, 31)];
     t.quant_g[i] = t.expand6[mul8(v, 63)];
   }
