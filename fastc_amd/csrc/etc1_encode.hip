@@ -47,6 +47,7 @@ struct Opt {
   uint32_t px[8];
   int luma_sorted[8];
   int idx_sorted[8];
+  uint32_t px_sorted[8];  // px[idx_sorted[k]]
   float avg[3];
   int limit;
   int br, bg, bb;
@@ -89,6 +90,7 @@ __device__ void opt_init(Opt &o, const uint32_t sub[8], bool color4) {
       if (k == rank) {
         o.luma_sorted[k] = luma[i];
         o.idx_sorted[k] = i;
+        o.px_sorted[k] = sub[i];
       }
   }
 }
@@ -131,7 +133,8 @@ __device__ bool evaluate_fast(const Opt &o, int cr, int cg, int cb, Solution &be
     const int bi0 = c0r + c0g + c0b, bi1 = c1r + c1g + c1b, bi2 = c2r + c2g + c2b,
               bi3 = c3r + c3g + c3b;
     const int mid0 = bi0 + bi1, mid1 = bi1 + bi2, mid2 = bi2 + bi3;
-    uint64_t total = 0;
+    // a row's error is at most 8 * 3 * 255^2 < 2^21: 32-bit sums are exact
+    uint32_t total = 0;
     uint32_t sel = 0;
     if (lmax * 2 < mid0) {
       if (bi0 > lmax && (uint64_t)(bi0 - lmax) >= terr) continue;
@@ -139,7 +142,7 @@ __device__ bool evaluate_fast(const Opt &o, int cr, int cg, int cb, Solution &be
       for (int c = 0; c < 8; c++) {
         const int dr = chan(o.px[c], 0) - c0r, dg = chan(o.px[c], 1) - c0g,
                   db = chan(o.px[c], 2) - c0b;
-        total += (uint64_t)(dr * dr + dg * dg + db * db);
+        total += (uint32_t)(dr * dr + dg * dg + db * db);
       }
       sel = 0;
     } else if (lmin * 2 >= mid2) {
@@ -148,28 +151,27 @@ __device__ bool evaluate_fast(const Opt &o, int cr, int cg, int cb, Solution &be
       for (int c = 0; c < 8; c++) {
         const int dr = chan(o.px[c], 0) - c3r, dg = chan(o.px[c], 1) - c3g,
                   db = chan(o.px[c], 2) - c3b;
-        total += (uint64_t)(dr * dr + dg * dg + db * db);
+        total += (uint32_t)(dr * dr + dg * dg + db * db);
       }
       sel = 0xFFFFu;
     } else {
       // the reference walks cur up the midpoints as the sorted lumas rise;
       // the midpoints do not decrease (the row's offsets increase), so the
-      // walk ends at the number of midpoints at or below 2 y
+      // walk ends at the number of midpoints at or below 2 y: 3 if the last
+      // is, else 2 if the middle one is, ...
 #pragma unroll
       for (int c = 0; c < 8; c++) {
         const int y2x = o.luma_sorted[c] * 2;
-        const int cur = (y2x >= mid0) + (y2x >= mid1) + (y2x >= mid2);
+        const bool g0 = y2x >= mid0, g1 = y2x >= mid1, g2 = y2x >= mid2;
+        const uint32_t cur = g2 ? 3u : (g1 ? 2u : (g0 ? 1u : 0u));
         const int pi = o.idx_sorted[c];
-        uint32_t p = 0;
-#pragma unroll
-        for (int k = 0; k < 8; k++)
-          if (k == pi) p = o.px[k];
-        const int er = cur == 0 ? c0r : (cur == 1 ? c1r : (cur == 2 ? c2r : c3r));
-        const int eg = cur == 0 ? c0g : (cur == 1 ? c1g : (cur == 2 ? c2g : c3g));
-        const int eb = cur == 0 ? c0b : (cur == 1 ? c1b : (cur == 2 ? c2b : c3b));
+        const uint32_t p = o.px_sorted[c];
+        const int er = g2 ? c3r : (g1 ? c2r : (g0 ? c1r : c0r));
+        const int eg = g2 ? c3g : (g1 ? c2g : (g0 ? c1g : c0g));
+        const int eb = g2 ? c3b : (g1 ? c2b : (g0 ? c1b : c0b));
         const int dr = chan(p, 0) - er, dg = chan(p, 1) - eg, db = chan(p, 2) - eb;
-        total += (uint64_t)(dr * dr + dg * dg + db * db);
-        sel |= (uint32_t)cur << (2 * pi);
+        total += (uint32_t)(dr * dr + dg * dg + db * db);
+        sel |= cur << (2 * pi);
       }
     }
     if (total < terr) {
