@@ -2279,6 +2279,14 @@ __global__ void __launch_bounds__(256, 2) rank_select_kernel(BlockRange r, Bc7Pa
       sum = (m > -INFINITY ? sum * expf(m - mn) : 0.0f) + ps;
       m = mn;
     }
+    // The list changes only if a logit of this phase beats its kk-th entry
+    // (an equal one comes later in list order: its index is larger): when
+    // no row of the wave has one, as in most waves in the mode-2/3 phase, the
+    // merge below would rebuild the same list.
+    float kth = -INFINITY;
+#pragma unroll
+    for (int i = 0; i < 8; i++) kth = i == kk - 1 ? lv[i] : kth;
+    if (!__ballot(pm > kth)) continue;
     // top-kk of the running list and this phase's logits: the t-th entry is
     // the first in list order after entry t - 1
     const int jbase = 128 * p + 4 * seg;

@@ -18,12 +18,17 @@ namespace mlp {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-// float -> bfloat16 bits, round to nearest even (inf/nan kept).
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)(u >> 16);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+// float -> bfloat16 bits, round to nearest even (v_cvt_pk_bf16_f32): for
+// finite values and infinities the same bits as adding 0x7FFF plus the
+// kept part's lowest bit and truncating.
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+
+// b / 255.0f, correctly rounded, for a byte b: b / 255 is 0, 1 or not a
+// dyadic rational, so it lies further (relative 2^-41) from every float
+// rounding boundary than the double product (within 2^-52 of it), which
+// therefore rounds to the same float.
+__device__ __forceinline__ float byte_unit(uint32_t b) {
+  return (float)((double)b * (1.0 / 255.0));
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
@@ -54,8 +59,8 @@ __device__ __forceinline__ void rank_features(const BlockRange &r, uint32_t n, u
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int c = 0; c < 4; c += 2) {
-      const uint16_t f0 = real ? f2bf((float)((p4[i] >> (8 * c)) & 0xFFu) / 255.0f) : 0;
-      const uint16_t f1 = real ? f2bf((float)((p4[i] >> (8 * c + 8)) & 0xFFu) / 255.0f) : 0;
+      const uint16_t f0 = real ? f2bf(byte_unit((p4[i] >> (8 * c)) & 0xFFu)) : 0;
+      const uint16_t f1 = real ? f2bf(byte_unit((p4[i] >> (8 * c + 8)) & 0xFFu)) : 0;
       packed[2 * i + c / 2] = f0 | ((uint32_t)f1 << 16);
     }
   *reinterpret_cast<uint4 *>(xs + row * kXs + 16 * y) =
