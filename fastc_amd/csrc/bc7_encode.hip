@@ -109,6 +109,11 @@ __constant__ uint8_t kMode5Tab[256][2];
 // CompressClusters' mode order (Compressor.cpp:1757).
 __constant__ int kOrder[8] = {0, 2, 1, 3, 7, 4, 5, 6};
 __constant__ uint8_t kDxt6Tab[256][2][3];
+// 1 / n in double for the k-means means (index n = 1..16)
+__constant__ double kInvCount[17] = {0.0,        1.0,        1.0 / 2,  1.0 / 3,  1.0 / 4,
+                                     1.0 / 5,    1.0 / 6,    1.0 / 7,  1.0 / 8,  1.0 / 9,
+                                     1.0 / 10,   1.0 / 11,   1.0 / 12, 1.0 / 13, 1.0 / 14,
+                                     1.0 / 15,   1.0 / 16};
 
 __device__ __forceinline__ uint32_t chan(uint32_t p, int k) { return (p >> (8 * k)) & 0xFFu; }
 
@@ -603,7 +608,10 @@ __device__ __forceinline__ void principal_axis(const uint32_t px[16], uint32_t s
     for (int i = 0; i < 4; i++)
 #pragma unroll
       for (int j = 0; j <= i; j++) {
-        cov[i * 4 + j] = sum[e++] / 3.0f;
+        // x / 3.0f for a float x as a double product, correctly rounded:
+        // x / 3 is a float or, with 3 left in its denominator, at least
+        // 2^-27 (relative) from every float rounding boundary
+        cov[i * 4 + j] = (float)((double)sum[e++] * (1.0 / 3.0));
         cov[j * 4 + i] = cov[i * 4 + j];
       }
   }
@@ -800,8 +808,15 @@ __device__ __forceinline__ void kmeans_setup(const Fit &f, uint32_t q, const uin
     for (int j = 0; j < NB; j++) {
       const float cn = km[(j * 5 + 4) * 64];
       float s[4];
+      // sum / cn in float, correctly rounded, as a double product: the sum
+      // is an integer below 2^13 and cn one of 1..16, so sum / cn is either
+      // a float or, in lowest terms p / q with q odd, at least 2^-29
+      // (relative) from every float rounding boundary, far beyond the
+      // product's 2^-52
+      const double inv = kInvCount[(int)cn];
 #pragma unroll
-      for (int k = 0; k < 4; k++) s[k] = cn != 0.0f ? km[(j * 5 + k) * 64] / cn : 0.0f;
+      for (int k = 0; k < 4; k++)
+        s[k] = cn != 0.0f ? (float)((double)km[(j * 5 + k) * 64] * inv) : 0.0f;
       if (s[0] != c[j][0] || s[1] != c[j][1] || s[2] != c[j][2] || s[3] != c[j][3])
         fixed = false;
 #pragma unroll
@@ -1027,7 +1042,9 @@ __device__ __forceinline__ uint32_t alpha_fit(const Fit &f, const uint32_t px[16
             s += avals[i];
             c += 1.0f;
           }
-        if (c > 0.0f) s /= c;
+        // integer sum over a count of 1..16: the double product is the
+        // correctly rounded quotient (as for the colour k-means)
+        if (c > 0.0f) s = (float)((double)s * kInvCount[(int)c]);
         npts[j] = c;
         if (j < nb) fixed = fixed && (s == vals[j]);
         vals[j] = s;
@@ -2493,8 +2510,10 @@ __device__ __forceinline__ uint32_t fit_candidate(const Cand &k, const uint32_t 
 #pragma unroll
         for (int kk = 0; kk < 4; kk++) avg[kk] += (float)chan(px[i], kk);
       }
+      // integer sums over n in 1..16: the double product is the correctly
+      // rounded quotient (as for the k-means means)
 #pragma unroll
-      for (int kk = 0; kk < 4; kk++) avg[kk] = avg[kk] / (float)n;
+      for (int kk = 0; kk < 4; kk++) avg[kk] = (float)((double)avg[kk] * kInvCount[n]);
       SubStart ss;
       subset_setup<NU>(f, quality, sctab, px, smask, 0, n, avg, same, ss, mu, mu, km);
       if constexpr (NU) {
