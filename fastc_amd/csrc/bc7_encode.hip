@@ -2192,6 +2192,22 @@ __device__ __forceinline__ bool top_before(float va, int ja, float vb, int jb) {
   return (va > vb) | ((va == vb) & (ja < jb));
 }
 
+// The same order as one unsigned 64-bit key: the logit's bits made
+// monotone (-0 counted as +0, as the float comparisons do) above the
+// complemented index, so a larger key comes first.
+__device__ __forceinline__ uint64_t top_key(float v, int j) {
+  const uint32_t u = __float_as_uint(v + 0.0f);
+  const uint32_t o = u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
+  return ((uint64_t)o << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)j);
+}
+__device__ __forceinline__ float key_value(uint64_t k) {
+  const uint32_t o = (uint32_t)(k >> 32);
+  return __uint_as_float(o ^ ((uint32_t)((int32_t)~o >> 31) | 0x80000000u));
+}
+__device__ __forceinline__ int key_index(uint64_t k) {
+  return (int)(0xFFFFFFFFu - (uint32_t)k);
+}
+
 // Logits per mode that name a real candidate (logit_real).
 __device__ __forceinline__ int real_logits(int mode) {
   return mode == 0 ? 16 : (mode == 4 ? 8 : (mode == 5 ? 4 : (mode == 6 ? 1 : 64)));
@@ -2237,13 +2253,11 @@ __global__ void __launch_bounds__(256, 2) rank_select_kernel(BlockRange r, Bc7Pa
   const int kk = prm.topk < 1 ? 1 : (prm.topk > 8 ? 8 : (int)prm.topk);
   const float *b3 = params + mlp::kRankOffB3;
   float m = -INFINITY, sum = 0.0f;
-  float lv[8];
-  int lj[8];
+  // the list as keys; an empty entry is (-inf, 2^30)
+  const uint64_t kFill = top_key(-INFINITY, 1 << 30);
+  uint64_t lk[8];
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    lv[i] = -INFINITY;
-    lj[i] = 1 << 30;
-  }
+  for (int i = 0; i < 8; i++) lk[i] = kFill;
   // the output layer in four phases of 128 logits (modes 2p and 2p + 1),
   // 32 per wave; each phase is staged in LDS and folded into the row state
 #pragma unroll 1
@@ -2300,63 +2314,55 @@ __global__ void __launch_bounds__(256, 2) rank_select_kernel(BlockRange r, Bc7Pa
     // (an equal one comes later in list order: its index is larger): when
     // no row of the wave has one, as in most waves in the mode-2/3 phase, the
     // merge below would rebuild the same list.
-    float kth = -INFINITY;
+    uint64_t kthk = kFill;
 #pragma unroll
-    for (int i = 0; i < 8; i++) kth = i == kk - 1 ? lv[i] : kth;
-    if (!__ballot(pm > kth)) continue;
-    // top-kk of the running list and this phase's logits: the t-th entry is
-    // the first in list order after entry t - 1
+    for (int i = 0; i < 8; i++) kthk = i == kk - 1 ? lk[i] : kthk;
+    if (!__ballot(pm > key_value(kthk))) continue;
+    // this phase's logits as keys (index 128 p + 16 i + 4 seg + e); a masked
+    // one gets key 0, below every real key and the empty entry, so it is
+    // never taken
     const int jbase = 128 * p + 4 * seg;
-    float nv[8];
-    int nj[8];
+    uint64_t zk[32];
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      nv[i] = -INFINITY;
-      nj[i] = 1 << 30;
-    }
-    float pv = INFINITY;
-    int pj = -1;
+    for (int k = 0; k < 32; k++)
+      zk[k] = z[k] > -INFINITY ? top_key(z[k], jbase + 16 * (k >> 2) + (k & 3)) : 0ull;
+    // top-kk of the running list and this phase's logits: the t-th entry is
+    // the largest key below entry t - 1's
+    uint64_t nk[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) nk[i] = kFill;
+    uint64_t pk = ~0ull;
 #pragma unroll 1
     for (int t = 0; t < kk; t++) {
-      float bv = -INFINITY;
-      int bj = 1 << 30;
-      // j rises with k, so among equal logits the first seen stays
+      uint64_t bk = kFill;
 #pragma unroll
       for (int k = 0; k < 32; k++) {
-        const int j = jbase + 16 * (k >> 2) + (k & 3);
-        const bool take = top_before(pv, pj, z[k], j) & (z[k] > bv);
-        bv = take ? z[k] : bv;
-        bj = take ? j : bj;
+        const bool take = (zk[k] < pk) & (zk[k] > bk);
+        bk = take ? zk[k] : bk;
       }
 #pragma unroll
       for (int i = 0; i < 8; i++) {
-        const bool after = top_before(pv, pj, lv[i], lj[i]);
-        const bool better = top_before(lv[i], lj[i], bv, bj);
-        const bool take = (seg == 0) & after & better;
-        bv = take ? lv[i] : bv;
-        bj = take ? lj[i] : bj;
+        const bool take = (seg == 0) & (lk[i] < pk) & (lk[i] > bk);
+        bk = take ? lk[i] : bk;
       }
 #pragma unroll
       for (int off = 1; off < 4; off <<= 1) {
-        const float ov = __shfl_xor(bv, off);
-        const int oj = __shfl_xor(bj, off);
-        const bool take = top_before(ov, oj, bv, bj);
-        bv = take ? ov : bv;
-        bj = take ? oj : bj;
+        const uint64_t ok = __shfl_xor(bk, off);
+        bk = ok > bk ? ok : bk;
       }
 #pragma unroll
-      for (int i = 0; i < 8; i++) {
-        nv[i] = i == t ? bv : nv[i];
-        nj[i] = i == t ? bj : nj[i];
-      }
-      pv = bv;
-      pj = bj;
+      for (int i = 0; i < 8; i++) nk[i] = i == t ? bk : nk[i];
+      pk = bk;
     }
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      lv[i] = nv[i];
-      lj[i] = nj[i];
-    }
+    for (int i = 0; i < 8; i++) lk[i] = nk[i];
+  }
+  float lv[8];
+  int lj[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    lv[i] = key_value(lk[i]);
+    lj[i] = key_index(lk[i]);
   }
 
   // solid and transparent blocks, from the row's four texel rows
