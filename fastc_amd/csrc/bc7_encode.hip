@@ -2301,13 +2301,17 @@ __global__ void __launch_bounds__(256, 2) rank_select_kernel(BlockRange r, Bc7Pa
     pm = fmaxf(pm, __shfl_xor(pm, 2));
     const float mn = fmaxf(m, pm);
     const float shift = mn > -INFINITY ? mn : 0.0f;
+    // the hardware exponential (v_exp_f32 of x log2 e, relative error about
+    // 2^-21): the top probability 1 / sum is a threshold test, and its
+    // rounding (like that of the summation order) only matters for blocks
+    // within about 1e-6 of the threshold
     float ps = 0.0f;
 #pragma unroll
-    for (int k = 0; k < 32; k++) ps += expf(z[k] - shift);
+    for (int k = 0; k < 32; k++) ps += __expf(z[k] - shift);
     ps += __shfl_xor(ps, 1);
     ps += __shfl_xor(ps, 2);
     if (mn > -INFINITY) {  // the same in the row's four threads
-      sum = (m > -INFINITY ? sum * expf(m - mn) : 0.0f) + ps;
+      sum = (m > -INFINITY ? sum * __expf(m - mn) : 0.0f) + ps;
       m = mn;
     }
     // The list changes only if a logit of this phase beats its kk-th entry
