@@ -2779,7 +2779,7 @@ __device__ __forceinline__ void score_pixels_nu(const Lane &L, const SubConst &s
   }
 }
 
-// The same pass split over a group of G lanes (G = 4, 8 or 16, aligned in
+// The same pass split over a group of G lanes (G = 2, 4, 8 or 16, aligned in
 // the wave) that hold the same candidate: member m scores pixels m, m + G,
 // ...; the sums and index bits are combined over the group.  The per-subset
 // errors are integers, so the split does not change them.
@@ -2974,9 +2974,18 @@ __device__ void lane_finish(Lane &L, Work &w) {
 // finishes them in lane groups.
 constexpr int kTailLanes = 16;
 // Lanes per candidate for the rotation modes (4 and 5), whose annealing
-// chains are the longest (hundreds to a few thousand steps against tens for
+// chains are the longest (hundreds to over a thousand steps against tens for
 // the other modes): run one lane each, they would set the launch's length.
-constexpr int kRotGroup = 4;
+// Pairs split the pixel pass in two; every member repeats the rest of the
+// step, so larger groups shorten a step less than they cost (measured: 4
+// and 8 lanes slower overall).
+constexpr int kRotGroup = 2;
+// The wave mask of each group's first lane (lanes 0, g, 2g, ...).
+constexpr uint64_t group_leaders(int g, int lane = 0) {
+  return lane >= 64 ? 0ull : (1ull << lane) | group_leaders(g, lane + g);
+}
+static_assert(group_leaders(2) == 0x5555555555555555ull && group_leaders(4) == 0x1111111111111111ull,
+              "group leaders");
 
 // UNROLL annealing steps run between refills of finished lanes.
 // WAVES: the minimum waves per SIMD the register allocation must allow.  3
@@ -3039,19 +3048,17 @@ anneal_kernel(BlockRange r, Bc7Params prm, Work w) {
     rot_begin = lo;
   }
   // Phase A, in the first eighth of the launch (a quarter of the waves that
-  // fit at once): the rotation states in
-  // groups of kRotGroup lanes that share each step's pixel pass (every
-  // member holds the same state and texels), fetched from the end of the
-  // suffix.  A group that finishes takes the next; once the suffix is used
-  // up the wave's groups finish theirs and the wave joins phase B.  The
-  // other waves start on phase B at once, so the longest chains run from
-  // the start at a quarter of a lane's instructions per step while the rest
-  // of the list proceeds beside them.
+  // fit at once): the rotation states in groups of kRotGroup lanes that
+  // share each step's pixel pass (every member holds the same state and
+  // texels), fetched from the end of the suffix.  A group that finishes
+  // takes the next; once the suffix is used up the wave's groups finish
+  // theirs and the wave joins phase B.  The other waves start on phase B at
+  // once, so the longest chains run from the start, with shorter steps,
+  // while the rest of the list proceeds beside them.
   if (!NU && rot_begin < listed && blockIdx.x < (gridDim.x + 7) / 8) {
     const uint32_t nrot = listed - rot_begin;
     const int m = lane % kRotGroup;
-    static_assert(kRotGroup == 4, "the group leaders' mask");
-    const uint64_t leaders = 0x1111111111111111ull;
+    constexpr uint64_t leaders = group_leaders(kRotGroup);
     Lane T;
     T.px = s_px + lane * 33;
     T.pt = T.px + 16;
