@@ -152,10 +152,10 @@ __device__ __forceinline__ uint32_t point_of(uint32_t pix, int ptmode) {
 }
 
 // QuantizeChannel (RGBAEndpoints.cpp:126-165).
-__device__ __forceinline__ uint32_t quant_channel(uint32_t val, uint32_t mask, int pbit) {
+__host__ __device__ __forceinline__ uint32_t quant_channel(uint32_t val, uint32_t mask, int pbit) {
   if (mask == 0xFFu) return val;
   if (mask == 0u) return 0xFFu;
-  uint32_t prec = __popc(mask);
+  uint32_t prec = (uint32_t)__builtin_popcount(mask);
   const uint32_t step = 1u << (8 - prec);
   uint32_t lval = val & mask;
   uint32_t hval = lval + step;
@@ -174,6 +174,26 @@ __device__ __forceinline__ uint32_t quant_channel(uint32_t val, uint32_t mask, i
   const uint32_t dl = val > l8 ? val - l8 : l8 - val;
   const uint32_t dh = val > h8 ? val - h8 : h8 - val;
   return dl < dh ? l8 : h8;
+}
+
+// quant_channel of every byte for each channel precision (the number of
+// bits of the channel's mask, which keeps the top bits: 0..8) and p-bit
+// (-1, 0, 1), at [(bits * 3 + pbit + 1) * 256 + byte]; built on the host
+// from quant_channel itself.
+__constant__ uint8_t kQuant[9 * 3 * 256];
+
+// to_pixel_int through kQuant: a load per channel in place of the
+// comparisons and shifts (the annealing step quantises both endpoints of
+// every subset).
+__device__ __forceinline__ uint32_t to_pixel_tab(uint32_t p, uint32_t mask, int pbit) {
+  const uint32_t pb = (uint32_t)(pbit + 1) * 256u;
+  uint32_t r = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t bits = (uint32_t)__builtin_popcount(chan(mask, k));
+    r |= (uint32_t)kQuant[bits * 768u + pb + chan(p, k)] << (8 * k);
+  }
+  return r;
 }
 
 // ToPixel on integer-valued endpoints (uint32(c + 0.5) is then c).
@@ -2832,8 +2852,8 @@ __device__ bool lane_step(Lane &L, uint32_t q, int G = 1, int m = 0) {
     pick_neighbor(f, rng, L.p1[s], L.p2[s], L.cpbc[s], np1[s], np2[s], npbc[s], L.bp1[s],
                   L.bp2[s], L.bpbc[s]);
     L.ctr[s] = rng.ctr;
-    e1[s] = to_pixel_int(np1[s], f.qmask, pbc_bit(f.pbit_type, npbc[s], 0));
-    e2[s] = to_pixel_int(np2[s], f.qmask, pbc_bit(f.pbit_type, npbc[s], 1));
+    e1[s] = to_pixel_tab(np1[s], f.qmask, pbc_bit(f.pbit_type, npbc[s], 0));
+    e2[s] = to_pixel_tab(np2[s], f.qmask, pbc_bit(f.pbit_type, npbc[s], 1));
   }
   const SubConst sc0 = sub_const(e1[0], e2[0]);
   const SubConst sc1 = sub_const(e1[1], e2[1]);
@@ -3767,6 +3787,15 @@ hipError_t bc7_upload_tables(int device) {
   build_bc7_mode5_table(m5);
   build_bc7_dxt6_table(d6);
   hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(bc7::kMode5Tab), m5, sizeof(m5));
+  if (e != hipSuccess) return e;
+  static uint8_t quant[9 * 3 * 256];
+  for (int bits = 0; bits <= 8; bits++) {
+    const uint32_t mask = bits == 0 ? 0u : ((0xFFu << (8 - bits)) & 0xFFu);
+    for (int pb = -1; pb <= 1; pb++)
+      for (uint32_t v = 0; v < 256; v++)
+        quant[(bits * 3 + pb + 1) * 256 + v] = (uint8_t)bc7::quant_channel(v, mask, pb);
+  }
+  e = hipMemcpyToSymbol(HIP_SYMBOL(bc7::kQuant), quant, sizeof(quant));
   if (e != hipSuccess) return e;
   e = hipMemcpyToSymbol(HIP_SYMBOL(bc7::kDxt6Tab), d6, sizeof(d6));
   if (e != hipSuccess) return e;
