@@ -186,14 +186,12 @@ __constant__ uint8_t kQuant[9 * 3 * 256];
 // comparisons and shifts (the annealing step quantises both endpoints of
 // every subset).
 __device__ __forceinline__ uint32_t to_pixel_tab(uint32_t p, uint32_t mask, int pbit) {
+  // the colour channels share one precision (qmask_of), alpha has its own
   const uint32_t pb = (uint32_t)(pbit + 1) * 256u;
-  uint32_t r = 0;
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const uint32_t bits = (uint32_t)__builtin_popcount(chan(mask, k));
-    r |= (uint32_t)kQuant[bits * 768u + pb + chan(p, k)] << (8 * k);
-  }
-  return r;
+  const uint32_t cb = (uint32_t)__builtin_popcount(mask & 0xFFu) * 768u + pb;
+  const uint32_t ab = (uint32_t)__builtin_popcount(mask >> 24) * 768u + pb;
+  return (uint32_t)kQuant[cb + chan(p, 0)] | ((uint32_t)kQuant[cb + chan(p, 1)] << 8) |
+         ((uint32_t)kQuant[cb + chan(p, 2)] << 16) | ((uint32_t)kQuant[ab + chan(p, 3)] << 24);
 }
 
 // ToPixel on integer-valued endpoints (uint32(c + 0.5) is then c).
