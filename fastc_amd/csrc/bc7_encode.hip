@@ -182,6 +182,11 @@ __host__ __device__ __forceinline__ uint32_t quant_channel(uint32_t val, uint32_
 // from quant_channel itself.
 __constant__ uint8_t kQuant[9 * 3 * 256];
 
+// quant_channel(val, the channel mask byte, pbit) for a top-aligned mask
+__device__ __forceinline__ uint32_t quant_tab(uint32_t val, uint32_t mask_byte, int pbit) {
+  return kQuant[(uint32_t)__builtin_popcount(mask_byte) * 768u + (uint32_t)(pbit + 1) * 256u + val];
+}
+
 // to_pixel_int through kQuant: a load per channel in place of the
 // comparisons and shifts (the annealing step quantises both endpoints of
 // every subset).
@@ -198,7 +203,7 @@ __device__ __forceinline__ uint32_t to_pixel_tab(uint32_t p, uint32_t mask, int 
 __device__ __forceinline__ uint32_t to_pixel_int(uint32_t p, uint32_t mask, int pbit) {
   uint32_t r = 0;
 #pragma unroll
-  for (int k = 0; k < 4; k++) r |= quant_channel(chan(p, k), chan(mask, k), pbit) << (8 * k);
+  for (int k = 0; k < 4; k++) r |= quant_tab(chan(p, k), chan(mask, k), pbit) << (8 * k);
   return r;
 }
 
@@ -209,11 +214,12 @@ __device__ __forceinline__ uint32_t round_half_up(float c) {
   return (uint32_t)f + ((c - f) >= 0.5f ? 1u : 0u);
 }
 
+
 __device__ __forceinline__ uint32_t to_pixel_f(const float p[4], uint32_t mask, int pbit) {
   uint32_t r = 0;
 #pragma unroll
   for (int k = 0; k < 4; k++)
-    r |= quant_channel(round_half_up(p[k]) & 0xFFu, chan(mask, k), pbit) << (8 * k);
+    r |= quant_tab(round_half_up(p[k]) & 0xFFu, chan(mask, k), pbit) << (8 * k);
   return r;
 }
 
