@@ -3013,9 +3013,9 @@ static_assert(group_leaders(2) == 0x5555555555555555ull && group_leaders(4) == 0
 
 // UNROLL annealing steps run between refills of finished lanes.
 // WAVES: the minimum waves per SIMD the register allocation must allow.  3
-// (168 VGPRs, a small spill) suits the long lists of the full search; the
-// learned search's short, filtered list runs faster with the uncapped
-// allocation (WAVES 1, two waves fit).
+// (168 VGPRs, a small spill) suits both searches since the annealing step's
+// quantisation became table loads (before, the learned search ran alike at 2
+// and 3 waves per SIMD: 7.66 and 7.70 ms; now 6.08 and 5.78 ms).
 // NU: the nonuniform metric (float errors; the tail below, which splits a
 // candidate's pixel sums over lanes, would change their order and is not
 // used: the lanes finish their own candidates).
@@ -3825,18 +3825,14 @@ hipError_t bc7_upload_tables(int device) {
 
 void launch_anneal(uint32_t grid, const BlockRange &r, const Bc7Params &p, const bc7::Work &w,
                    hipStream_t stream) {
-  if (p.work_counts && !w.nu_total) {  // the counting instances
-    if (w.alist)
-      hipLaunchKernelGGL((bc7::anneal_kernel<4, 1, false, true>), dim3(grid), dim3(64), 0,
-                         stream, r, p, w);
-    else
-      hipLaunchKernelGGL((bc7::anneal_kernel<4, 3, false, true>), dim3(grid), dim3(64), 0,
-                         stream, r, p, w);
-  } else if (w.nu_total)  // nonuniform metric (the full search or given selections)
+  if (p.work_counts && !w.nu_total)  // the counting instance (both searches)
+    hipLaunchKernelGGL((bc7::anneal_kernel<4, 3, false, true>), dim3(grid), dim3(64), 0, stream,
+                       r, p, w);
+  else if (w.nu_total)  // nonuniform metric (the full search or given selections)
     hipLaunchKernelGGL((bc7::anneal_kernel<4, 1, true>), dim3(grid), dim3(64), 0, stream, r, p,
                        w);
   else if (w.alist)  // learned search: four steps between refills
-    hipLaunchKernelGGL((bc7::anneal_kernel<4, 1>), dim3(grid), dim3(64), 0, stream, r, p, w);
+    hipLaunchKernelGGL((bc7::anneal_kernel<4, 3>), dim3(grid), dim3(64), 0, stream, r, p, w);
   else if (p.anneal_unroll >= 4)
     hipLaunchKernelGGL((bc7::anneal_kernel<4, 3>), dim3(grid), dim3(64), 0, stream, r, p, w);
   else if (p.anneal_unroll == 2)
