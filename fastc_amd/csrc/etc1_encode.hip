@@ -35,6 +35,22 @@ struct Tables {
 __device__ __forceinline__ int clamp255(int v) { return v < 0 ? 0 : (v > 255 ? 255 : v); }
 __device__ __forceinline__ int chan(uint32_t p, int k) { return (int)((p >> (8 * k)) & 0xFFu); }
 
+typedef short s16x2_t __attribute__((ext_vector_type(2)));
+
+// Red and green of texel p as two 16-bit lanes.
+__device__ __forceinline__ uint32_t rg_of(uint32_t p) {
+  return __builtin_amdgcn_perm(0u, p, 0x0c010c00u);
+}
+
+// Squared RGB distance of texel p to the colour (rg: red and green lanes,
+// b): the red/green differences as 16-bit lanes in one two-way dot product
+// (all terms below 2^17, so the sums are exact).
+__device__ __forceinline__ uint32_t dist2(uint32_t p, uint32_t rg, int b) {
+  const s16x2_t d = __builtin_bit_cast(s16x2_t, rg_of(p)) - __builtin_bit_cast(s16x2_t, rg);
+  const int db = chan(p, 2) - b;
+  return (uint32_t)__builtin_amdgcn_sdot2(d, d, db * db, false);
+}
+
 struct Solution {
   int r, g, b;      // unscaled block colour
   int inten;
@@ -133,26 +149,22 @@ __device__ bool evaluate_fast(const Opt &o, int cr, int cg, int cb, Solution &be
     const int bi0 = c0r + c0g + c0b, bi1 = c1r + c1g + c1b, bi2 = c2r + c2g + c2b,
               bi3 = c3r + c3g + c3b;
     const int mid0 = bi0 + bi1, mid1 = bi1 + bi2, mid2 = bi2 + bi3;
+    const uint32_t c0rg = (uint32_t)c0r | ((uint32_t)c0g << 16);
+    const uint32_t c1rg = (uint32_t)c1r | ((uint32_t)c1g << 16);
+    const uint32_t c2rg = (uint32_t)c2r | ((uint32_t)c2g << 16);
+    const uint32_t c3rg = (uint32_t)c3r | ((uint32_t)c3g << 16);
     // a row's error is at most 8 * 3 * 255^2 < 2^21: 32-bit sums are exact
     uint32_t total = 0;
     uint32_t sel = 0;
     if (lmax * 2 < mid0) {
       if (bi0 > lmax && (uint64_t)(bi0 - lmax) >= terr) continue;
 #pragma unroll
-      for (int c = 0; c < 8; c++) {
-        const int dr = chan(o.px[c], 0) - c0r, dg = chan(o.px[c], 1) - c0g,
-                  db = chan(o.px[c], 2) - c0b;
-        total += (uint32_t)(dr * dr + dg * dg + db * db);
-      }
+      for (int c = 0; c < 8; c++) total += dist2(o.px[c], c0rg, c0b);
       sel = 0;
     } else if (lmin * 2 >= mid2) {
       if (lmin > bi3 && (uint64_t)(lmin - bi3) >= terr) continue;
 #pragma unroll
-      for (int c = 0; c < 8; c++) {
-        const int dr = chan(o.px[c], 0) - c3r, dg = chan(o.px[c], 1) - c3g,
-                  db = chan(o.px[c], 2) - c3b;
-        total += (uint32_t)(dr * dr + dg * dg + db * db);
-      }
+      for (int c = 0; c < 8; c++) total += dist2(o.px[c], c3rg, c3b);
       sel = 0xFFFFu;
     } else {
       // the reference walks cur up the midpoints as the sorted lumas rise;
@@ -166,11 +178,9 @@ __device__ bool evaluate_fast(const Opt &o, int cr, int cg, int cb, Solution &be
         const uint32_t cur = g2 ? 3u : (g1 ? 2u : (g0 ? 1u : 0u));
         const int pi = o.idx_sorted[c];
         const uint32_t p = o.px_sorted[c];
-        const int er = g2 ? c3r : (g1 ? c2r : (g0 ? c1r : c0r));
-        const int eg = g2 ? c3g : (g1 ? c2g : (g0 ? c1g : c0g));
+        const uint32_t erg = g2 ? c3rg : (g1 ? c2rg : (g0 ? c1rg : c0rg));
         const int eb = g2 ? c3b : (g1 ? c2b : (g0 ? c1b : c0b));
-        const int dr = chan(p, 0) - er, dg = chan(p, 1) - eg, db = chan(p, 2) - eb;
-        total += (uint32_t)(dr * dr + dg * dg + db * db);
+        total += dist2(p, erg, eb);
         sel |= cur << (2 * pi);
       }
     }
