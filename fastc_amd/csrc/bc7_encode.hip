@@ -1489,8 +1489,8 @@ __host__ __device__ __forceinline__ uint32_t weight_of(WeightFn fn, uint32_t j) 
 struct SubConst {
   uint32_t a02, a13;  // endpoint 1, channel pairs
   uint32_t d02, d13;  // endpoint 2 - endpoint 1 (signed pairs)
-  int l;              // squared endpoint distance (0: endpoints equal)
-  float rl;           // approximately 1 / l
+  int l;              // squared endpoint distance, 1 when the endpoints are equal
+  float rl;           // 1 / l from below (see pixel_bounds)
 };
 
 __device__ __forceinline__ SubConst sub_const(uint32_t e1, uint32_t e2) {
@@ -1504,8 +1504,11 @@ __device__ __forceinline__ SubConst sub_const(uint32_t e1, uint32_t e2) {
                                        __builtin_amdgcn_sdot2(as_s16x2(c.d13),
                                                               as_s16x2(c.d13), 0, false),
                                        false);
-  c.l = l;
-  c.rl = __builtin_amdgcn_rcpf((float)(l > 0 ? l : 1));
+  // equal endpoints give d = 0 for every pixel, so l = 1 yields index 0
+  c.l = l > 0 ? l : 1;
+  // v_rcp_f32 is within 1 ulp (2^-23); scaled by 1 - 2^-20 the product
+  // num * rl stays strictly below num / l for num > 0
+  c.rl = __builtin_amdgcn_rcpf((float)c.l) * (1.0f - 0x1p-20f);
   return c;
 }
 
@@ -1538,7 +1541,11 @@ __device__ __forceinline__ uint32_t interp_err(uint32_t p02, uint32_t p13, const
 // |t| <= 16 (larger |t| is clamped away), and fl(fl(k / kf) kf) = k for every
 // integer 0 <= k <= kf in {3, 7, 15} (checked exhaustively).  So the quotient
 // is taken in integers: an estimate from a reciprocal, corrected by one step
-// with the exact remainder.
+// with the exact remainder.  The estimate comes from below (rl < 1 / l with
+// a margin larger than the float errors, and num / l <= 15 * 510 / sqrt(l)
+// keeps its shortfall under 1), so for num >= 0 it is the quotient or one
+// less, and one upward step fixes it.  For num < 0 the index is 0 whatever
+// the estimate: j1 clamps to 0 and j2 <= j1.
 __device__ __forceinline__ void pixel_bounds(uint32_t pt, const SubConst &c, int nb, int &j1,
                                              int &j2) {
   const uint32_t t02 = pt & 0x00FF00FFu, t13 = (pt >> 8) & 0x00FF00FFu;
@@ -1546,18 +1553,15 @@ __device__ __forceinline__ void pixel_bounds(uint32_t pt, const SubConst &c, int
       as_s16x2(t02) - as_s16x2(c.a02), as_s16x2(c.d02),
       __builtin_amdgcn_sdot2(as_s16x2(t13) - as_s16x2(c.a13), as_s16x2(c.d13), 0, false),
       false);
-  const bool zero = c.l == 0;
   const int num = __mul24(d, nb - 1);  // |d| <= 4 * 255^2, so 24-bit operands suffice
   int q = (int)floorf((float)num * c.rl);
   int rem = num - __mul24(q, c.l);
-  const bool lo = rem < 0, hi = rem >= c.l;
-  q += lo ? -1 : (hi ? 1 : 0);
-  rem += lo ? c.l : (hi ? -c.l : 0);
+  const bool hi = rem >= c.l;
+  q += hi ? 1 : 0;
+  rem -= hi ? c.l : 0;
   j1 = q < 0 ? 0 : (q > nb - 1 ? nb - 1 : q);
   j2 = q + (rem != 0 ? 1 : 0);
   j2 = j2 > nb - 1 ? nb - 1 : j2;
-  j1 = zero ? 0 : j1;
-  j2 = zero ? 0 : j2;
 }
 
 __device__ __forceinline__ uint32_t pixel_err(uint32_t pix, uint32_t pt, const SubConst &c,
