@@ -1484,22 +1484,29 @@ __host__ __device__ __forceinline__ uint32_t weight_of(WeightFn fn, uint32_t j) 
 }
 
 // Per-subset constants of one annealing step.  The interpolated value is
-// (e1 * (64 - w) + e2 * w + 32) >> 6 = e1 + ((e2 - e1) * w + 32) >> 6 (the
-// 64 * e1 term is a multiple of 64), so endpoint 1 and the difference suffice.
+// (e1 * (64 - w) + e2 * w + 32) >> 6 = (64 e1 + 32 + (e2 - e1) * w) >> 6, so
+// 64 e1 + 32 and the difference suffice.
 struct SubConst {
-  uint32_t a02, a13;  // endpoint 1, channel pairs
+  uint32_t a02, a13;  // 64 * endpoint 1 + 32, channel pairs (see interp_err)
   uint32_t d02, d13;  // endpoint 2 - endpoint 1 (signed pairs)
+  int ad;             // endpoint 1 . (endpoint 2 - endpoint 1)
   int l;              // squared endpoint distance, 1 when the endpoints are equal
   float rl;           // 1 / l from below (see pixel_bounds)
 };
 
 __device__ __forceinline__ SubConst sub_const(uint32_t e1, uint32_t e2) {
   SubConst c;
-  c.a02 = e1 & 0x00FF00FFu;
-  c.a13 = (e1 >> 8) & 0x00FF00FFu;
+  const uint32_t a02 = e1 & 0x00FF00FFu, a13 = (e1 >> 8) & 0x00FF00FFu;
   const uint32_t b02 = e2 & 0x00FF00FFu, b13 = (e2 >> 8) & 0x00FF00FFu;
-  c.d02 = __builtin_bit_cast(uint32_t, as_s16x2(b02) - as_s16x2(c.a02));
-  c.d13 = __builtin_bit_cast(uint32_t, as_s16x2(b13) - as_s16x2(c.a13));
+  c.d02 = __builtin_bit_cast(uint32_t, as_s16x2(b02) - as_s16x2(a02));
+  c.d13 = __builtin_bit_cast(uint32_t, as_s16x2(b13) - as_s16x2(a13));
+  // each 16-bit lane: 64 a + 32 <= 16352
+  c.a02 = (a02 << 6) + 0x00200020u;
+  c.a13 = (a13 << 6) + 0x00200020u;
+  // endpoint 1's projection on the axis, subtracted from each pixel's
+  c.ad = __builtin_amdgcn_sdot2(as_s16x2(a02), as_s16x2(c.d02),
+                                __builtin_amdgcn_sdot2(as_s16x2(a13), as_s16x2(c.d13), 0, false),
+                                false);
   const int l = __builtin_amdgcn_sdot2(as_s16x2(c.d02), as_s16x2(c.d02),
                                        __builtin_amdgcn_sdot2(as_s16x2(c.d13),
                                                               as_s16x2(c.d13), 0, false),
@@ -1512,16 +1519,16 @@ __device__ __forceinline__ SubConst sub_const(uint32_t e1, uint32_t e2) {
   return c;
 }
 
-// Squared error of the pixel pairs against index weight w1: the products
-// (e2 - e1) * w1 lie within +-255 * 64 and the interpolants within 0..255,
-// so signed 16-bit lanes hold every intermediate exactly.
+// Squared error of the pixel pairs against index weight w1: each lane
+// computes (64 e1 + 32 + (e2 - e1) w1) >> 6, where (e2 - e1) w1 lies within
+// +-255 * 64 and the sum is 64 times an interpolant in 0..255 plus 32 (at
+// most 16352), so signed 16-bit lanes hold every intermediate exactly.
 __device__ __forceinline__ uint32_t interp_err(uint32_t p02, uint32_t p13, const SubConst &c,
                                                uint32_t w1) {
   const s16x2 W = {(short)w1, (short)w1};
-  const s16x2 K = {32, 32};
   const s16x2 S = {6, 6};
-  const s16x2 i02 = as_s16x2(c.a02) + ((as_s16x2(c.d02) * W + K) >> S);
-  const s16x2 i13 = as_s16x2(c.a13) + ((as_s16x2(c.d13) * W + K) >> S);
+  const s16x2 i02 = (as_s16x2(c.d02) * W + as_s16x2(c.a02)) >> S;
+  const s16x2 i13 = (as_s16x2(c.d13) * W + as_s16x2(c.a13)) >> S;
   const s16x2 e02 = as_s16x2(p02) - i02;
   const s16x2 e13 = as_s16x2(p13) - i13;
   return (uint32_t)__builtin_amdgcn_sdot2(e02, e02, __builtin_amdgcn_sdot2(e13, e13, 0, false),
@@ -1550,8 +1557,8 @@ __device__ __forceinline__ void pixel_bounds(uint32_t pt, const SubConst &c, int
                                              int &j2) {
   const uint32_t t02 = pt & 0x00FF00FFu, t13 = (pt >> 8) & 0x00FF00FFu;
   const int d = __builtin_amdgcn_sdot2(
-      as_s16x2(t02) - as_s16x2(c.a02), as_s16x2(c.d02),
-      __builtin_amdgcn_sdot2(as_s16x2(t13) - as_s16x2(c.a13), as_s16x2(c.d13), 0, false),
+      as_s16x2(t02), as_s16x2(c.d02),
+      __builtin_amdgcn_sdot2(as_s16x2(t13), as_s16x2(c.d13), -c.ad, false),
       false);
   const int num = __mul24(d, nb - 1);  // |d| <= 4 * 255^2, so 24-bit operands suffice
   int q = (int)floorf((float)num * c.rl);
@@ -1582,10 +1589,9 @@ __device__ __forceinline__ uint32_t pixel_err(uint32_t pix, uint32_t pt, const S
 __device__ __forceinline__ float interp_err_nu(uint32_t p02, uint32_t p13, const SubConst &c,
                                                uint32_t w1, const Weights &m) {
   const s16x2 W = {(short)w1, (short)w1};
-  const s16x2 K = {32, 32};
   const s16x2 S = {6, 6};
-  const s16x2 i02 = as_s16x2(c.a02) + ((as_s16x2(c.d02) * W + K) >> S);
-  const s16x2 i13 = as_s16x2(c.a13) + ((as_s16x2(c.d13) * W + K) >> S);
+  const s16x2 i02 = (as_s16x2(c.d02) * W + as_s16x2(c.a02)) >> S;
+  const s16x2 i13 = (as_s16x2(c.d13) * W + as_s16x2(c.a13)) >> S;
   const s16x2 e02 = as_s16x2(p02) - i02;
   const s16x2 e13 = as_s16x2(p13) - i13;
   return werr(e02.x, e13.x, e02.y, e13.y, m);
@@ -1610,6 +1616,7 @@ __device__ __forceinline__ SubConst pick_sub(bool c, const SubConst &x, const Su
   SubConst r;
   r.a02 = c ? x.a02 : y.a02;
   r.a13 = c ? x.a13 : y.a13;
+  r.ad = c ? x.ad : y.ad;
   r.d02 = c ? x.d02 : y.d02;
   r.d13 = c ? x.d13 : y.d13;
   r.l = c ? x.l : y.l;
